@@ -1,0 +1,517 @@
+// Golden-vector harness built directly against the reference PANSFEM2 headers.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is compiled by oracle/Makefile with
+// -I <reference>/src into oracle/_ref/ref_harness; nothing in the product
+// links it.  It composes the reference's own functions the way the reference
+// sample drivers do (sample/optimize/sample_optimize_density_oc.cpp:22-211,
+// sample/optimize/sample_optimize_density_mma.cpp:80-204,
+// sample/solid/sample_linear.cpp:19-58) with the parameters of the SIMP
+// workloads spelled out below, and writes raw little-endian arrays that
+// tests/golden/make_golden.py packs into fixtures.
+//
+// Modes
+//   simp <nx> <ny> <iters> <oc|mma> <outdir> [nsave]
+//       plane-strain SIMP compliance loop (DensityFilter R=1.5, E1=1,
+//       E0=1e-9, nu=0.3, t=1, p=3, vf=0.5, s0=0.5, left edge clamped, unit
+//       downward load at the bottom-right node, ScalingCG(100000, 1e-10)).
+//   operator <nx> <ny> <seed> <outdir>
+//       assembled K(rho) for rho iid U[1e-3,1] (SplitMix64), y = K x for x iid
+//       U[-1,1], and the ScalingCG solve of K u = F to 1e-10.
+//   solid <csvdir> <outdir>
+//       the hex8 linear-solid sample (Node/Element/Dirichlet/Neumann.csv).
+//   mmatest <outdir>
+//       the Svanberg cantilever driver of src/Optimize/Solver/test_MMA.cpp.
+//   time <nx> <ny> <cg_itrmax>
+//       one OC iteration with per-phase steady_clock timings (CPU baseline).
+#include <cstdio>
+#include <cstdint>
+#include <cstdlib>
+#include <cstring>
+#include <chrono>
+#include <fstream>
+#include <iostream>
+#include <iomanip>
+#include <sstream>
+#include <string>
+#include <vector>
+#include <cmath>
+
+// The CSR arrays (indptr/indices/data) are private members of the reference's
+// CSR<T>; the harness reads them to write the sparsity fixture, so access is
+// widened for these includes only.  No arithmetic is affected.
+#define private public
+#include "LinearAlgebra/Models/CSR.h"
+#include "LinearAlgebra/Models/LILCSR.h"
+#undef private
+#include "LinearAlgebra/Models/Vector.h"
+#include "FEM/Equation/PlaneStrain.h"
+#include "FEM/Equation/Solid.h"
+#include "FEM/Controller/ShapeFunction.h"
+#include "FEM/Controller/GaussIntegration.h"
+#include "FEM/Controller/BoundaryCondition.h"
+#include "FEM/Controller/Assembling.h"
+#include "LinearAlgebra/Solvers/CG.h"
+#include "Optimize/Solver/OC.h"
+#include "Optimize/Solver/MMA.h"
+#include "Optimize/Filter/DensityFilter.h"
+#include "PrePost/Mesher/SquareMesh.h"
+#include "PrePost/Import/ImportFromCSV.h"
+#include "FEM/Equation/General.h"
+
+using namespace PANSFEM2;
+
+namespace {
+
+// Portable PRNG: SplitMix64, u = (x >> 11) * 2^-53.
+struct SplitMix64 {
+    uint64_t s;
+    explicit SplitMix64(uint64_t seed) : s(seed) {}
+    uint64_t next() {
+        uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    }
+    double uniform() { return (double)(next() >> 11) * (1.0 / 9007199254740992.0); }
+};
+
+template <class V>
+void dump(const std::string& dir, const std::string& name, const std::vector<V>& v) {
+    std::string path = dir + "/" + name;
+    FILE* fp = std::fopen(path.c_str(), "wb");
+    if (!fp) { std::perror(path.c_str()); std::exit(2); }
+    if (!v.empty()) std::fwrite(v.data(), sizeof(V), v.size(), fp);
+    std::fclose(fp);
+}
+
+double seconds_since(std::chrono::steady_clock::time_point t0) {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// Filter neighbour lists in ascending element id.  The reference sample uses
+// an all-pairs search (sample_optimize_density_oc.cpp:53-60); at large grids
+// a window of +-(ceil(R)+1) elements around element i tests exactly the
+// candidates that can pass the same `Norm() <= R` predicate, in the same
+// ascending order.  `brute` selects the all-pairs search.
+void build_neighbors(std::vector<Vector<double> >& cg, int nx, int ny, double R, bool brute,
+                     std::vector<std::vector<int> >& nb, std::vector<std::vector<double> >& w) {
+    int ne = nx * ny;
+    nb.assign(ne, std::vector<int>());
+    w.assign(ne, std::vector<double>());
+    int win = (int)std::ceil(R) + 1;
+    for (int i = 0; i < ne; i++) {
+        if (brute) {
+            for (int j = 0; j < ne; j++) {
+                if ((cg[i] - cg[j]).Norm() <= R) {
+                    nb[i].push_back(j);
+                    w[i].push_back((R - (cg[i] - cg[j]).Norm()) / R);
+                }
+            }
+        } else {
+            int ix = i / ny, iy = i % ny;
+            for (int jx = std::max(0, ix - win); jx <= std::min(nx - 1, ix + win); jx++) {
+                for (int jy = std::max(0, iy - win); jy <= std::min(ny - 1, iy + win); jy++) {
+                    int j = ny * jx + jy;
+                    if ((cg[i] - cg[j]).Norm() <= R) {
+                        nb[i].push_back(j);
+                        w[i].push_back((R - (cg[i] - cg[j]).Norm()) / R);
+                    }
+                }
+            }
+        }
+    }
+}
+
+struct Problem2D {
+    int nx, ny;
+    std::vector<Vector<double> > x;
+    std::vector<std::vector<int> > elements;
+    std::vector<std::pair<std::pair<int, int>, double> > ufixed, qfixed;
+    std::vector<Vector<double> > cg;
+};
+
+Problem2D make_problem(int nx, int ny) {
+    Problem2D P;
+    P.nx = nx; P.ny = ny;
+    double LX = (double)nx, LY = (double)ny;
+    SquareMesh<double> mesh = SquareMesh<double>(LX, LY, nx, ny);
+    P.x = mesh.GenerateNodes();
+    P.elements = mesh.GenerateElements();
+    P.ufixed = mesh.GenerateFixedlist({0, 1}, [](Vector<double> _x) { return std::abs(_x(0)) < 1.0e-5; });
+    P.qfixed = mesh.GenerateFixedlist({1}, [LX](Vector<double> _x) {
+        return std::abs(_x(0) - LX) < 1.0e-5 && std::abs(_x(1)) < 1.0e-5;
+    });
+    for (auto& q : P.qfixed) q.second = -1.0;
+    P.cg.resize(P.elements.size());
+    for (size_t i = 0; i < P.elements.size(); i++) P.cg[i] = CenterOfGravity(P.x, P.elements[i]);
+    return P;
+}
+
+struct Assembled {
+    int N;
+    std::vector<std::vector<int> > nodetoglobal;
+    std::vector<Vector<double> > u;
+    CSR<double>* K;
+    std::vector<double> F;
+};
+
+void assemble(Problem2D& P, const std::vector<double>& rho, double E1, double E0, double p, Assembled& A) {
+    A.u = std::vector<Vector<double> >(P.x.size(), Vector<double>(2));
+    A.nodetoglobal = std::vector<std::vector<int> >(P.x.size(), std::vector<int>(2, 0));
+    SetDirichlet(A.u, A.nodetoglobal, P.ufixed);
+    A.N = Renumbering(A.nodetoglobal);
+    LILCSR<double> K = LILCSR<double>(A.N, A.N);
+    A.F = std::vector<double>(A.N, 0.0);
+    for (size_t i = 0; i < P.elements.size(); i++) {
+        double E = E1 * pow(rho[i], p) + E0 * (1.0 - pow(rho[i], p));
+        std::vector<std::vector<std::pair<int, int> > > nodetoelement;
+        Matrix<double> Ke;
+        PlaneStrainStiffness<double, ShapeFunction4Square, Gauss4Square>(Ke, nodetoelement, P.elements[i], {0, 1}, P.x, E, 0.3, 1.0);
+        Assembling(K, A.F, A.u, Ke, A.nodetoglobal, nodetoelement, P.elements[i]);
+    }
+    Assembling(A.F, P.qfixed, A.nodetoglobal);
+    A.K = new CSR<double>(K);
+}
+
+std::vector<double> flat_nodes(const std::vector<Vector<double> >& v, int d) {
+    std::vector<double> out(v.size() * d);
+    for (size_t i = 0; i < v.size(); i++)
+        for (int k = 0; k < d; k++) out[i * d + k] = const_cast<Vector<double>&>(v[i])(k);
+    return out;
+}
+
+// ---------------------------------------------------------------- simp ----
+int run_simp(int nx, int ny, int iters, bool use_mma, const std::string& out, int nsave) {
+    Problem2D P = make_problem(nx, ny);
+    const int ne = (int)P.elements.size();
+    const double R = 1.5, E0 = 1.0e-9, E1 = 1.0, p = 3.0, weightlimit = 0.5, scale0 = 1.0, scale1 = 1.0;
+    std::vector<std::vector<int> > neighbors;
+    std::vector<std::vector<double> > w;
+    build_neighbors(P.cg, nx, ny, R, ne <= 6000, neighbors, w);
+    if (ne <= 6000 && ne >= 100) {
+        // the windowed search must reproduce the all-pairs lists exactly
+        std::vector<std::vector<int> > nb2;
+        std::vector<std::vector<double> > w2;
+        build_neighbors(P.cg, nx, ny, R, false, nb2, w2);
+        if (nb2 != neighbors || w2 != w) { std::fprintf(stderr, "windowed neighbour search differs\n"); return 3; }
+    }
+    DensityFilter<double> filter = DensityFilter<double>(ne, neighbors, w);
+    std::vector<double> s(ne, 0.5);
+    if (const char* s0 = std::getenv("PF_HARNESS_S0")) {
+        // optional starting design read from a raw float64 file (ne values)
+        FILE* fp = std::fopen(s0, "rb");
+        if (!fp || std::fread(s.data(), sizeof(double), ne, fp) != (size_t)ne) { std::fprintf(stderr, "bad %s\n", s0); return 2; }
+        std::fclose(fp);
+    }
+
+    const double lambdamax = 1.0e9;
+    OC<double> oc = OC<double>(ne, 0.5, 0.0, lambdamax, 1.0e-3, 0.2, std::vector<double>(ne, 0.01), std::vector<double>(ne, 1.0));
+    MMA<double> mma = MMA<double>(ne, 1, 1.0, std::vector<double>(1, 0.0), std::vector<double>(1, 10000.0),
+                                  std::vector<double>(1, 0.0), std::vector<double>(ne, 0.01), std::vector<double>(ne, 1.0));
+    mma.SetParameters(1.0e-5, 0.1, 0.2, 0.5, 0.7, 1.2, 1.0e-6);
+
+    std::vector<double> hist_f, hist_g, hist_lambda, hist_conv, hist_nbis;
+    std::vector<int32_t> hist_decisions;  // bisection decisions (1: g>0) of every OC update, flattened
+    std::vector<double> saved_s, saved_rho;
+    for (int k = 0; k < iters; k++) {
+        std::vector<double> rho = filter.GetFilteredVariables(s);
+        double g = 0.0;
+        std::vector<double> dgdrho(ne, 0.0);
+        for (int i = 0; i < ne; i++) {
+            g += scale1 * rho[i] / (weightlimit * ne);
+            dgdrho[i] = scale1 / (weightlimit * ne);
+        }
+        g -= 1.0 * scale1;
+
+        Assembled A;
+        assemble(P, rho, E1, E0, p, A);
+        std::vector<double> result = ScalingCG(*A.K, A.F, 100000, 1.0e-10);
+        Disassembling(A.u, result, A.nodetoglobal);
+
+        RemoveBoundaryConditions(A.nodetoglobal);
+        int KD = Renumbering(A.nodetoglobal);
+        std::vector<double> RF(KD, 0.0);
+        std::vector<Vector<double> > r(P.x.size(), Vector<double>(2));
+        for (int i = 0; i < ne; i++) {
+            double E = E1 * pow(rho[i], p) + E0 * (1.0 - pow(rho[i], p));
+            std::vector<std::vector<std::pair<int, int> > > nodetoelement;
+            Matrix<double> Ke;
+            PlaneStrainStiffness<double, ShapeFunction4Square, Gauss4Square>(Ke, nodetoelement, P.elements[i], {0, 1}, P.x, E, 0.3, 1.0);
+            Vector<double> Keue = Ke * ElementVector(A.u, nodetoelement, P.elements[i]);
+            Assembling(RF, Keue, A.nodetoglobal, nodetoelement, P.elements[i]);
+        }
+        Disassembling(r, RF, A.nodetoglobal);
+        double f = scale0 * std::inner_product(A.u.begin(), A.u.end(), r.begin(), 0.0);
+        std::vector<double> dfdrho(ne, 0.0);
+        for (int i = 0; i < ne; i++) {
+            std::vector<std::vector<std::pair<int, int> > > nodetoelement;
+            Matrix<double> Ke;
+            PlaneStrainStiffness<double, ShapeFunction4Square, Gauss4Square>(Ke, nodetoelement, P.elements[i], {0, 1}, P.x, 1.0, 0.3, 1.0);
+            Vector<double> ue = ElementVector(A.u, nodetoelement, P.elements[i]);
+            dfdrho[i] = -scale0 * p * (-E0 + E1) * pow(rho[i], p - 1.0) * (ue * (Ke * ue));
+        }
+        std::vector<double> dfds = filter.GetFilteredSensitivitis(s, dfdrho);
+        std::vector<double> dgds = filter.GetFilteredSensitivitis(s, dgdrho);
+
+        if (k == 0) {
+            std::vector<int32_t> dofmap;
+            {
+                // iteration-0 DOF map, recomputed (RemoveBoundaryConditions overwrote it)
+                std::vector<std::vector<int> > n2g(P.x.size(), std::vector<int>(2, 0));
+                std::vector<Vector<double> > u0(P.x.size(), Vector<double>(2));
+                SetDirichlet(u0, n2g, P.ufixed);
+                Renumbering(n2g);
+                for (auto& nd : n2g) for (int d : nd) dofmap.push_back(d);
+            }
+            dump(out, "dofmap.i32", dofmap);
+            std::vector<int32_t> ip(A.K->indptr.begin(), A.K->indptr.end()), ix(A.K->indices.begin(), A.K->indices.end());
+            dump(out, "csr_indptr.i32", ip);
+            dump(out, "csr_indices.i32", ix);
+            dump(out, "csr_data.f64", A.K->data);
+            dump(out, "F.f64", A.F);
+            dump(out, "u0_free.f64", result);
+            dump(out, "u0_nodal.f64", flat_nodes(A.u, 2));
+            dump(out, "r0_nodal.f64", flat_nodes(r, 2));
+            dump(out, "rho0.f64", rho);
+            dump(out, "dfdrho0.f64", dfdrho);
+            dump(out, "dfds0.f64", dfds);
+            dump(out, "dgds0.f64", dgds);
+            std::vector<double> diag = GetDiagonal(*A.K);
+            dump(out, "diag0.f64", diag);
+            // element 0 stiffness at E = 1 and the filter lists
+            std::vector<std::vector<std::pair<int, int> > > n2e;
+            Matrix<double> Ke;
+            PlaneStrainStiffness<double, ShapeFunction4Square, Gauss4Square>(Ke, n2e, P.elements[0], {0, 1}, P.x, 1.0, 0.3, 1.0);
+            std::vector<double> ke(64);
+            for (int a = 0; a < 8; a++) for (int b = 0; b < 8; b++) ke[a * 8 + b] = Ke(a, b);
+            dump(out, "ke_e0.f64", ke);
+            std::vector<int32_t> nbptr(1, 0), nbidx;
+            std::vector<double> nbw;
+            for (int i = 0; i < ne; i++) {
+                for (size_t j = 0; j < neighbors[i].size(); j++) { nbidx.push_back(neighbors[i][j]); nbw.push_back(w[i][j]); }
+                nbptr.push_back((int32_t)nbidx.size());
+            }
+            dump(out, "filter_ptr.i32", nbptr);
+            dump(out, "filter_idx.i32", nbidx);
+            dump(out, "filter_w.f64", nbw);
+        }
+        delete A.K;
+
+        hist_f.push_back(f);
+        hist_g.push_back(g);
+        hist_conv.push_back(oc.IsConvergence(f) ? 1.0 : 0.0);
+        if (!use_mma) {
+            std::ostringstream cap;
+            std::streambuf* old = std::cout.rdbuf(cap.rdbuf());
+            std::cout << std::setprecision(17);
+            int nbis = 0;
+            oc.UpdateVariables(s, f, dfds, g, dgds, [&](std::vector<double> _xkp1) {
+                double gg = 0.0;
+                std::vector<double> rr = filter.GetFilteredVariables(_xkp1);
+                for (int i = 0; i < ne; i++) gg += scale1 * rr[i] / (weightlimit * ne);
+                gg = gg - 1.0 * scale1;
+                hist_decisions.push_back(gg > 0.0 ? 1 : 0);
+                nbis++;
+                return gg;
+            });
+            std::cout.rdbuf(old);
+            hist_lambda.push_back(std::stod(cap.str()));
+            hist_nbis.push_back(nbis);
+        } else {
+            mma.UpdateVariables(s, f, dfds, {g}, {dgds});
+            hist_lambda.push_back(0.0);
+            hist_nbis.push_back(0);
+        }
+        if (k < nsave) { saved_s.insert(saved_s.end(), s.begin(), s.end()); saved_rho.insert(saved_rho.end(), rho.begin(), rho.end()); }
+    }
+    std::vector<double> rho_final = filter.GetFilteredVariables(s);
+    dump(out, "hist_f.f64", hist_f);
+    dump(out, "hist_g.f64", hist_g);
+    dump(out, "hist_lambda.f64", hist_lambda);
+    dump(out, "hist_conv.f64", hist_conv);
+    dump(out, "hist_nbis.f64", hist_nbis);
+    dump(out, "hist_decisions.i32", hist_decisions);
+    dump(out, "s_final.f64", s);
+    dump(out, "rho_final.f64", rho_final);
+    dump(out, "s_saved.f64", saved_s);
+    dump(out, "rho_saved.f64", saved_rho);
+    return 0;
+}
+
+// ------------------------------------------------------------ operator ----
+int run_operator(int nx, int ny, uint64_t seed, const std::string& out) {
+    Problem2D P = make_problem(nx, ny);
+    const int ne = (int)P.elements.size();
+    SplitMix64 rng(seed);
+    std::vector<double> rho(ne);
+    for (int i = 0; i < ne; i++) rho[i] = 1.0e-3 + (1.0 - 1.0e-3) * rng.uniform();
+    Assembled A;
+    assemble(P, rho, 1.0, 1.0e-9, 3.0, A);
+    std::vector<double> xv(A.N);
+    for (int i = 0; i < A.N; i++) xv[i] = 2.0 * rng.uniform() - 1.0;
+    std::vector<double> y = (*A.K) * xv;
+    std::vector<double> u = ScalingCG(*A.K, A.F, 100000, 1.0e-10);
+    dump(out, "rho.f64", rho);
+    dump(out, "x.f64", xv);
+    dump(out, "y.f64", y);
+    dump(out, "F.f64", A.F);
+    dump(out, "u.f64", u);
+    std::vector<int32_t> ip(A.K->indptr.begin(), A.K->indptr.end()), ix(A.K->indices.begin(), A.K->indices.end());
+    dump(out, "csr_indptr.i32", ip);
+    dump(out, "csr_indices.i32", ix);
+    dump(out, "csr_data.f64", A.K->data);
+    delete A.K;
+    return 0;
+}
+
+// --------------------------------------------------------------- solid ----
+int run_solid(const std::string& dir, const std::string& out) {
+    std::vector<Vector<double> > x;
+    ImportNodesFromCSV(x, dir + "/Node.csv");
+    std::vector<std::vector<int> > elements;
+    ImportElementsFromCSV(elements, dir + "/Element.csv");
+    std::vector<std::pair<std::pair<int, int>, double> > ufixed, qfixed;
+    ImportDirichletFromCSV(ufixed, dir + "/Dirichlet.csv");
+    ImportNeumannFromCSV(qfixed, dir + "/Neumann.csv");
+    std::vector<Vector<double> > u(x.size(), Vector<double>(3));
+    std::vector<std::vector<int> > n2g(x.size(), std::vector<int>(3, 0));
+    SetDirichlet(u, n2g, ufixed);
+    int N = Renumbering(n2g);
+    LILCSR<double> K = LILCSR<double>(N, N);
+    std::vector<double> F(N, 0.0);
+    std::vector<double> ke0;
+    for (auto element : elements) {
+        std::vector<std::vector<std::pair<int, int> > > n2e;
+        Matrix<double> Ke;
+        SolidLinearIsotropicElastic<double, ShapeFunction8Cubic, Gauss8Cubic>(Ke, n2e, element, {0, 1, 2}, x, 210000.0, 0.3);
+        if (ke0.empty()) for (int a = 0; a < 24; a++) for (int b = 0; b < 24; b++) ke0.push_back(Ke(a, b));
+        Assembling(K, F, u, Ke, n2g, n2e, element);
+    }
+    Assembling(F, qfixed, n2g);
+    CSR<double> Kmod = CSR<double>(K);
+    std::vector<double> result = ScalingCG(Kmod, F, 100000, 1.0e-10);
+    Disassembling(u, result, n2g);
+    std::vector<int32_t> dofmap;
+    for (auto& nd : n2g) for (int d : nd) dofmap.push_back(d);
+    std::vector<int32_t> conn;
+    for (auto& e : elements) for (int n : e) conn.push_back(n);
+    std::vector<int32_t> ip(Kmod.indptr.begin(), Kmod.indptr.end()), ix(Kmod.indices.begin(), Kmod.indices.end());
+    dump(out, "nodes.f64", flat_nodes(x, 3));
+    dump(out, "conn.i32", conn);
+    dump(out, "dofmap.i32", dofmap);
+    dump(out, "ke_e0.f64", ke0);
+    dump(out, "csr_indptr.i32", ip);
+    dump(out, "csr_indices.i32", ix);
+    dump(out, "csr_data.f64", Kmod.data);
+    dump(out, "F.f64", F);
+    dump(out, "u_free.f64", result);
+    dump(out, "u_nodal.f64", flat_nodes(u, 3));
+    return 0;
+}
+
+// ------------------------------------------------------------- mmatest ----
+int run_mmatest(const std::string& out) {
+    const double C1 = 0.0624, C2 = 1.0;
+    std::vector<double> s(5, 5.0);
+    MMA<double> opt = MMA<double>(5, 1, 1.0, std::vector<double>(1, 0.0), std::vector<double>(1, 1000.0),
+                                  std::vector<double>(1, 1.0), std::vector<double>(5, 1.0), std::vector<double>(5, 10.0));
+    opt.SetParameters(1.0e-5, 0.1, 0.5, 0.5, 0.7, 1.2, 1.0e-5);
+    std::vector<double> traj;
+    for (int k = 0; k < 10; k++) {
+        double obj = C1 * (s[0] + s[1] + s[2] + s[3] + s[4]);
+        std::vector<double> dobj(5, C1);
+        std::vector<double> con(1);
+        std::vector<std::vector<double> > dcon(1, std::vector<double>(5));
+        con[0] = 61.0 * pow(s[0], -3.0) + 37.0 * pow(s[1], -3.0) + 19.0 * pow(s[2], -3.0) + 7.0 * pow(s[3], -3.0) + 1.0 * pow(s[4], -3.0) - C2;
+        const double cc[5] = {61.0, 37.0, 19.0, 7.0, 1.0};
+        for (int j = 0; j < 5; j++) dcon[0][j] = -3.0 * cc[j] * pow(s[j], -4.0);
+        traj.push_back(obj);
+        traj.push_back(con[0]);
+        traj.insert(traj.end(), s.begin(), s.end());
+        if (opt.IsConvergence(obj)) break;
+        opt.UpdateVariables(s, obj, dobj, con, dcon);
+    }
+    dump(out, "traj.f64", traj);
+    return 0;
+}
+
+// ---------------------------------------------------------------- time ----
+int run_time(int nx, int ny, int cg_itrmax) {
+    using clk = std::chrono::steady_clock;
+    auto t_setup = clk::now();
+    Problem2D P = make_problem(nx, ny);
+    const int ne = (int)P.elements.size();
+    std::vector<std::vector<int> > neighbors;
+    std::vector<std::vector<double> > w;
+    build_neighbors(P.cg, nx, ny, 1.5, false, neighbors, w);
+    DensityFilter<double> filter = DensityFilter<double>(ne, neighbors, w);
+    double setup_s = seconds_since(t_setup);
+    std::vector<double> s(ne, 0.5);
+    const double E0 = 1.0e-9, E1 = 1.0, p = 3.0;
+
+    auto t0 = clk::now();
+    std::vector<double> rho = filter.GetFilteredVariables(s);
+    double filter_s = seconds_since(t0);
+
+    t0 = clk::now();
+    std::vector<Vector<double> > u(P.x.size(), Vector<double>(2));
+    std::vector<std::vector<int> > n2g(P.x.size(), std::vector<int>(2, 0));
+    SetDirichlet(u, n2g, P.ufixed);
+    int N = Renumbering(n2g);
+    LILCSR<double> K = LILCSR<double>(N, N);
+    std::vector<double> F(N, 0.0);
+    for (int i = 0; i < ne; i++) {
+        double E = E1 * pow(rho[i], p) + E0 * (1.0 - pow(rho[i], p));
+        std::vector<std::vector<std::pair<int, int> > > n2e;
+        Matrix<double> Ke;
+        PlaneStrainStiffness<double, ShapeFunction4Square, Gauss4Square>(Ke, n2e, P.elements[i], {0, 1}, P.x, E, 0.3, 1.0);
+        Assembling(K, F, u, Ke, n2g, n2e, P.elements[i]);
+    }
+    Assembling(F, P.qfixed, n2g);
+    double assemble_s = seconds_since(t0);
+    t0 = clk::now();
+    CSR<double> Kmod = CSR<double>(K);
+    double csr_s = seconds_since(t0);
+
+    t0 = clk::now();
+    std::vector<double> result = ScalingCG(Kmod, F, cg_itrmax, 1.0e-10);
+    double cg_s = seconds_since(t0);
+    Disassembling(u, result, n2g);
+
+    t0 = clk::now();
+    std::vector<double> dfdrho(ne, 0.0);
+    for (int i = 0; i < ne; i++) {
+        std::vector<std::vector<std::pair<int, int> > > n2e;
+        Matrix<double> Ke;
+        PlaneStrainStiffness<double, ShapeFunction4Square, Gauss4Square>(Ke, n2e, P.elements[i], {0, 1}, P.x, 1.0, 0.3, 1.0);
+        Vector<double> ue = ElementVector(u, n2e, P.elements[i]);
+        dfdrho[i] = -p * (-E0 + E1) * pow(rho[i], p - 1.0) * (ue * (Ke * ue));
+    }
+    double sens_s = seconds_since(t0);
+
+    t0 = clk::now();
+    std::vector<double> dfds = filter.GetFilteredSensitivitis(s, dfdrho);
+    double sfilter_s = seconds_since(t0);
+    std::printf("{\"nx\": %d, \"ny\": %d, \"N\": %d, \"nnz\": %zu, \"setup_s\": %.6f, \"filter_s\": %.6f, "
+                "\"assemble_s\": %.6f, \"csr_s\": %.6f, \"cg_s\": %.6f, \"cg_itrmax\": %d, \"sens_s\": %.6f, "
+                "\"sens_filter_s\": %.6f}\n",
+                nx, ny, N, Kmod.data.size(), setup_s, filter_s, assemble_s, csr_s, cg_s, cg_itrmax, sens_s, sfilter_s);
+    return 0;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    if (argc < 2) { std::fprintf(stderr, "usage: ref_harness <mode> ...\n"); return 1; }
+    std::string mode = argv[1];
+    if (mode == "simp" && argc >= 7)
+        return run_simp(std::atoi(argv[2]), std::atoi(argv[3]), std::atoi(argv[4]), std::string(argv[5]) == "mma", argv[6],
+                        argc >= 8 ? std::atoi(argv[7]) : 0);
+    if (mode == "operator" && argc >= 6)
+        return run_operator(std::atoi(argv[2]), std::atoi(argv[3]), std::strtoull(argv[4], nullptr, 10), argv[5]);
+    if (mode == "solid" && argc >= 4) return run_solid(argv[2], argv[3]);
+    if (mode == "mmatest" && argc >= 3) return run_mmatest(argv[2]);
+    if (mode == "time" && argc >= 5) return run_time(std::atoi(argv[2]), std::atoi(argv[3]), std::atoi(argv[4]));
+    std::fprintf(stderr, "bad arguments for mode %s\n", mode.c_str());
+    return 1;
+}
