@@ -1,0 +1,218 @@
+/* pansfem2_amd: MI355X (gfx950) implementation of PANSFEM2's SIMP topology-
+ * optimisation inner loop behind a plain C ABI.
+ *
+ * Conventions
+ *   - Every entry point returns int: PF_OK (0), PF_NOT_CONVERGED (>0, a valid
+ *     last iterate is returned, as ScalingCG does at
+ *     src/LinearAlgebra/Solvers/CG.h:451-452), or a negative error code.  The
+ *     message of the last error is pf_last_error(ctx).
+ *   - Host pointers are owned by the caller and only read/written during the
+ *     call.  *_dev entry points take device pointers (hipMalloc'd, on the
+ *     context's device) and enqueue on the context's stream; they return after
+ *     the work is complete unless stated otherwise.
+ *   - One context per host thread.  A context owns one HIP stream (default
+ *     priority) and all device buffers of its grid.
+ *   - Vectors named *_free are in the reference's free-DOF numbering
+ *     (SetDirichlet + Renumbering); element arrays are in element-id order.
+ *
+ * The reference is header-only C++ templates (no ABI of its own); each entry
+ * point below names the reference routine it replaces.  The C++ drop-in
+ * facade include/pansfem2/pansfem2.hpp binds these entry points under the
+ * reference's own names.
+ */
+#ifndef PANSFEM2_AMD_H
+#define PANSFEM2_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    PF_OK = 0,
+    PF_NOT_CONVERGED = 1,
+    PF_ERR_ARG = -1,
+    PF_ERR_HIP = -2,
+    PF_ERR_STATE = -3,
+    PF_ERR_NUMERIC = -4,
+    PF_ERR_NOMEM = -5,
+    PF_ERR_UNSUPPORTED = -6
+};
+
+/* operators for pf_apply_K / pf_pcg */
+enum { PF_OP_CSR64 = 0, PF_OP_EBE64 = 1, PF_OP_EBE32 = 2 };
+/* filter modes for pf_filter */
+enum { PF_FILTER_VARIABLES = 0, PF_FILTER_SENSITIVITIES = 1 };
+
+typedef struct pf_ctx pf_ctx;
+
+typedef struct pf_solve_stats {
+    int iterations;        /* loop iterations run (k+1 of the converging one)  */
+    int status;            /* 1 converged, 2 itrmax reached, 3 breakdown        */
+    double rel_residual;   /* recursive ||r||/||b|| at exit                     */
+    double true_residual;  /* FP64 ||b - K x||/||b|| recomputed on the device   */
+    double seconds;        /* device wall time of the solve                     */
+    int outer;             /* refinement steps (mixed precision), else 0       */
+    int fallbacks;         /* flexible-CG -> Jacobi-PCG fallbacks (learned)     */
+} pf_solve_stats;
+
+typedef struct pf_oc_params {
+    double iota, lambdamin, lambdamax, lambdaeps, movelimit; /* OC<T> ctor (OC.h:45-60) */
+    double vf, scale1;     /* volume closure of sample_optimize_density_oc.cpp:199-206 */
+    int max_steps;         /* safety cap on bisection steps (the reference has none) */
+} pf_oc_params;
+
+typedef struct pf_mma_params {
+    int m;                                   /* constraints; 1 supported on the device */
+    double a0, a, c, d;                      /* MMA<T> ctor (MMA.h:63-88), m = 1 */
+    double xmin, xmax;                       /* uniform bounds */
+    double raa0, albefa, move, asyinit, asydecr, asyincr, epsvalue; /* SetParameters (MMA.h:91-100) */
+    int max_newton;                          /* cap of the interior-point loop (MMA.h:199 has none) */
+} pf_mma_params;
+
+typedef struct pf_simp_params {
+    double E1, E0, nu, t, p;       /* SIMP: E = E1 rho^p + E0 (1 - rho^p) (sample...oc.cpp:123) */
+    double vf, R, scale0, scale1;  /* volume fraction, filter radius, objective/constraint scales */
+    double cg_eps; int cg_itrmax;  /* ScalingCG(K, F, itrmax, eps) (sample...oc.cpp:132) */
+    int use_mma;                   /* 0: OC update, 1: MMA update */
+    int solver;                    /* PF_OP_EBE64 (plain FP64) or PF_OP_CSR64 */
+    int mixed;                     /* 1: FP32 EbE inner PCG inside FP64 refinement */
+    double inner_eps;              /* inner tolerance of the mixed solve */
+    int learned;                   /* 1: learned-preconditioned flexible CG (after training) */
+    int stop_on_convergence;       /* honour OC/MMA IsConvergence (OC.h:67-73) */
+    pf_oc_params oc;
+    pf_mma_params mma;
+} pf_simp_params;
+
+typedef struct pf_simp_history {
+    double* f;        /* compliance per iteration (iters)         */
+    double* g;        /* volume constraint per iteration          */
+    double* lambda;   /* last OC lambda per iteration (0 for MMA) */
+    int* cg_iters;    /* PCG iterations per iteration             */
+    int* bisect;      /* OC bisection steps / MMA Newton steps    */
+    double* seconds;  /* device wall time per iteration           */
+    int iterations;   /* iterations run                           */
+} pf_simp_history;
+
+/* ---- lifecycle --------------------------------------------------------- */
+int pf_create(pf_ctx** ctx, int device);
+int pf_destroy(pf_ctx* ctx);
+const char* pf_last_error(pf_ctx* ctx);
+/* Compiled device architecture and build flags of the library. */
+const char* pf_build_info(void);
+
+/* ---- mesh and DOFs ----------------------------------------------------- */
+/* SquareMesh<T>(lx, ly, nx, ny) with GenerateNodes/GenerateElements
+ * (src/PrePost/Mesher/SquareMesh.h:48-70,97-106): quad4, 2 DOF per node. */
+int pf_grid2d(pf_ctx* ctx, int nx, int ny, double lx, double ly);
+/* Structured hex8 grid with ShapeFunction8Cubic node order
+ * (src/FEM/Controller/ShapeFunction.h:299): 3 DOF per node.  The reference
+ * has no 3D mesher; numbering is documented in pf_internal.h. */
+int pf_grid3d(pf_ctx* ctx, int nx, int ny, int nz, double lx, double ly, double lz);
+/* SetDirichlet + Renumbering (src/FEM/Controller/BoundaryCondition.h:19-25,
+ * src/FEM/Controller/Assembling.h:175-186): the listed (node, dof) pairs are
+ * fixed (prescribed value 0); free DOFs are numbered in (node, dof) order. */
+int pf_set_dirichlet(pf_ctx* ctx, const int32_t* node, const int32_t* dof, int nfixed, int32_t* n_free);
+/* Read back nodetoglobal (nn*ndpn, -1 for fixed). */
+int pf_get_dofmap(pf_ctx* ctx, int32_t* n2g);
+/* Assembling(F, qfixed, nodetoglobal) (Assembling.h:151-158): nodal point
+ * loads accumulated into the right-hand side used by pf_simp_run. */
+int pf_set_point_loads(pf_ctx* ctx, const int32_t* node, const int32_t* dof, const double* value, int n);
+/* Read back the assembled load vector in free-DOF numbering (N). */
+int pf_get_load(pf_ctx* ctx, double* F_free);
+
+/* ---- element matrices and material ------------------------------------- */
+/* Shared unit element matrix K0 (nd x nd, row-major) at E = 1: the result
+ * of PlaneStrainStiffness (src/FEM/Equation/PlaneStrain.h:19-58) or
+ * SolidLinearIsotropicElastic (src/FEM/Equation/Solid.h:19-64) on one
+ * element of the grid.  Used by the matrix-free operator and compliance. */
+int pf_set_element_matrix(pf_ctx* ctx, const double* K0, int nd);
+/* PlaneStrainStiffness / SolidLinearIsotropicElastic evaluated on the device
+ * in the reference's operation order for element `e` with modulus E (check
+ * path): Ke is nd x nd. */
+int pf_element_stiffness(pf_ctx* ctx, int e, double E, double nu, double t, double* Ke);
+/* SIMP material: E = E1 rho^p + E0 (1 - rho^p) (sample...oc.cpp:123,142). */
+int pf_set_material(pf_ctx* ctx, double E1, double E0, double p, double nu, double t);
+/* Element densities (ne) -> device; element moduli computed on the device. */
+int pf_set_density(pf_ctx* ctx, const double* rho);
+/* Element moduli given directly (ne), as the driver passes E to
+ * PlaneStrainStiffness (sample...oc.cpp:123-126). */
+int pf_set_moduli(pf_ctx* ctx, const double* E);
+
+/* ---- assembled CSR check path ------------------------------------------ */
+/* Assembling(LILCSR&, ...) over all elements + CSR<T>(LILCSR&)
+ * (Assembling.h:46-66, src/LinearAlgebra/Models/LILCSR.h:92-114,
+ * src/LinearAlgebra/Models/CSR.h:92-105) on the device, element matrices in
+ * reference arithmetic from the current moduli.  Structure and values are
+ * bit-identical to the reference's.  *nnz receives the entry count. */
+int pf_assemble_csr(pf_ctx* ctx, int64_t* nnz);
+/* Copy the assembled CSR to the host (indptr N+1, indices/data nnz). */
+int pf_get_csr(pf_ctx* ctx, int32_t* indptr, int32_t* indices, double* data);
+
+/* ---- operators and solvers --------------------------------------------- */
+/* y = K x on free-DOF vectors.  PF_OP_CSR64 is CSR<T>::operator*
+ * (CSR.h:108-122), bit-identical to the reference; PF_OP_EBE64/EBE32 apply
+ * sum_e E_e K0 element by element without assembly. */
+int pf_apply_K(pf_ctx* ctx, int op, const double* x_free, double* y_free);
+/* ScalingCG(A, b, itrmax, eps) (CG.h:419-453): Jacobi-PCG from x0 = 0 that
+ * stops when ||r_k|| < eps ||b|| (recursive residual).  op selects the
+ * operator; PF_OP_EBE32 runs FP32 inner solves inside FP64 iterative
+ * refinement to a true FP64 residual of eps. */
+int pf_pcg(pf_ctx* ctx, int op, const double* b_free, double* x_free, double eps, int itrmax, pf_solve_stats* st);
+/* Same on device-resident free-DOF vectors. */
+int pf_pcg_dev(pf_ctx* ctx, int op, const double* b_free_dev, double* x_free_dev, double eps, int itrmax,
+               pf_solve_stats* st);
+/* Compliance and sensitivities (sample_optimize_density_oc.cpp:136-162):
+ * f = scale0 u^T K u and dfdrho_e = -scale0 p (E1 - E0) rho_e^(p-1) u_e^T K0 u_e
+ * for the current densities.  u_free is a free-DOF vector (N). */
+int pf_compliance_sens(pf_ctx* ctx, const double* u_free, double scale0, double* f, double* dfdrho);
+
+/* ---- filter ------------------------------------------------------------ */
+/* DensityFilter<T> with the sample's neighbour rule |cg_i - cg_j| <= R,
+ * w = (R - |.|)/R (src/Optimize/Filter/DensityFilter.h:34-71,
+ * sample_optimize_density_oc.cpp:44-60).  Weights are recomputed on the
+ * device from the element centroids in reference arithmetic. */
+int pf_filter_setup(pf_ctx* ctx, double R);
+/* mode 0: GetFilteredVariables (DensityFilter.h:45-56);
+ * mode 1: GetFilteredSensitivitis (DensityFilter.h:60-71). ne in, ne out. */
+int pf_filter(pf_ctx* ctx, int mode, const double* in, double* out);
+
+/* ---- optimisers -------------------------------------------------------- */
+/* OC<T>::UpdateVariables (src/Optimize/Solver/OC.h:76-107) with the
+ * sample's filtered-volume closure; x (ne) updated in place. */
+int pf_oc_update(pf_ctx* ctx, double* x, const double* dfdx, const double* dgdx, const pf_oc_params* prm,
+                 double* lambda_out, int* steps_out);
+/* MMA<T> (src/Optimize/Solver/MMA.h:15-509), m = 1, state kept in the
+ * context: pf_mma_init resets it (ctor + SetParameters), pf_mma_update runs
+ * UpdateVariables(x, f, dfdx, {g}, {dgdx}). */
+int pf_mma_init(pf_ctx* ctx, const pf_mma_params* prm);
+int pf_mma_update(pf_ctx* ctx, double* x, double f, const double* dfdx, double g, const double* dgdx,
+                  int* newton_steps);
+
+/* ---- the whole loop ---------------------------------------------------- */
+/* The SIMP loop of sample_optimize_density_oc.cpp:83-208 (or the MMA
+ * variant, sample_optimize_density_mma.cpp:88-204) run device-resident for
+ * `iters` iterations on the current grid, Dirichlet set, loads and K0:
+ * filter -> volume -> moduli -> PCG -> compliance/sensitivities -> filtered
+ * sensitivities -> OC/MMA.  s (ne) holds the initial design and receives the
+ * final one; rho_out (ne, may be NULL) the final filtered density. */
+int pf_simp_run(pf_ctx* ctx, const pf_simp_params* prm, int iters, double* s, double* rho_out, pf_simp_history* hist);
+/* Fill prm with the defaults of the SIMP workloads (E1 1, E0 1e-9, nu 0.3,
+ * t 1, p 3, vf 0.5, R 1.5, scales 1, ScalingCG(100000, 1e-10), OC(0.5, 0,
+ * 1e9, 1e-3, 0.2), MMA(a0 1, a 0, c 1e4, d 0, [0.01,1], 1e-5, 0.1, 0.2, 0.5,
+ * 0.7, 1.2, 1e-6)). */
+void pf_simp_default_params(pf_simp_params* prm);
+
+/* ---- learned preconditioner -------------------------------------------- */
+/* 4-level 3x3 convolutional network (BF16 weights and activations, FP32
+ * accumulation) mapping (rho, r_x, r_y) to a displacement correction, used
+ * inside flexible CG.  No counterpart in the reference. */
+int pf_precond_init(pf_ctx* ctx, int channels, uint64_t seed);
+int pf_precond_save(pf_ctx* ctx, float* params, int64_t* count);
+int pf_precond_load(pf_ctx* ctx, const float* params, int64_t count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

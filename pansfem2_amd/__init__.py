@@ -1,0 +1,328 @@
+"""Python host mirror of the pansfem2_amd C ABI (include/pansfem2_amd.h).
+
+Loads the in-tree libpansfem2_amd.so (built by __graft_entry__.build() /
+`make -C pansfem2_amd/csrc`) and exposes the entry points under the names of
+the reference routines they replace.  There is no fallback: if the library
+is missing, or a call fails, an exception is raised.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpansfem2_amd.so")
+
+PF_OK, PF_NOT_CONVERGED = 0, 1
+PF_OP_CSR64, PF_OP_EBE64, PF_OP_EBE32 = 0, 1, 2
+PF_FILTER_VARIABLES, PF_FILTER_SENSITIVITIES = 0, 1
+
+_i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+_f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+
+
+class PfError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"pansfem2_amd error {code}: {msg}")
+        self.code = code
+
+
+class SolveStats(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int), ("status", ctypes.c_int), ("rel_residual", ctypes.c_double),
+                ("true_residual", ctypes.c_double), ("seconds", ctypes.c_double), ("outer", ctypes.c_int),
+                ("fallbacks", ctypes.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class OcParams(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in ("iota", "lambdamin", "lambdamax", "lambdaeps", "movelimit", "vf",
+                                               "scale1")] + [("max_steps", ctypes.c_int)]
+
+
+class MmaParams(ctypes.Structure):
+    _fields_ = [("m", ctypes.c_int)] + [(n, ctypes.c_double) for n in (
+        "a0", "a", "c", "d", "xmin", "xmax", "raa0", "albefa", "move", "asyinit", "asydecr", "asyincr",
+        "epsvalue")] + [("max_newton", ctypes.c_int)]
+
+
+class SimpParams(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in ("E1", "E0", "nu", "t", "p", "vf", "R", "scale0", "scale1",
+                                               "cg_eps")] + \
+               [("cg_itrmax", ctypes.c_int), ("use_mma", ctypes.c_int), ("solver", ctypes.c_int),
+                ("mixed", ctypes.c_int), ("inner_eps", ctypes.c_double), ("learned", ctypes.c_int),
+                ("stop_on_convergence", ctypes.c_int), ("oc", OcParams), ("mma", MmaParams)]
+
+
+class SimpHistory(ctypes.Structure):
+    _fields_ = [("f", ctypes.c_void_p), ("g", ctypes.c_void_p), ("lambda_", ctypes.c_void_p),
+                ("cg_iters", ctypes.c_void_p), ("bisect", ctypes.c_void_p), ("seconds", ctypes.c_void_p),
+                ("iterations", ctypes.c_int)]
+
+
+_lib = None
+
+
+def lib():
+    """The loaded C library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                              "or make -C pansfem2_amd/csrc")
+        L = ctypes.CDLL(LIB_PATH)
+        c_int, c_double, c_void_p, c_int64 = ctypes.c_int, ctypes.c_double, ctypes.c_void_p, ctypes.c_int64
+        P = ctypes.POINTER
+        sig = {
+            "pf_create": ([P(c_void_p), c_int], c_int),
+            "pf_destroy": ([c_void_p], c_int),
+            "pf_last_error": ([c_void_p], ctypes.c_char_p),
+            "pf_build_info": ([], ctypes.c_char_p),
+            "pf_grid2d": ([c_void_p, c_int, c_int, c_double, c_double], c_int),
+            "pf_grid3d": ([c_void_p, c_int, c_int, c_int, c_double, c_double, c_double], c_int),
+            "pf_set_dirichlet": ([c_void_p, _i32p, _i32p, c_int, P(ctypes.c_int32)], c_int),
+            "pf_get_dofmap": ([c_void_p, _i32p], c_int),
+            "pf_set_point_loads": ([c_void_p, _i32p, _i32p, _f64p, c_int], c_int),
+            "pf_get_load": ([c_void_p, _f64p], c_int),
+            "pf_set_element_matrix": ([c_void_p, _f64p, c_int], c_int),
+            "pf_element_stiffness": ([c_void_p, c_int, c_double, c_double, c_double, _f64p], c_int),
+            "pf_set_material": ([c_void_p, c_double, c_double, c_double, c_double, c_double], c_int),
+            "pf_set_density": ([c_void_p, _f64p], c_int),
+            "pf_set_moduli": ([c_void_p, _f64p], c_int),
+            "pf_assemble_csr": ([c_void_p, P(c_int64)], c_int),
+            "pf_get_csr": ([c_void_p, _i32p, _i32p, _f64p], c_int),
+            "pf_apply_K": ([c_void_p, c_int, _f64p, _f64p], c_int),
+            "pf_pcg": ([c_void_p, c_int, _f64p, _f64p, c_double, c_int, P(SolveStats)], c_int),
+            "pf_pcg_dev": ([c_void_p, c_int, c_void_p, c_void_p, c_double, c_int, P(SolveStats)], c_int),
+            "pf_compliance_sens": ([c_void_p, _f64p, c_double, P(c_double), _f64p], c_int),
+            "pf_filter_setup": ([c_void_p, c_double], c_int),
+            "pf_filter": ([c_void_p, c_int, _f64p, _f64p], c_int),
+            "pf_oc_update": ([c_void_p, _f64p, _f64p, _f64p, P(OcParams), P(c_double), P(c_int)], c_int),
+            "pf_mma_init": ([c_void_p, P(MmaParams)], c_int),
+            "pf_mma_update": ([c_void_p, _f64p, c_double, _f64p, c_double, _f64p, P(c_int)], c_int),
+            "pf_simp_run": ([c_void_p, P(SimpParams), c_int, _f64p, c_void_p, P(SimpHistory)], c_int),
+            "pf_simp_default_params": ([P(SimpParams)], None),
+            "pf_precond_init": ([c_void_p, c_int, ctypes.c_uint64], c_int),
+            "pf_precond_save": ([c_void_p, c_void_p, P(c_int64)], c_int),
+            "pf_precond_load": ([c_void_p, _f32p, c_int64], c_int),
+        }
+        for name, (args, res) in sig.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = L
+    return _lib
+
+
+def simp_default_params(**kw):
+    p = SimpParams()
+    lib().pf_simp_default_params(ctypes.byref(p))
+    for k, v in kw.items():
+        if k.startswith("oc_"):
+            setattr(p.oc, k[3:], v)
+        elif k.startswith("mma_"):
+            setattr(p.mma, k[4:], v)
+        else:
+            setattr(p, k, v)
+    return p
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+class Context:
+    """One pf_ctx: a device, a stream and the buffers of one grid."""
+
+    def __init__(self, device=0):
+        self._h = ctypes.c_void_p()
+        rc = lib().pf_create(ctypes.byref(self._h), device)
+        if rc != PF_OK:
+            raise PfError(rc, "pf_create failed (no HIP device?)")
+        self.N = 0
+        self.nx = self.ny = self.nz = 0
+
+    def close(self):
+        if self._h:
+            lib().pf_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _ck(self, rc, allow_nc=False):
+        if rc == PF_OK or (allow_nc and rc == PF_NOT_CONVERGED):
+            return rc
+        raise PfError(rc, lib().pf_last_error(self._h).decode())
+
+    # ---- mesh / DOFs -----------------------------------------------------
+    def grid2d(self, nx, ny, lx=None, ly=None):
+        """SquareMesh(lx, ly, nx, ny) (SquareMesh.h:48-106)."""
+        self._ck(lib().pf_grid2d(self._h, nx, ny, float(nx if lx is None else lx), float(ny if ly is None else ly)))
+        self.nx, self.ny, self.dim, self.ndpn = nx, ny, 2, 2
+        self.nn, self.ne = (nx + 1) * (ny + 1), nx * ny
+        return self
+
+    def grid3d(self, nx, ny, nz, lx=None, ly=None, lz=None):
+        self._ck(lib().pf_grid3d(self._h, nx, ny, nz, float(nx if lx is None else lx),
+                                 float(ny if ly is None else ly), float(nz if lz is None else lz)))
+        self.nx, self.ny, self.nz, self.dim, self.ndpn = nx, ny, nz, 3, 3
+        self.nn, self.ne = (nx + 1) * (ny + 1) * (nz + 1), nx * ny * nz
+        return self
+
+    def set_dirichlet(self, nodes, dofs):
+        """SetDirichlet + Renumbering (BoundaryCondition.h:19-25, Assembling.h:175-186)."""
+        nodes, dofs = _i32(nodes), _i32(dofs)
+        n = ctypes.c_int32(0)
+        self._ck(lib().pf_set_dirichlet(self._h, nodes, dofs, len(nodes), ctypes.byref(n)))
+        self.N = n.value
+        return self.N
+
+    def dofmap(self):
+        out = np.zeros(self.nn * self.ndpn, np.int32)
+        self._ck(lib().pf_get_dofmap(self._h, out))
+        return out
+
+    def set_point_loads(self, nodes, dofs, values):
+        """Assembling(F, qfixed, nodetoglobal) (Assembling.h:151-158)."""
+        self._ck(lib().pf_set_point_loads(self._h, _i32(nodes), _i32(dofs), _f64(values), len(nodes)))
+
+    def load(self):
+        out = np.zeros(self.N)
+        self._ck(lib().pf_get_load(self._h, out))
+        return out
+
+    # ---- element matrices / material ---------------------------------------
+    def set_element_matrix(self, K0):
+        K0 = _f64(K0)
+        self._ck(lib().pf_set_element_matrix(self._h, K0.reshape(-1), K0.shape[0]))
+
+    def element_stiffness(self, e, E=1.0, nu=0.3, t=1.0):
+        """PlaneStrainStiffness for element e evaluated on the device in reference order."""
+        nd = 8 if self.dim == 2 else 24
+        out = np.zeros(nd * nd)
+        self._ck(lib().pf_element_stiffness(self._h, e, E, nu, t, out))
+        return out.reshape(nd, nd)
+
+    def set_material(self, E1=1.0, E0=1e-9, p=3.0, nu=0.3, t=1.0):
+        self._ck(lib().pf_set_material(self._h, E1, E0, p, nu, t))
+
+    def set_density(self, rho):
+        self._ck(lib().pf_set_density(self._h, _f64(rho)))
+
+    def set_moduli(self, E):
+        self._ck(lib().pf_set_moduli(self._h, _f64(E)))
+
+    # ---- CSR check path ----------------------------------------------------
+    def assemble_csr(self):
+        nnz = ctypes.c_int64(0)
+        self._ck(lib().pf_assemble_csr(self._h, ctypes.byref(nnz)))
+        self.nnz = nnz.value
+        return self.nnz
+
+    def csr(self):
+        ip = np.zeros(self.N + 1, np.int32)
+        ix = np.zeros(max(self.nnz, 1), np.int32)
+        dv = np.zeros(max(self.nnz, 1))
+        self._ck(lib().pf_get_csr(self._h, ip, ix, dv))
+        return ip, ix[:self.nnz], dv[:self.nnz]
+
+    # ---- operators / solvers ----------------------------------------------
+    def apply_K(self, x, op=PF_OP_EBE64):
+        y = np.zeros(self.N)
+        self._ck(lib().pf_apply_K(self._h, op, _f64(x), y))
+        return y
+
+    def pcg(self, b, op=PF_OP_EBE64, eps=1e-10, itrmax=100000):
+        """ScalingCG(K, b, itrmax, eps) (CG.h:419-453) on the device."""
+        x = np.zeros(self.N)
+        st = SolveStats()
+        self._ck(lib().pf_pcg(self._h, op, _f64(b), x, eps, itrmax, ctypes.byref(st)), allow_nc=True)
+        return x, st.as_dict()
+
+    def pcg_dev(self, b_ptr, x_ptr, op=PF_OP_EBE64, eps=1e-10, itrmax=100000):
+        st = SolveStats()
+        self._ck(lib().pf_pcg_dev(self._h, op, ctypes.c_void_p(b_ptr), ctypes.c_void_p(x_ptr), eps, itrmax,
+                                  ctypes.byref(st)), allow_nc=True)
+        return st.as_dict()
+
+    def compliance_sens(self, u, scale0=1.0):
+        f = ctypes.c_double(0.0)
+        d = np.zeros(self.ne)
+        self._ck(lib().pf_compliance_sens(self._h, _f64(u), scale0, ctypes.byref(f), d))
+        return f.value, d
+
+    def filter_setup(self, R=1.5):
+        self._ck(lib().pf_filter_setup(self._h, R))
+
+    def filter(self, x, mode=PF_FILTER_VARIABLES):
+        out = np.zeros(self.ne)
+        self._ck(lib().pf_filter(self._h, mode, _f64(x), out))
+        return out
+
+    def oc_update(self, x, dfdx, dgdx, iota=0.5, lambdamin=0.0, lambdamax=1e9, lambdaeps=1e-3, movelimit=0.2,
+                  vf=0.5, scale1=1.0, max_steps=200):
+        """OC<T>::UpdateVariables (OC.h:76-107) with the filtered-volume closure."""
+        prm = OcParams(iota, lambdamin, lambdamax, lambdaeps, movelimit, vf, scale1, max_steps)
+        x = np.array(x, dtype=np.float64)
+        lam = ctypes.c_double(0.0)
+        steps = ctypes.c_int(0)
+        self._ck(lib().pf_oc_update(self._h, x, _f64(dfdx), _f64(dgdx), ctypes.byref(prm), ctypes.byref(lam),
+                                    ctypes.byref(steps)))
+        return x, lam.value, steps.value
+
+    def mma_init(self, **kw):
+        p = simp_default_params().mma
+        for k, v in kw.items():
+            setattr(p, k, v)
+        self._ck(lib().pf_mma_init(self._h, ctypes.byref(p)))
+
+    def mma_update(self, x, f, dfdx, g, dgdx):
+        x = np.array(x, dtype=np.float64)
+        steps = ctypes.c_int(0)
+        self._ck(lib().pf_mma_update(self._h, x, f, _f64(dfdx), g, _f64(dgdx), ctypes.byref(steps)))
+        return x, steps.value
+
+    def simp_run(self, iters, params=None, s0=0.5):
+        """The device-resident SIMP loop (sample_optimize_density_oc.cpp:83-208)."""
+        params = params or simp_default_params()
+        s = np.full(self.ne, s0) if np.isscalar(s0) else np.array(s0, dtype=np.float64)
+        rho = np.zeros(self.ne)
+        h = {k: np.zeros(iters) for k in ("f", "g", "lambda", "seconds")}
+        h["cg_iters"] = np.zeros(iters, np.int32)
+        h["bisect"] = np.zeros(iters, np.int32)
+        H = SimpHistory(h["f"].ctypes.data, h["g"].ctypes.data, h["lambda"].ctypes.data, h["cg_iters"].ctypes.data,
+                        h["bisect"].ctypes.data, h["seconds"].ctypes.data, 0)
+        self._ck(lib().pf_simp_run(self._h, ctypes.byref(params), iters, s, rho.ctypes.data, ctypes.byref(H)))
+        n = H.iterations
+        return dict(s=s, rho=rho, **{k: v[:n] for k, v in h.items()})
+
+
+def cantilever2d(nx, ny, device=0, ctx=None, E1=1.0, E0=1e-9, p=3.0, nu=0.3, t=1.0, R=1.5):
+    """The SIMP workload's grid: SquareMesh(nx, ny, nx, ny), left edge clamped,
+    unit downward load at the bottom-right node, K0 = element 0 at E = 1."""
+    ctx = ctx or Context(device)
+    ctx.grid2d(nx, ny, float(nx), float(ny))
+    nodes = np.repeat(np.arange(ny + 1, dtype=np.int32), 2)
+    dofs = np.tile(np.array([0, 1], np.int32), ny + 1)
+    ctx.set_dirichlet(nodes, dofs)
+    ctx.set_point_loads([(ny + 1) * nx], [1], [-1.0])
+    ctx.set_material(E1, E0, p, nu, t)
+    ctx.set_element_matrix(ctx.element_stiffness(0, 1.0, nu, t))
+    ctx.filter_setup(R)
+    return ctx

@@ -1,0 +1,938 @@
+// C-ABI of libpansfem2_amd: context, grids, DOF maps, operators, solvers,
+// filter, OC and the device-resident SIMP loop.  Declarations and the
+// reference routine each entry point replaces: include/pansfem2_amd.h.
+#include "pf_kernels.hip"
+
+#include <algorithm>
+#include <chrono>
+#include <type_traits>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+
+using namespace pf;
+
+namespace {
+
+int grid_cap() {
+    static int cap = [] {
+        const char* s = std::getenv("PF_GRID_CAP");
+        int v = s ? std::atoi(s) : 1024;
+        return std::max(64, std::min(v, kMaxPartials));
+    }();
+    return cap;
+}
+
+unsigned gsize(long n) { return (unsigned)std::max(1L, std::min((long)grid_cap(), (n + kBlock - 1) / kBlock)); }
+
+template <class T>
+int dalloc(pf_ctx* ctx, T** p, long n) {
+    if (*p) {
+        hipFree(*p);
+        *p = nullptr;
+    }
+    if (n <= 0) return PF_OK;
+    hipError_t e = hipMalloc((void**)p, sizeof(T) * (size_t)n);
+    if (e != hipSuccess) {
+        ctx->err = std::string("hipMalloc: ") + hipGetErrorString(e);
+        *p = nullptr;
+        return PF_ERR_NOMEM;
+    }
+    return PF_OK;
+}
+
+template <class T>
+void dfree(T*& p) {
+    if (p) hipFree(p);
+    p = nullptr;
+}
+
+int set_device(pf_ctx* ctx) {
+    PF_HIP(hipSetDevice(ctx->device));
+    return PF_OK;
+}
+
+int sync(pf_ctx* ctx) {
+    PF_HIP(hipStreamSynchronize(ctx->stream));
+    return PF_OK;
+}
+
+int check_launch(pf_ctx* ctx) {
+    PF_HIP(hipGetLastError());
+    return PF_OK;
+}
+
+bool is2d(pf_ctx* ctx) { return ctx->dim == 2; }
+
+void free_grid(pf_ctx* ctx) {
+    dfree(ctx->d_n2g); dfree(ctx->d_g2n); dfree(ctx->d_fix);
+    dfree(ctx->d_rho); dfree(ctx->d_E); dfree(ctx->d_Ef);
+    dfree(ctx->d_x); dfree(ctx->d_r); dfree(ctx->d_z); dfree(ctx->d_q); dfree(ctx->d_p0); dfree(ctx->d_p1);
+    dfree(ctx->d_D); dfree(ctx->d_b); dfree(ctx->d_t0); dfree(ctx->d_t1);
+    dfree(ctx->f_x); dfree(ctx->f_r); dfree(ctx->f_z); dfree(ctx->f_q); dfree(ctx->f_p0); dfree(ctx->f_p1);
+    dfree(ctx->f_D); dfree(ctx->f_b);
+    dfree(ctx->d_e0); dfree(ctx->d_e1); dfree(ctx->d_e2); dfree(ctx->d_e3); dfree(ctx->d_e4); dfree(ctx->d_e5);
+    dfree(ctx->d_cg); dfree(ctx->d_load);
+    dfree(ctx->d_indptr); dfree(ctx->d_indices); dfree(ctx->d_data); dfree(ctx->d_ke_all);
+    ctx->nnz = -1;
+    ctx->N = 0;
+    ctx->have_E = false;
+    ctx->have_filter = false;
+}
+
+int alloc_grid(pf_ctx* ctx) {
+    long nv = ctx->nv, ne = ctx->ne, nn = ctx->nn;
+    PF_TRY(dalloc(ctx, &ctx->d_fix, nn));
+    PF_TRY(dalloc(ctx, &ctx->d_n2g, nv));
+    PF_TRY(dalloc(ctx, &ctx->d_rho, ne));
+    PF_TRY(dalloc(ctx, &ctx->d_E, ne));
+    PF_TRY(dalloc(ctx, &ctx->d_Ef, ne));
+    double** dv[] = {&ctx->d_x, &ctx->d_r, &ctx->d_z, &ctx->d_q, &ctx->d_p0, &ctx->d_p1, &ctx->d_D, &ctx->d_b,
+                     &ctx->d_t0, &ctx->d_t1, &ctx->d_load};
+    for (double** p : dv) PF_TRY(dalloc(ctx, p, nv));
+    float** fv[] = {&ctx->f_x, &ctx->f_r, &ctx->f_z, &ctx->f_q, &ctx->f_p0, &ctx->f_p1, &ctx->f_D, &ctx->f_b};
+    for (float** p : fv) PF_TRY(dalloc(ctx, p, nv));
+    double** de[] = {&ctx->d_e0, &ctx->d_e1, &ctx->d_e2, &ctx->d_e3, &ctx->d_e4, &ctx->d_e5};
+    for (double** p : de) PF_TRY(dalloc(ctx, p, ne));
+    PF_TRY(dalloc(ctx, &ctx->d_cg, ne * ctx->dim));
+    PF_HIP(hipMemsetAsync(ctx->d_load, 0, sizeof(double) * nv, ctx->stream));
+    PF_HIP(hipMemsetAsync(ctx->d_fix, 0, nn, ctx->stream));
+    return PF_OK;
+}
+
+// Host rule of SetDirichlet + Renumbering (BoundaryCondition.h:19-25,
+// Assembling.h:175-186), uploaded with the inverse map and fixity bits.
+int upload_dofmap(pf_ctx* ctx, const std::vector<uint8_t>& fix) {
+    long nv = ctx->nv;
+    ctx->h_n2g.assign(nv, 0);
+    std::vector<int32_t> g2n;
+    g2n.reserve(nv);
+    int N = 0;
+    for (long n = 0; n < ctx->nn; n++)
+        for (int d = 0; d < ctx->ndpn; d++) {
+            long idx = n * ctx->ndpn + d;
+            if (fix[n] & (1u << d)) {
+                ctx->h_n2g[idx] = -1;
+            } else {
+                ctx->h_n2g[idx] = N++;
+                g2n.push_back((int32_t)idx);
+            }
+        }
+    ctx->N = N;
+    PF_TRY(dalloc(ctx, &ctx->d_g2n, std::max(N, 1)));
+    PF_HIP(hipMemcpyAsync(ctx->d_n2g, ctx->h_n2g.data(), sizeof(int32_t) * nv, hipMemcpyHostToDevice, ctx->stream));
+    if (N > 0)
+        PF_HIP(hipMemcpyAsync(ctx->d_g2n, g2n.data(), sizeof(int32_t) * N, hipMemcpyHostToDevice, ctx->stream));
+    PF_HIP(hipMemcpyAsync(ctx->d_fix, fix.data(), ctx->nn, hipMemcpyHostToDevice, ctx->stream));
+    ctx->nnz = -1;
+    return sync(ctx);
+}
+
+int require_ready(pf_ctx* ctx, bool need_E) {
+    PF_CHECK(ctx->dim == 2 || ctx->dim == 3, PF_ERR_STATE, "no grid: call pf_grid2d/pf_grid3d first");
+    PF_CHECK(ctx->N > 0, PF_ERR_STATE, "no DOF map: call pf_set_dirichlet first");
+    PF_CHECK(!ctx->h_K0.empty(), PF_ERR_STATE, "no element matrix: call pf_set_element_matrix first");
+    if (need_E) PF_CHECK(ctx->have_E, PF_ERR_STATE, "no element moduli: call pf_set_density or pf_set_moduli first");
+    return PF_OK;
+}
+
+// ------------------------------------------------------------ operators ---
+int apply_nodal(pf_ctx* ctx, bool f32, const void* x, void* y, bool mask) {
+    if (is2d(ctx)) {
+        if (f32)
+            k_apply_q4<float><<<gsize(ctx->nn), kBlock, 0, ctx->stream>>>(ctx->nx, ctx->ny, (const float*)x, (float*)y,
+                                                                        ctx->d_Ef, ctx->d_K0f, ctx->d_fix, mask);
+        else
+            k_apply_q4<double><<<gsize(ctx->nn), kBlock, 0, ctx->stream>>>(ctx->nx, ctx->ny, (const double*)x,
+                                                                         (double*)y, ctx->d_E, ctx->d_K0, ctx->d_fix,
+                                                                         mask);
+        return check_launch(ctx);
+    }
+    ctx->err = "3D operator not available";
+    return PF_ERR_UNSUPPORTED;
+}
+
+int diag_nodal(pf_ctx* ctx, bool f32) {
+    if (is2d(ctx)) {
+        if (f32)
+            k_diag_q4<float><<<gsize(ctx->nn), kBlock, 0, ctx->stream>>>(ctx->nx, ctx->ny, ctx->d_Ef, ctx->d_K0f,
+                                                                       ctx->d_fix, ctx->f_D);
+        else
+            k_diag_q4<double><<<gsize(ctx->nn), kBlock, 0, ctx->stream>>>(ctx->nx, ctx->ny, ctx->d_E, ctx->d_K0,
+                                                                        ctx->d_fix, ctx->d_D);
+        return check_launch(ctx);
+    }
+    ctx->err = "3D operator not available";
+    return PF_ERR_UNSUPPORTED;
+}
+
+int to_nodal(pf_ctx* ctx, const double* xf_dev, double* xn_dev) {
+    PF_HIP(hipMemsetAsync(xn_dev, 0, sizeof(double) * ctx->nv, ctx->stream));
+    k_free_to_nodal<<<nblocks(ctx->N), kBlock, 0, ctx->stream>>>(ctx->N, ctx->d_g2n, xf_dev, xn_dev);
+    return check_launch(ctx);
+}
+
+int to_free(pf_ctx* ctx, const double* xn_dev, double* xf_dev) {
+    k_nodal_to_free<<<nblocks(ctx->N), kBlock, 0, ctx->stream>>>(ctx->N, ctx->d_g2n, xn_dev, xf_dev);
+    return check_launch(ctx);
+}
+
+// ---------------------------------------------------------------- PCG -----
+struct PcgRun {
+    int iterations = 0, status = 0;
+    double rnorm = 0, bnorm = 0;
+};
+
+// Jacobi-PCG on device vectors: nodal (EbE, n = nv) or free-numbered (CSR,
+// n = N).  b, x, D are of the matching layout; x receives the solution.
+template <class T>
+int pcg_device(pf_ctx* ctx, bool csr, const T* b, T* x, const T* D, T* r, T* z, T* p0, T* p1, T* q, double eps,
+               int itrmax, PcgRun& out) {
+    const long n = csr ? ctx->N : ctx->nv;
+    PcgDev* S = (PcgDev*)ctx->d_pcg;
+    double* partA = ctx->red.partials;
+    double* partB = ctx->red.partials + 2 * kMaxPartials;
+    const unsigned g1 = csr ? gsize(ctx->N) : gsize(ctx->nn);
+    const unsigned g2 = gsize(n);
+    k_pcg_init<T><<<g2, kBlock, 0, ctx->stream>>>(n, b, D, x, r, z, p0, partB, S);
+    PF_TRY(check_launch(ctx));
+    const int chunk = 32;
+    int k = 0;
+    // one chunk is kept in flight while the previous chunk's flag is read
+    hipEvent_t ev[2];
+    PF_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+    PF_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+    volatile int* hflag = (volatile int*)ctx->h_red;  // two ints: done flags of chunks in flight
+    hflag[0] = hflag[1] = 0;
+    int slot = 0, pending = 0;
+    bool done = false;
+    auto enqueue_chunk = [&](int kend) -> int {
+        for (; k < kend; k++) {
+            T* pold = (k & 1) ? p1 : p0;
+            T* pnew = (k & 1) ? p0 : p1;
+            if (csr) {
+                k_pcg_k1_csr<<<g1, kBlock, 0, ctx->stream>>>(ctx->N, k, ctx->d_indptr, ctx->d_indices, ctx->d_data,
+                                                            (const double*)pold, (const double*)z, (double*)pnew,
+                                                            (double*)q, partB, (int)g2, partA, S, eps);
+            } else if (is2d(ctx)) {
+                const T* Ev = std::is_same<T, float>::value ? (const T*)ctx->d_Ef : (const T*)ctx->d_E;
+                const T* Kv = std::is_same<T, float>::value ? (const T*)ctx->d_K0f : (const T*)ctx->d_K0;
+                k_pcg_k1_q4<T><<<g1, kBlock, 0, ctx->stream>>>(ctx->nx, ctx->ny, k, pold, z, pnew, q, Ev, Kv,
+                                                             ctx->d_fix, partB, (int)g2, partA, S, eps);
+            }
+            k_pcg_k2<T><<<g2, kBlock, 0, ctx->stream>>>(n, k, x, r, z, pnew, q, D, partA, (int)g1, partB, S);
+        }
+        PF_TRY(check_launch(ctx));
+        PF_HIP(hipMemcpyAsync((void*)&hflag[slot], &S->done, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        PF_HIP(hipEventRecord(ev[slot], ctx->stream));
+        return PF_OK;
+    };
+    while (!done && k < itrmax) {
+        PF_TRY(enqueue_chunk(std::min(itrmax, k + chunk)));
+        if (pending) {
+            PF_HIP(hipEventSynchronize(ev[slot ^ 1]));
+            if (hflag[slot ^ 1]) done = true;
+        }
+        pending = 1;
+        slot ^= 1;
+    }
+    // drain
+    PF_HIP(hipStreamSynchronize(ctx->stream));
+    hipEventDestroy(ev[0]);
+    hipEventDestroy(ev[1]);
+    k_pcg_final<<<1, kBlock, 0, ctx->stream>>>(k, partB, (int)g2, S, eps);
+    PF_TRY(check_launch(ctx));
+    PF_HIP(hipMemcpyAsync(ctx->h_pcg, S, sizeof(PcgDev), hipMemcpyDeviceToHost, ctx->stream));
+    PF_TRY(sync(ctx));
+    const PcgDev* H = (const PcgDev*)ctx->h_pcg;
+    out.iterations = H->iterations;
+    out.status = H->status;
+    out.rnorm = H->rnorm;
+    out.bnorm = H->bnorm;
+    return PF_OK;
+}
+
+double reduce_host(pf_ctx* ctx, const double* part, int nparts, double scale, double shift, int* rc) {
+    k_reduce_final<<<1, kBlock, 0, ctx->stream>>>(part, nparts, 1, scale, shift, ctx->red.out);
+    hipMemcpyAsync(ctx->h_red + 8, ctx->red.out, sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+        ctx->err = std::string("reduce: ") + hipGetErrorString(e);
+        *rc = PF_ERR_HIP;
+        return 0.0;
+    }
+    *rc = PF_OK;
+    return ctx->h_red[8];
+}
+
+// True FP64 residual ||b - K x|| of nodal vectors (EbE operator).
+int true_residual(pf_ctx* ctx, const double* b, const double* x, double* r, double* out_norm) {
+    const unsigned g = gsize(ctx->nn);
+    if (!is2d(ctx)) {
+        ctx->err = "3D residual not available";
+        return PF_ERR_UNSUPPORTED;
+    }
+    k_residual_q4<<<g, kBlock, 0, ctx->stream>>>(ctx->nx, ctx->ny, b, x, r, ctx->d_E, ctx->d_K0, ctx->d_fix,
+                                                ctx->red.partials);
+    PF_TRY(check_launch(ctx));
+    int rc;
+    double rr = reduce_host(ctx, ctx->red.partials, (int)g, 1.0, 0.0, &rc);
+    PF_TRY(rc);
+    *out_norm = std::sqrt(rr);
+    return PF_OK;
+}
+
+double norm_nodal(pf_ctx* ctx, const double* a, int* rc) {
+    const unsigned g = gsize(ctx->nv);
+    k_dot_partials<<<g, kBlock, 0, ctx->stream>>>(ctx->nv, a, a, ctx->red.partials);
+    return std::sqrt(reduce_host(ctx, ctx->red.partials, (int)g, 1.0, 0.0, rc));
+}
+
+// Solve K x = b for nodal device vectors b, x with the selected operator.
+int solve_nodal(pf_ctx* ctx, int op, const double* b, double* x, double eps, int itrmax, double inner_eps,
+                pf_solve_stats* st) {
+    auto t0 = std::chrono::steady_clock::now();
+    pf_solve_stats s{};
+    {
+        int rc0;
+        double bn0 = norm_nodal(ctx, b, &rc0);
+        PF_TRY(rc0);
+        if (bn0 == 0.0) {
+            // b = 0: x = 0 is exact (the reference would divide 0 by 0 here)
+            PF_HIP(hipMemsetAsync(x, 0, sizeof(double) * ctx->nv, ctx->stream));
+            s.status = 1;
+            PF_TRY(sync(ctx));
+            s.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            if (st) *st = s;
+            return PF_OK;
+        }
+    }
+    if (op == PF_OP_EBE64) {
+        PF_TRY(diag_nodal(ctx, false));
+        PcgRun run;
+        PF_TRY(pcg_device<double>(ctx, false, b, x, ctx->d_D, ctx->d_r, ctx->d_z, ctx->d_p0, ctx->d_p1, ctx->d_q, eps,
+                                  itrmax, run));
+        s.iterations = run.iterations;
+        s.status = run.status;
+        s.rel_residual = run.bnorm > 0 ? run.rnorm / run.bnorm : 0.0;
+    } else if (op == PF_OP_CSR64) {
+        PF_CHECK(ctx->nnz >= 0, PF_ERR_STATE, "no assembled CSR: call pf_assemble_csr first");
+        // free-numbered work vectors live in the nodal buffers' storage (N <= nv)
+        double *bf = ctx->d_t0, *xf = ctx->d_t1;
+        PF_TRY(to_free(ctx, b, bf));
+        k_csr_diag<<<gsize(ctx->N), kBlock, 0, ctx->stream>>>(ctx->N, ctx->d_indptr, ctx->d_indices, ctx->d_data,
+                                                             ctx->d_D);
+        PF_TRY(check_launch(ctx));
+        PcgRun run;
+        PF_TRY(pcg_device<double>(ctx, true, bf, xf, ctx->d_D, ctx->d_r, ctx->d_z, ctx->d_p0, ctx->d_p1, ctx->d_q, eps,
+                                  itrmax, run));
+        PF_TRY(to_nodal(ctx, xf, x));
+        s.iterations = run.iterations;
+        s.status = run.status;
+        s.rel_residual = run.bnorm > 0 ? run.rnorm / run.bnorm : 0.0;
+    } else if (op == PF_OP_EBE32) {
+        // FP32 EbE Jacobi-PCG inner solves inside FP64 iterative refinement:
+        // r = b - K x (FP64), solve K d = r in FP32 to inner_eps, x += d,
+        // until ||b - K x|| <= eps ||b||.
+        PF_TRY(diag_nodal(ctx, true));
+        int rc;
+        double bn = norm_nodal(ctx, b, &rc);
+        PF_TRY(rc);
+        PF_HIP(hipMemsetAsync(x, 0, sizeof(double) * ctx->nv, ctx->stream));
+        double rn = bn;
+        int outer = 0, total = 0, status = 2;
+        const int max_outer = 60;
+        double* r64 = ctx->d_t0;
+        while (outer < max_outer) {
+            PF_TRY(true_residual(ctx, b, x, r64, &rn));
+            if (bn == 0.0 || rn <= eps * bn) {
+                status = 1;
+                break;
+            }
+            if (total >= itrmax) break;
+            k_convert<double, float><<<gsize(ctx->nv), kBlock, 0, ctx->stream>>>(ctx->nv, r64, ctx->f_b);
+            PcgRun run;
+            PF_TRY(pcg_device<float>(ctx, false, ctx->f_b, ctx->f_x, ctx->f_D, ctx->f_r, ctx->f_z, ctx->f_p0,
+                                     ctx->f_p1, ctx->f_q, inner_eps, std::max(1, itrmax - total), run));
+            total += run.iterations > 0 ? run.iterations : itrmax;
+            k_axpy_f32<<<gsize(ctx->nv), kBlock, 0, ctx->stream>>>(ctx->nv, ctx->f_x, x);
+            PF_TRY(check_launch(ctx));
+            outer++;
+        }
+        s.iterations = total;
+        s.status = status;
+        s.outer = outer;
+        s.rel_residual = bn > 0 ? rn / bn : 0.0;
+    } else {
+        ctx->err = "unknown operator";
+        return PF_ERR_ARG;
+    }
+    if (is2d(ctx)) {
+        double trn;
+        int rc;
+        PF_TRY(true_residual(ctx, b, x, ctx->d_t0, &trn));
+        double bn = norm_nodal(ctx, b, &rc);
+        PF_TRY(rc);
+        s.true_residual = bn > 0 ? trn / bn : 0.0;
+    }
+    PF_TRY(sync(ctx));
+    s.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (st) *st = s;
+    return s.status == 1 ? PF_OK : PF_NOT_CONVERGED;
+}
+
+// ------------------------------------------------------------------ OC ----
+int oc_device(pf_ctx* ctx, const pf_oc_params* prm, const double* x, const double* dfdx, const double* dgdx,
+              double* xplus, double* lambda_out, int* steps_out) {
+    PF_CHECK(ctx->have_filter, PF_ERR_STATE, "no filter: call pf_filter_setup first");
+    OcDev* S = (OcDev*)ctx->d_oc;
+    OcDev init{};
+    init.l0i = prm->lambdamin;
+    init.l1i = prm->lambdamax;
+    PF_HIP(hipMemcpyAsync(S, &init, sizeof(OcDev), hipMemcpyHostToDevice, ctx->stream));
+    PF_HIP(hipMemsetAsync(xplus, 0, sizeof(double) * ctx->ne, ctx->stream));  // xkp1 starts as zeros (OC.h:81)
+    const unsigned g = gsize(ctx->ne);
+    const int max_steps = prm->max_steps > 0 ? prm->max_steps : 200;
+    const int chunk = 16;
+    int k = 0;
+    bool done = false;
+    volatile OcDev* H = (volatile OcDev*)ctx->h_oc;
+    while (!done && k < max_steps) {
+        int kend = std::min(max_steps, k + chunk);
+        for (; k < kend; k++) {
+            k_oc_xplus<<<g, kBlock, 0, ctx->stream>>>(ctx->ne, k, x, dfdx, dgdx, prm->iota, prm->movelimit,
+                                                     prm->lambdaeps, prm->scale1, ctx->red.partials, (int)g, xplus, S);
+            k_oc_volume<<<g, kBlock, 0, ctx->stream>>>(ctx->nx, ctx->ny, ctx->win, ctx->R, ctx->d_cg, xplus, prm->vf,
+                                                      prm->scale1, ctx->red.partials, S);
+        }
+        PF_TRY(check_launch(ctx));
+        PF_HIP(hipMemcpyAsync((void*)H, S, sizeof(OcDev), hipMemcpyDeviceToHost, ctx->stream));
+        PF_TRY(sync(ctx));
+        done = H->done != 0;
+    }
+    if (!done) {
+        // the last step's decision has not been applied yet; apply it once more
+        k_oc_xplus<<<g, kBlock, 0, ctx->stream>>>(ctx->ne, k, x, dfdx, dgdx, prm->iota, prm->movelimit, prm->lambdaeps,
+                                                 prm->scale1, ctx->red.partials, (int)g, xplus, S);
+        PF_HIP(hipMemcpyAsync((void*)H, S, sizeof(OcDev), hipMemcpyDeviceToHost, ctx->stream));
+        PF_TRY(sync(ctx));
+        if (!H->done) {
+            ctx->err = "OC bisection did not reach lambdaeps within max_steps";
+            return PF_NOT_CONVERGED;
+        }
+    }
+    if (lambda_out) *lambda_out = H->lambda_last;
+    if (steps_out) *steps_out = H->steps;
+    return PF_OK;
+}
+
+int filter_device(pf_ctx* ctx, const double* in, double* out) {
+    PF_CHECK(ctx->have_filter, PF_ERR_STATE, "no filter: call pf_filter_setup first");
+    if (!is2d(ctx)) {
+        ctx->err = "3D filter not available";
+        return PF_ERR_UNSUPPORTED;
+    }
+    k_filter_q4<<<gsize(ctx->ne), kBlock, 0, ctx->stream>>>(ctx->nx, ctx->ny, ctx->win, ctx->R, ctx->d_cg, in, out);
+    return check_launch(ctx);
+}
+
+int moduli_from_rho(pf_ctx* ctx) {
+    k_E_from_rho<<<gsize(ctx->ne), kBlock, 0, ctx->stream>>>(ctx->ne, ctx->d_rho, ctx->E1, ctx->E0, ctx->p, ctx->d_E,
+                                                            ctx->d_Ef);
+    PF_TRY(check_launch(ctx));
+    ctx->have_E = true;
+    return PF_OK;
+}
+
+// f = scale0 sum_e E_e u_e^T K0 u_e and dfdrho (device, nodal u).
+int comp_sens_device(pf_ctx* ctx, const double* u, double scale0, double* dfdrho, double* f) {
+    if (!is2d(ctx)) {
+        ctx->err = "3D compliance not available";
+        return PF_ERR_UNSUPPORTED;
+    }
+    const double coef = ((-scale0) * ctx->p) * ((-ctx->E0) + ctx->E1);
+    const unsigned g = gsize(ctx->ne);
+    k_comp_sens_q4<<<g, kBlock, 0, ctx->stream>>>(ctx->nx, ctx->ny, u, ctx->d_E, ctx->d_rho, ctx->d_K0, coef,
+                                                 ctx->p - 1.0, dfdrho, ctx->red.partials);
+    PF_TRY(check_launch(ctx));
+    int rc;
+    *f = reduce_host(ctx, ctx->red.partials, (int)g, scale0, 0.0, &rc);
+    return rc;
+}
+
+}  // namespace
+
+// =================================================================== ABI ==
+extern "C" {
+
+const char* pf_build_info(void) {
+    return "pansfem2_amd: gfx950, hipcc -O3, FP64 kernels without contraction, packed-fp32 disabled";
+}
+
+int pf_create(pf_ctx** out, int device) {
+    if (!out) return PF_ERR_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return PF_ERR_HIP;
+    if (device < 0 || device >= ndev) return PF_ERR_ARG;
+    pf_ctx* ctx = new pf_ctx();
+    ctx->device = device;
+    int rc = set_device(ctx);
+    if (rc == PF_OK && hipStreamCreate(&ctx->stream) != hipSuccess) rc = PF_ERR_HIP;
+    if (rc == PF_OK) rc = dalloc(ctx, &ctx->red.partials, 4L * kMaxPartials);
+    if (rc == PF_OK) rc = dalloc(ctx, &ctx->red.counter, 1);
+    if (rc == PF_OK) rc = dalloc(ctx, &ctx->red.out, 16);
+    if (rc == PF_OK && hipMalloc((void**)&ctx->d_pcg, sizeof(PcgDev)) != hipSuccess) rc = PF_ERR_NOMEM;
+    if (rc == PF_OK && hipMalloc((void**)&ctx->d_oc, sizeof(OcDev)) != hipSuccess) rc = PF_ERR_NOMEM;
+    if (rc == PF_OK && hipHostMalloc((void**)&ctx->h_pcg, sizeof(PcgDev)) != hipSuccess) rc = PF_ERR_NOMEM;
+    if (rc == PF_OK && hipHostMalloc((void**)&ctx->h_oc, sizeof(OcDev)) != hipSuccess) rc = PF_ERR_NOMEM;
+    if (rc == PF_OK && hipHostMalloc((void**)&ctx->h_red, sizeof(double) * 16) != hipSuccess) rc = PF_ERR_NOMEM;
+    if (rc == PF_OK && hipMemset(ctx->d_pcg, 0, sizeof(PcgDev)) != hipSuccess) rc = PF_ERR_HIP;
+    if (rc != PF_OK) {
+        pf_destroy(ctx);
+        return rc;
+    }
+    *out = ctx;
+    return PF_OK;
+}
+
+int pf_destroy(pf_ctx* ctx) {
+    if (!ctx) return PF_OK;
+    hipSetDevice(ctx->device);
+    if (ctx->stream) hipStreamSynchronize(ctx->stream);
+    free_grid(ctx);
+    dfree(ctx->d_K0);
+    dfree(ctx->d_K0f);
+    dfree(ctx->red.partials);
+    dfree(ctx->red.counter);
+    dfree(ctx->red.out);
+    if (ctx->d_pcg) hipFree(ctx->d_pcg);
+    if (ctx->d_oc) hipFree(ctx->d_oc);
+    if (ctx->h_pcg) hipHostFree(ctx->h_pcg);
+    if (ctx->h_oc) hipHostFree(ctx->h_oc);
+    if (ctx->h_red) hipHostFree(ctx->h_red);
+    if (ctx->stream) hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return PF_OK;
+}
+
+const char* pf_last_error(pf_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int pf_grid2d(pf_ctx* ctx, int nx, int ny, double lx, double ly) {
+    if (!ctx) return PF_ERR_ARG;
+    PF_CHECK(nx >= 1 && ny >= 1 && lx > 0 && ly > 0, PF_ERR_ARG, "pf_grid2d: nx, ny >= 1 and lx, ly > 0 required");
+    PF_CHECK((long)(nx + 1) * (ny + 1) * 2 < (1L << 31), PF_ERR_ARG, "pf_grid2d: grid exceeds int32 DOF numbering");
+    PF_TRY(set_device(ctx));
+    free_grid(ctx);
+    ctx->dim = 2;
+    ctx->nx = nx; ctx->ny = ny; ctx->nz = 0;
+    ctx->lx = lx; ctx->ly = ly; ctx->lz = 0;
+    ctx->nn = (long)(nx + 1) * (ny + 1);
+    ctx->ne = (long)nx * ny;
+    ctx->ndpn = 2;
+    ctx->nd = 8;
+    ctx->nv = ctx->nn * 2;
+    PF_TRY(alloc_grid(ctx));
+    k_centroids_q4<<<gsize(ctx->ne), kBlock, 0, ctx->stream>>>(nx, ny, lx, ly, ctx->d_cg);
+    PF_TRY(check_launch(ctx));
+    return sync(ctx);
+}
+
+int pf_grid3d(pf_ctx* ctx, int nx, int ny, int nz, double lx, double ly, double lz) {
+    if (!ctx) return PF_ERR_ARG;
+    ctx->err = "pf_grid3d: hex8 grids are not available in this build";
+    return PF_ERR_UNSUPPORTED;
+}
+
+int pf_set_dirichlet(pf_ctx* ctx, const int32_t* node, const int32_t* dof, int nfixed, int32_t* n_free) {
+    if (!ctx) return PF_ERR_ARG;
+    PF_CHECK(ctx->dim > 0, PF_ERR_STATE, "no grid");
+    PF_CHECK(nfixed == 0 || (node && dof), PF_ERR_ARG, "null fixed list");
+    std::vector<uint8_t> fix(ctx->nn, 0);
+    for (int k = 0; k < nfixed; k++) {
+        PF_CHECK(node[k] >= 0 && node[k] < ctx->nn && dof[k] >= 0 && dof[k] < ctx->ndpn, PF_ERR_ARG,
+                 "pf_set_dirichlet: node/dof out of range");
+        fix[node[k]] |= (uint8_t)(1u << dof[k]);
+    }
+    PF_TRY(set_device(ctx));
+    PF_TRY(upload_dofmap(ctx, fix));
+    if (n_free) *n_free = ctx->N;
+    return PF_OK;
+}
+
+int pf_get_dofmap(pf_ctx* ctx, int32_t* n2g) {
+    if (!ctx || !n2g) return PF_ERR_ARG;
+    PF_CHECK(!ctx->h_n2g.empty(), PF_ERR_STATE, "no DOF map");
+    std::memcpy(n2g, ctx->h_n2g.data(), sizeof(int32_t) * ctx->h_n2g.size());
+    return PF_OK;
+}
+
+int pf_set_point_loads(pf_ctx* ctx, const int32_t* node, const int32_t* dof, const double* value, int n) {
+    if (!ctx) return PF_ERR_ARG;
+    PF_CHECK(ctx->N > 0, PF_ERR_STATE, "no DOF map");
+    std::vector<double> F(ctx->nv, 0.0);
+    for (int k = 0; k < n; k++) {
+        PF_CHECK(node[k] >= 0 && node[k] < ctx->nn && dof[k] >= 0 && dof[k] < ctx->ndpn, PF_ERR_ARG,
+                 "pf_set_point_loads: node/dof out of range");
+        long idx = (long)node[k] * ctx->ndpn + dof[k];
+        if (ctx->h_n2g[idx] != -1) F[idx] += value[k];  // Assembling.h:151-158
+    }
+    PF_TRY(set_device(ctx));
+    PF_HIP(hipMemcpyAsync(ctx->d_load, F.data(), sizeof(double) * ctx->nv, hipMemcpyHostToDevice, ctx->stream));
+    return sync(ctx);
+}
+
+int pf_get_load(pf_ctx* ctx, double* F_free) {
+    if (!ctx || !F_free) return PF_ERR_ARG;
+    PF_CHECK(ctx->N > 0, PF_ERR_STATE, "no DOF map");
+    PF_TRY(set_device(ctx));
+    PF_TRY(to_free(ctx, ctx->d_load, ctx->d_t0));
+    PF_HIP(hipMemcpyAsync(F_free, ctx->d_t0, sizeof(double) * ctx->N, hipMemcpyDeviceToHost, ctx->stream));
+    return sync(ctx);
+}
+
+int pf_set_element_matrix(pf_ctx* ctx, const double* K0, int nd) {
+    if (!ctx || !K0) return PF_ERR_ARG;
+    PF_CHECK(ctx->dim > 0, PF_ERR_STATE, "no grid");
+    PF_CHECK(nd == ctx->nd, PF_ERR_ARG, "pf_set_element_matrix: nd does not match the grid's element");
+    PF_TRY(set_device(ctx));
+    ctx->h_K0.assign(K0, K0 + nd * nd);
+    std::vector<float> k32(nd * nd);
+    for (int i = 0; i < nd * nd; i++) k32[i] = (float)K0[i];
+    PF_TRY(dalloc(ctx, &ctx->d_K0, nd * nd));
+    PF_TRY(dalloc(ctx, &ctx->d_K0f, nd * nd));
+    PF_HIP(hipMemcpyAsync(ctx->d_K0, K0, sizeof(double) * nd * nd, hipMemcpyHostToDevice, ctx->stream));
+    PF_HIP(hipMemcpyAsync(ctx->d_K0f, k32.data(), sizeof(float) * nd * nd, hipMemcpyHostToDevice, ctx->stream));
+    return sync(ctx);
+}
+
+int pf_element_stiffness(pf_ctx* ctx, int e, double E, double nu, double t, double* Ke) {
+    if (!ctx || !Ke) return PF_ERR_ARG;
+    PF_CHECK(ctx->dim == 2, PF_ERR_UNSUPPORTED, "pf_element_stiffness: quad4 grids only in this build");
+    PF_CHECK(e >= 0 && e < ctx->ne, PF_ERR_ARG, "element id out of range");
+    PF_TRY(set_device(ctx));
+    double* dK;
+    PF_HIP(hipMalloc((void**)&dK, sizeof(double) * 64 * 1));
+    // compute only element e: one workgroup, one active thread
+    k_ke_one_q4<<<1, 64, 0, ctx->stream>>>(ctx->nx, ctx->ny, ctx->lx, ctx->ly, e, E, nu, t, dK);
+    hipError_t le = hipGetLastError();
+    hipError_t ce = hipMemcpyAsync(Ke, dK, sizeof(double) * 64, hipMemcpyDeviceToHost, ctx->stream);
+    hipError_t se = hipStreamSynchronize(ctx->stream);
+    hipFree(dK);
+    PF_HIP(le);
+    PF_HIP(ce);
+    PF_HIP(se);
+    return PF_OK;
+}
+
+int pf_set_material(pf_ctx* ctx, double E1, double E0, double p, double nu, double t) {
+    if (!ctx) return PF_ERR_ARG;
+    ctx->E1 = E1; ctx->E0 = E0; ctx->p = p; ctx->nu = nu; ctx->t = t;
+    return PF_OK;
+}
+
+int pf_set_density(pf_ctx* ctx, const double* rho) {
+    if (!ctx || !rho) return PF_ERR_ARG;
+    PF_CHECK(ctx->dim > 0, PF_ERR_STATE, "no grid");
+    for (long e = 0; e < ctx->ne; e++)
+        PF_CHECK(std::isfinite(rho[e]), PF_ERR_NUMERIC, "pf_set_density: non-finite density");
+    PF_TRY(set_device(ctx));
+    PF_HIP(hipMemcpyAsync(ctx->d_rho, rho, sizeof(double) * ctx->ne, hipMemcpyHostToDevice, ctx->stream));
+    PF_TRY(moduli_from_rho(ctx));
+    ctx->nnz = ctx->nnz >= 0 ? -2 : ctx->nnz;  // values out of date
+    return sync(ctx);
+}
+
+int pf_set_moduli(pf_ctx* ctx, const double* E) {
+    if (!ctx || !E) return PF_ERR_ARG;
+    PF_CHECK(ctx->dim > 0, PF_ERR_STATE, "no grid");
+    for (long e = 0; e < ctx->ne; e++)
+        PF_CHECK(std::isfinite(E[e]), PF_ERR_NUMERIC, "pf_set_moduli: non-finite modulus");
+    PF_TRY(set_device(ctx));
+    PF_HIP(hipMemcpyAsync(ctx->d_E, E, sizeof(double) * ctx->ne, hipMemcpyHostToDevice, ctx->stream));
+    k_convert<double, float><<<gsize(ctx->ne), kBlock, 0, ctx->stream>>>(ctx->ne, ctx->d_E, ctx->d_Ef);
+    PF_TRY(check_launch(ctx));
+    ctx->have_E = true;
+    return sync(ctx);
+}
+
+int pf_assemble_csr(pf_ctx* ctx, int64_t* nnz) {
+    if (!ctx) return PF_ERR_ARG;
+    PF_TRY(require_ready(ctx, true));
+    PF_CHECK(is2d(ctx), PF_ERR_UNSUPPORTED, "pf_assemble_csr: quad4 grids only in this build");
+    PF_TRY(set_device(ctx));
+    const int N = ctx->N;
+    PF_TRY(dalloc(ctx, &ctx->d_indptr, N + 1));
+    int32_t* rowlen = nullptr;
+    PF_TRY(dalloc(ctx, &rowlen, N));
+    k_csr_count_q4<<<gsize(ctx->nn), kBlock, 0, ctx->stream>>>(ctx->nx, ctx->ny, ctx->d_n2g, ctx->d_fix, rowlen);
+    k_scan<<<1, 1024, 0, ctx->stream>>>(rowlen, ctx->d_indptr, N);
+    int32_t total = 0;
+    hipError_t e1 = hipGetLastError();
+    hipError_t e2 = hipMemcpyAsync(&total, ctx->d_indptr + N, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream);
+    hipError_t e3 = hipStreamSynchronize(ctx->stream);
+    hipFree(rowlen);
+    PF_HIP(e1);
+    PF_HIP(e2);
+    PF_HIP(e3);
+    ctx->nnz = total;
+    PF_TRY(dalloc(ctx, &ctx->d_indices, std::max<long>(total, 1)));
+    PF_TRY(dalloc(ctx, &ctx->d_data, std::max<long>(total, 1)));
+    PF_TRY(dalloc(ctx, &ctx->d_ke_all, ctx->ne * 64));
+    k_ke_all_q4<<<gsize(ctx->ne), kBlock, 0, ctx->stream>>>(ctx->nx, ctx->ny, ctx->lx, ctx->ly, ctx->d_E, ctx->nu,
+                                                           ctx->t, ctx->d_ke_all);
+    k_csr_fill_q4<<<gsize(ctx->nn), kBlock, 0, ctx->stream>>>(ctx->nx, ctx->ny, ctx->d_n2g, ctx->d_indptr,
+                                                             ctx->d_ke_all, ctx->d_indices, ctx->d_data);
+    PF_TRY(check_launch(ctx));
+    PF_TRY(sync(ctx));
+    dfree(ctx->d_ke_all);
+    if (nnz) *nnz = total;
+    return PF_OK;
+}
+
+int pf_get_csr(pf_ctx* ctx, int32_t* indptr, int32_t* indices, double* data) {
+    if (!ctx) return PF_ERR_ARG;
+    PF_CHECK(ctx->nnz >= 0, PF_ERR_STATE, "no assembled CSR");
+    PF_TRY(set_device(ctx));
+    if (indptr)
+        PF_HIP(hipMemcpyAsync(indptr, ctx->d_indptr, sizeof(int32_t) * (ctx->N + 1), hipMemcpyDeviceToHost, ctx->stream));
+    if (indices)
+        PF_HIP(hipMemcpyAsync(indices, ctx->d_indices, sizeof(int32_t) * ctx->nnz, hipMemcpyDeviceToHost, ctx->stream));
+    if (data) PF_HIP(hipMemcpyAsync(data, ctx->d_data, sizeof(double) * ctx->nnz, hipMemcpyDeviceToHost, ctx->stream));
+    return sync(ctx);
+}
+
+int pf_apply_K(pf_ctx* ctx, int op, const double* x_free, double* y_free) {
+    if (!ctx || !x_free || !y_free) return PF_ERR_ARG;
+    PF_TRY(require_ready(ctx, true));
+    PF_TRY(set_device(ctx));
+    const int N = ctx->N;
+    PF_HIP(hipMemcpyAsync(ctx->d_t1, x_free, sizeof(double) * N, hipMemcpyHostToDevice, ctx->stream));
+    if (op == PF_OP_CSR64) {
+        PF_CHECK(ctx->nnz >= 0, PF_ERR_STATE, "no assembled CSR: call pf_assemble_csr first");
+        k_csr_spmv<<<gsize(N), kBlock, 0, ctx->stream>>>(N, ctx->d_indptr, ctx->d_indices, ctx->d_data, ctx->d_t1,
+                                                        ctx->d_t0);
+        PF_TRY(check_launch(ctx));
+    } else if (op == PF_OP_EBE64) {
+        PF_TRY(to_nodal(ctx, ctx->d_t1, ctx->d_x));
+        PF_TRY(apply_nodal(ctx, false, ctx->d_x, ctx->d_q, true));
+        PF_TRY(to_free(ctx, ctx->d_q, ctx->d_t0));
+    } else if (op == PF_OP_EBE32) {
+        PF_TRY(to_nodal(ctx, ctx->d_t1, ctx->d_x));
+        k_convert<double, float><<<gsize(ctx->nv), kBlock, 0, ctx->stream>>>(ctx->nv, ctx->d_x, ctx->f_x);
+        PF_TRY(apply_nodal(ctx, true, ctx->f_x, ctx->f_q, true));
+        k_convert<float, double><<<gsize(ctx->nv), kBlock, 0, ctx->stream>>>(ctx->nv, ctx->f_q, ctx->d_q);
+        PF_TRY(to_free(ctx, ctx->d_q, ctx->d_t0));
+    } else {
+        ctx->err = "unknown operator";
+        return PF_ERR_ARG;
+    }
+    PF_HIP(hipMemcpyAsync(y_free, ctx->d_t0, sizeof(double) * N, hipMemcpyDeviceToHost, ctx->stream));
+    return sync(ctx);
+}
+
+int pf_pcg_dev(pf_ctx* ctx, int op, const double* b_free_dev, double* x_free_dev, double eps, int itrmax,
+               pf_solve_stats* st) {
+    if (!ctx || !b_free_dev || !x_free_dev) return PF_ERR_ARG;
+    PF_TRY(require_ready(ctx, true));
+    PF_CHECK(itrmax >= 0 && eps >= 0, PF_ERR_ARG, "pf_pcg: itrmax and eps must be non-negative");
+    PF_TRY(set_device(ctx));
+    PF_TRY(to_nodal(ctx, b_free_dev, ctx->d_b));
+    int rc = solve_nodal(ctx, op, ctx->d_b, ctx->d_x, eps, itrmax, 1e-5, st);
+    if (rc < 0) return rc;
+    PF_TRY(to_free(ctx, ctx->d_x, x_free_dev));
+    PF_TRY(sync(ctx));
+    return rc;
+}
+
+int pf_pcg(pf_ctx* ctx, int op, const double* b_free, double* x_free, double eps, int itrmax, pf_solve_stats* st) {
+    if (!ctx || !b_free || !x_free) return PF_ERR_ARG;
+    PF_TRY(require_ready(ctx, true));
+    for (int i = 0; i < ctx->N; i++) PF_CHECK(std::isfinite(b_free[i]), PF_ERR_NUMERIC, "pf_pcg: non-finite b");
+    PF_TRY(set_device(ctx));
+    double *bd = nullptr, *xd = nullptr;
+    PF_HIP(hipMalloc((void**)&bd, sizeof(double) * ctx->N));
+    PF_HIP(hipMalloc((void**)&xd, sizeof(double) * ctx->N));
+    int rc = PF_OK;
+    if (hipMemcpyAsync(bd, b_free, sizeof(double) * ctx->N, hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
+        rc = PF_ERR_HIP;
+    if (rc == PF_OK) rc = pf_pcg_dev(ctx, op, bd, xd, eps, itrmax, st);
+    if (rc >= 0 && hipMemcpyAsync(x_free, xd, sizeof(double) * ctx->N, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess)
+        rc = PF_ERR_HIP;
+    hipStreamSynchronize(ctx->stream);
+    hipFree(bd);
+    hipFree(xd);
+    return rc;
+}
+
+int pf_compliance_sens(pf_ctx* ctx, const double* u_free, double scale0, double* f, double* dfdrho) {
+    if (!ctx || !u_free || !f || !dfdrho) return PF_ERR_ARG;
+    PF_TRY(require_ready(ctx, true));
+    PF_TRY(set_device(ctx));
+    PF_HIP(hipMemcpyAsync(ctx->d_t1, u_free, sizeof(double) * ctx->N, hipMemcpyHostToDevice, ctx->stream));
+    PF_TRY(to_nodal(ctx, ctx->d_t1, ctx->d_x));
+    PF_TRY(comp_sens_device(ctx, ctx->d_x, scale0, ctx->d_e0, f));
+    PF_HIP(hipMemcpyAsync(dfdrho, ctx->d_e0, sizeof(double) * ctx->ne, hipMemcpyDeviceToHost, ctx->stream));
+    return sync(ctx);
+}
+
+int pf_filter_setup(pf_ctx* ctx, double R) {
+    if (!ctx) return PF_ERR_ARG;
+    PF_CHECK(ctx->dim > 0, PF_ERR_STATE, "no grid");
+    PF_CHECK(R > 0 && std::isfinite(R), PF_ERR_ARG, "pf_filter_setup: R must be positive");
+    double h = std::min(ctx->lx / ctx->nx, ctx->ly / ctx->ny);
+    if (ctx->dim == 3) h = std::min(h, ctx->lz / ctx->nz);
+    ctx->R = R;
+    ctx->win = (int)std::ceil(R / h) + 1;
+    ctx->have_filter = true;
+    return PF_OK;
+}
+
+int pf_filter(pf_ctx* ctx, int mode, const double* in, double* out) {
+    if (!ctx || !in || !out) return PF_ERR_ARG;
+    PF_CHECK(mode == PF_FILTER_VARIABLES || mode == PF_FILTER_SENSITIVITIES, PF_ERR_ARG, "pf_filter: bad mode");
+    PF_TRY(set_device(ctx));
+    PF_HIP(hipMemcpyAsync(ctx->d_e0, in, sizeof(double) * ctx->ne, hipMemcpyHostToDevice, ctx->stream));
+    // both modes are the same row-normalised stencil (DensityFilter.h:45-71)
+    PF_TRY(filter_device(ctx, ctx->d_e0, ctx->d_e1));
+    PF_HIP(hipMemcpyAsync(out, ctx->d_e1, sizeof(double) * ctx->ne, hipMemcpyDeviceToHost, ctx->stream));
+    return sync(ctx);
+}
+
+int pf_oc_update(pf_ctx* ctx, double* x, const double* dfdx, const double* dgdx, const pf_oc_params* prm,
+                 double* lambda_out, int* steps_out) {
+    if (!ctx || !x || !dfdx || !dgdx || !prm) return PF_ERR_ARG;
+    PF_CHECK(ctx->dim > 0, PF_ERR_STATE, "no grid");
+    PF_TRY(set_device(ctx));
+    size_t bytes = sizeof(double) * ctx->ne;
+    PF_HIP(hipMemcpyAsync(ctx->d_e0, x, bytes, hipMemcpyHostToDevice, ctx->stream));
+    PF_HIP(hipMemcpyAsync(ctx->d_e1, dfdx, bytes, hipMemcpyHostToDevice, ctx->stream));
+    PF_HIP(hipMemcpyAsync(ctx->d_e2, dgdx, bytes, hipMemcpyHostToDevice, ctx->stream));
+    int rc = oc_device(ctx, prm, ctx->d_e0, ctx->d_e1, ctx->d_e2, ctx->d_e3, lambda_out, steps_out);
+    if (rc < 0) return rc;
+    PF_HIP(hipMemcpyAsync(x, ctx->d_e3, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    PF_TRY(sync(ctx));
+    return rc;
+}
+
+int pf_mma_init(pf_ctx* ctx, const pf_mma_params* prm) {
+    if (!ctx) return PF_ERR_ARG;
+    ctx->err = "MMA is not available in this build";
+    return PF_ERR_UNSUPPORTED;
+}
+
+int pf_mma_update(pf_ctx* ctx, double* x, double f, const double* dfdx, double g, const double* dgdx,
+                  int* newton_steps) {
+    if (!ctx) return PF_ERR_ARG;
+    ctx->err = "MMA is not available in this build";
+    return PF_ERR_UNSUPPORTED;
+}
+
+void pf_simp_default_params(pf_simp_params* p) {
+    if (!p) return;
+    std::memset(p, 0, sizeof(*p));
+    p->E1 = 1.0; p->E0 = 1e-9; p->nu = 0.3; p->t = 1.0; p->p = 3.0;
+    p->vf = 0.5; p->R = 1.5; p->scale0 = 1.0; p->scale1 = 1.0;
+    p->cg_eps = 1e-10; p->cg_itrmax = 100000;
+    p->use_mma = 0; p->solver = PF_OP_EBE64; p->mixed = 0; p->inner_eps = 1e-5; p->learned = 0;
+    p->stop_on_convergence = 0;
+    p->oc.iota = 0.5; p->oc.lambdamin = 0.0; p->oc.lambdamax = 1e9; p->oc.lambdaeps = 1e-3; p->oc.movelimit = 0.2;
+    p->oc.vf = 0.5; p->oc.scale1 = 1.0; p->oc.max_steps = 200;
+    p->mma.m = 1; p->mma.a0 = 1.0; p->mma.a = 0.0; p->mma.c = 1e4; p->mma.d = 0.0;
+    p->mma.xmin = 0.01; p->mma.xmax = 1.0; p->mma.raa0 = 1e-5; p->mma.albefa = 0.1; p->mma.move = 0.2;
+    p->mma.asyinit = 0.5; p->mma.asydecr = 0.7; p->mma.asyincr = 1.2; p->mma.epsvalue = 1e-6; p->mma.max_newton = 500;
+}
+
+int pf_simp_run(pf_ctx* ctx, const pf_simp_params* prm, int iters, double* s, double* rho_out, pf_simp_history* hist) {
+    if (!ctx || !prm || !s) return PF_ERR_ARG;
+    PF_TRY(require_ready(ctx, false));
+    PF_CHECK(iters >= 0, PF_ERR_ARG, "iters must be non-negative");
+    PF_CHECK(!prm->use_mma, PF_ERR_UNSUPPORTED, "MMA is not available in this build");
+    PF_CHECK(!prm->learned, PF_ERR_UNSUPPORTED, "learned preconditioner is not available in this build");
+    PF_TRY(set_device(ctx));
+    ctx->E1 = prm->E1; ctx->E0 = prm->E0; ctx->p = prm->p; ctx->nu = prm->nu; ctx->t = prm->t;
+    if (!ctx->have_filter || ctx->R != prm->R) PF_TRY(pf_filter_setup(ctx, prm->R));
+    const long ne = ctx->ne;
+    const size_t eb = sizeof(double) * ne;
+    double *d_s = ctx->d_e0, *d_dfdrho = ctx->d_e1, *d_dfds = ctx->d_e2, *d_dgds = ctx->d_e3, *d_xplus = ctx->d_e4;
+    double* d_tmp = ctx->d_e5;
+    PF_HIP(hipMemcpyAsync(d_s, s, eb, hipMemcpyHostToDevice, ctx->stream));
+    // dgdrho is the constant scale1/(vf ne) (sample...oc.cpp:103); its filtered form is fixed
+    k_fill<double><<<gsize(ne), kBlock, 0, ctx->stream>>>(ne, d_tmp, prm->scale1 / (prm->vf * (double)ne));
+    PF_TRY(filter_device(ctx, d_tmp, d_dgds));
+    pf_oc_params oc = prm->oc;
+    oc.vf = prm->vf;
+    oc.scale1 = prm->scale1;
+    double prev_f = 0.0;
+    int it = 0;
+    for (; it < iters; it++) {
+        auto t0 = std::chrono::steady_clock::now();
+        // rho = filter(s); g = volume(rho)
+        PF_TRY(filter_device(ctx, d_s, ctx->d_rho));
+        const unsigned ge = gsize(ne);
+        k_volume_partials<<<ge, kBlock, 0, ctx->stream>>>(ne, ctx->d_rho, prm->vf, prm->scale1, ctx->red.partials);
+        int rc;
+        double g = reduce_host(ctx, ctx->red.partials, (int)ge, 1.0, 1.0 * prm->scale1, &rc);
+        PF_TRY(rc);
+        PF_TRY(moduli_from_rho(ctx));
+        // solve K u = F
+        pf_solve_stats st{};
+        int op = prm->mixed ? PF_OP_EBE32 : prm->solver;
+        if (op == PF_OP_CSR64) PF_TRY(pf_assemble_csr(ctx, nullptr));
+        int src = solve_nodal(ctx, op, ctx->d_load, ctx->d_x, prm->mixed ? std::max(prm->cg_eps, 1e-14) : prm->cg_eps,
+                              prm->cg_itrmax, prm->inner_eps, &st);
+        if (src < 0) return src;
+        // compliance, sensitivities, filtered sensitivities
+        double f;
+        PF_TRY(comp_sens_device(ctx, ctx->d_x, prm->scale0, d_dfdrho, &f));
+        PF_TRY(filter_device(ctx, d_dfdrho, d_dfds));
+        bool conv = std::fabs(f - prev_f) / (f + prev_f) < 1.0e-5;  // OC::IsConvergence (OC.h:67-73)
+        double lam = 0.0;
+        int steps = 0;
+        if (!(prm->stop_on_convergence && conv)) {
+            int orc = oc_device(ctx, &oc, d_s, d_dfds, d_dgds, d_xplus, &lam, &steps);
+            if (orc < 0) return orc;
+            PF_HIP(hipMemcpyAsync(d_s, d_xplus, eb, hipMemcpyDeviceToDevice, ctx->stream));
+        }
+        PF_TRY(sync(ctx));
+        double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        if (hist) {
+            if (hist->f) hist->f[it] = f;
+            if (hist->g) hist->g[it] = g;
+            if (hist->lambda) hist->lambda[it] = lam;
+            if (hist->cg_iters) hist->cg_iters[it] = st.iterations;
+            if (hist->bisect) hist->bisect[it] = steps;
+            if (hist->seconds) hist->seconds[it] = secs;
+        }
+        prev_f = f;
+        if (prm->stop_on_convergence && conv) {
+            it++;
+            break;
+        }
+    }
+    if (hist) hist->iterations = it;
+    PF_HIP(hipMemcpyAsync(s, d_s, eb, hipMemcpyDeviceToHost, ctx->stream));
+    if (rho_out) {
+        PF_TRY(filter_device(ctx, d_s, ctx->d_rho));
+        PF_HIP(hipMemcpyAsync(rho_out, ctx->d_rho, eb, hipMemcpyDeviceToHost, ctx->stream));
+    }
+    return sync(ctx);
+}
+
+int pf_precond_init(pf_ctx* ctx, int channels, uint64_t seed) {
+    if (!ctx) return PF_ERR_ARG;
+    ctx->err = "learned preconditioner is not available in this build";
+    return PF_ERR_UNSUPPORTED;
+}
+int pf_precond_save(pf_ctx* ctx, float* params, int64_t* count) {
+    if (!ctx) return PF_ERR_ARG;
+    ctx->err = "learned preconditioner is not available in this build";
+    return PF_ERR_UNSUPPORTED;
+}
+int pf_precond_load(pf_ctx* ctx, const float* params, int64_t count) {
+    if (!ctx) return PF_ERR_ARG;
+    ctx->err = "learned preconditioner is not available in this build";
+    return PF_ERR_UNSUPPORTED;
+}
+
+}  // extern "C"

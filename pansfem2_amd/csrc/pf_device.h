@@ -1,0 +1,89 @@
+// Device helpers shared by the kernels: deterministic reductions, the
+// structured-grid index maps and the arithmetic that has to round exactly as
+// the reference's x86-64 build does.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace pf {
+
+// ---------------------------------------------------------- reductions ----
+// Wavefront (64 lanes) butterfly sum: every lane ends with the same total and
+// the association order is fixed, so results do not depend on timing.
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Block sum of K values, valid in every thread on return.  Fixed order:
+// lanes -> waves in index order.  `lds` holds K * (blockDim/64) doubles.
+template <int K>
+__device__ inline void block_sum(double (&v)[K], double* lds) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+#pragma unroll
+    for (int k = 0; k < K; k++) v[k] = wave_sum(v[k]);
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < K; k++) lds[k * 16 + w] = v[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        double s = 0.0;
+        for (int i = 0; i < nw; i++) s += lds[k * 16 + i];
+        v[k] = s;
+    }
+    __syncthreads();
+}
+
+// Per-workgroup partial sums -> partials[blockIdx.x*K + k].
+template <int K>
+__device__ inline void store_partials(double (&v)[K], double* lds, double* partials) {
+    block_sum<K>(v, lds);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < K; k++) partials[blockIdx.x * K + k] = v[k];
+    }
+}
+
+// Every workgroup of the consuming launch reduces the producer's partials in
+// the same fixed order, so all of them obtain the same bits without atomics.
+template <int K>
+__device__ inline void reduce_partials(const double* partials, int nparts, double* lds, double (&out)[K]) {
+#pragma unroll
+    for (int k = 0; k < K; k++) out[k] = 0.0;
+    for (int i = threadIdx.x; i < nparts; i += blockDim.x) {
+#pragma unroll
+        for (int k = 0; k < K; k++) out[k] += partials[i * K + k];
+    }
+    block_sum<K>(out, lds);
+}
+
+// ----------------------------------------------------- reference maths ----
+// rho^3 rounded once (double-double product).  glibc's pow(x, 3.0), which
+// the reference calls, is within 0.52 ulp, so the two agree except in rare
+// near-halfway cases (1 ulp).
+__device__ inline double cube_rn(double x) {
+    double h = x * x;
+    double l = __fma_rn(x, x, -h);
+    double hh = h * x;
+    double e = __fma_rn(h, x, -hh);
+    return hh + __fma_rn(l, x, e);
+}
+
+// pow(rho, p) / pow(rho, p-1) as the SIMP driver evaluates them: the
+// reference's g++ build folds the exponent 2.0 to rho*rho.
+__device__ inline double simp_pow(double x, double e) {
+    if (e == 3.0) return cube_rn(x);
+    if (e == 2.0) return x * x;
+    if (e == 1.0) return x;
+    return pow(x, e);
+}
+
+__device__ inline double std_max0(double v) { return (0.0 < v) ? v : 0.0; }   // std::max(T(), v)
+__device__ inline double std_min1(double v) { return (v < 1.0) ? v : 1.0; }   // std::min(1.0, v)
+
+}  // namespace pf

@@ -1,0 +1,785 @@
+// HIP kernels (gfx950) of the SIMP loop on structured quad4 / hex8 grids.
+//
+// The whole file is compiled without floating-point contraction: every
+// a*b + c rounds twice, as the reference's x86-64 g++ build (no FMA) does,
+// so the check-path kernels (element matrices, CSR values, CSR product,
+// filter, OC step) reproduce the reference's bits.  These kernels are HBM /
+// latency bound; FMA would not make them faster.
+#pragma clang fp contract(off)
+
+#include "pf_device.h"
+#include "pf_internal.h"
+
+namespace pf {
+
+// ===================================================== grid geometry ======
+struct Grid2 {
+    int nx, ny;
+    __device__ int nnode() const { return (nx + 1) * (ny + 1); }
+};
+
+// ============================================================ moduli =====
+__global__ void k_E_from_rho(long ne, const double* __restrict__ rho, double E1, double E0, double p,
+                             double* __restrict__ E, float* __restrict__ Ef) {
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < ne; e += (long)gridDim.x * blockDim.x) {
+        double rp = simp_pow(rho[e], p);
+        double v = E1 * rp + E0 * (1.0 - rp);  // sample_optimize_density_oc.cpp:123
+        E[e] = v;
+        Ef[e] = (float)v;
+    }
+}
+
+// ====================================================== nodal <-> free ====
+__global__ void k_free_to_nodal(int N, const int32_t* __restrict__ g2n, const double* __restrict__ xf,
+                                double* __restrict__ xn) {
+    int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < N) xn[g2n[g]] = xf[g];
+}
+
+__global__ void k_nodal_to_free(int N, const int32_t* __restrict__ g2n, const double* __restrict__ xn,
+                                double* __restrict__ xf) {
+    int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < N) xf[g] = xn[g2n[g]];
+}
+
+template <class T>
+__global__ void k_fill(long n, T* __restrict__ p, T v) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) p[i] = v;
+}
+
+template <class A, class B>
+__global__ void k_convert(long n, const A* __restrict__ a, B* __restrict__ b) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) b[i] = (B)a[i];
+}
+
+// ===================================================== 2D EbE operator ====
+// Node (i,j) touches elements (i-1,j-1) [as TR, local 2], (i-1,j) [BR, 1],
+// (i,j-1) [TL, 3], (i,j) [BL, 0]; contributions are summed in that order,
+// which is ascending element id, the order Assembling adds them in.
+//
+// v[a][b] holds the nodal pair of node (i+a-1, j+b-1).
+template <class T>
+struct Vec2 { T x, y; };
+
+template <class T>
+__device__ inline void elem_rows_q4(const T* __restrict__ K0, int loc, const Vec2<T> n0, const Vec2<T> n1,
+                                    const Vec2<T> n2, const Vec2<T> n3, T& ox, T& oy) {
+    // rows 2*loc, 2*loc+1 of K0 times the element vector (BL, BR, TR, TL)
+    const T* r0 = K0 + (2 * loc) * 8;
+    const T* r1 = r0 + 8;
+    T a = 0, b = 0;
+    a += r0[0] * n0.x; a += r0[1] * n0.y; a += r0[2] * n1.x; a += r0[3] * n1.y;
+    a += r0[4] * n2.x; a += r0[5] * n2.y; a += r0[6] * n3.x; a += r0[7] * n3.y;
+    b += r1[0] * n0.x; b += r1[1] * n0.y; b += r1[2] * n1.x; b += r1[3] * n1.y;
+    b += r1[4] * n2.x; b += r1[5] * n2.y; b += r1[6] * n3.x; b += r1[7] * n3.y;
+    ox = a;
+    oy = b;
+}
+
+template <class T>
+__device__ inline void apply_q4_node(int nx, int ny, int i, int j, const Vec2<T> (&v)[3][3], const T* __restrict__ E,
+                                     const T* __restrict__ K0, T& yx, T& yy) {
+    T ax = 0, ay = 0, ex, ey;
+    if (i >= 1 && j >= 1) {  // element (i-1, j-1): this node is TR
+        T Ee = E[(long)ny * (i - 1) + (j - 1)];
+        elem_rows_q4(K0, 2, v[0][0], v[1][0], v[1][1], v[0][1], ex, ey);
+        ax += Ee * ex; ay += Ee * ey;
+    }
+    if (i >= 1 && j < ny) {  // element (i-1, j): BR
+        T Ee = E[(long)ny * (i - 1) + j];
+        elem_rows_q4(K0, 1, v[0][1], v[1][1], v[1][2], v[0][2], ex, ey);
+        ax += Ee * ex; ay += Ee * ey;
+    }
+    if (i < nx && j >= 1) {  // element (i, j-1): TL
+        T Ee = E[(long)ny * i + (j - 1)];
+        elem_rows_q4(K0, 3, v[1][0], v[2][0], v[2][1], v[1][1], ex, ey);
+        ax += Ee * ex; ay += Ee * ey;
+    }
+    if (i < nx && j < ny) {  // element (i, j): BL
+        T Ee = E[(long)ny * i + j];
+        elem_rows_q4(K0, 0, v[1][1], v[2][1], v[2][2], v[1][2], ex, ey);
+        ax += Ee * ex; ay += Ee * ey;
+    }
+    yx = ax;
+    yy = ay;
+}
+
+template <class T>
+__device__ inline Vec2<T> load2(const T* __restrict__ a, long n) {
+    Vec2<T> r;
+    r.x = a[2 * n];
+    r.y = a[2 * n + 1];
+    return r;
+}
+
+// y = K x (nodal vectors), fixed DOFs of y set to zero when mask_fixed.
+template <class T>
+__global__ void __launch_bounds__(256) k_apply_q4(int nx, int ny, const T* __restrict__ x, T* __restrict__ y,
+                                                  const T* __restrict__ E, const T* __restrict__ K0,
+                                                  const uint8_t* __restrict__ fix, int mask_fixed) {
+    const long nn = (long)(nx + 1) * (ny + 1);
+    for (long n = blockIdx.x * (long)blockDim.x + threadIdx.x; n < nn; n += (long)gridDim.x * blockDim.x) {
+        int i = (int)(n / (ny + 1)), j = (int)(n - (long)i * (ny + 1));
+        Vec2<T> v[3][3];
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+#pragma unroll
+            for (int b = 0; b < 3; b++) {
+                int ii = i + a - 1, jj = j + b - 1;
+                if (ii >= 0 && ii <= nx && jj >= 0 && jj <= ny) v[a][b] = load2(x, (long)(ny + 1) * ii + jj);
+                else v[a][b] = Vec2<T>{0, 0};
+            }
+        T yx, yy;
+        apply_q4_node(nx, ny, i, j, v, E, K0, yx, yy);
+        if (mask_fixed) {
+            uint8_t f = fix[n];
+            if (f & 1) yx = 0;
+            if (f & 2) yy = 0;
+        }
+        y[2 * n] = yx;
+        y[2 * n + 1] = yy;
+    }
+}
+
+// Jacobi diagonal of the EbE operator: sum of E_e K0(a,a) over the elements
+// of the node in ascending element id; fixed DOFs get 1.
+template <class T>
+__global__ void k_diag_q4(int nx, int ny, const T* __restrict__ E, const T* __restrict__ K0,
+                          const uint8_t* __restrict__ fix, T* __restrict__ D) {
+    const long nn = (long)(nx + 1) * (ny + 1);
+    for (long n = blockIdx.x * (long)blockDim.x + threadIdx.x; n < nn; n += (long)gridDim.x * blockDim.x) {
+        int i = (int)(n / (ny + 1)), j = (int)(n - (long)i * (ny + 1));
+        T dx = 0, dy = 0;
+        if (i >= 1 && j >= 1) { T e = E[(long)ny * (i - 1) + j - 1]; dx += e * K0[4 * 8 + 4]; dy += e * K0[5 * 8 + 5]; }
+        if (i >= 1 && j < ny) { T e = E[(long)ny * (i - 1) + j]; dx += e * K0[2 * 8 + 2]; dy += e * K0[3 * 8 + 3]; }
+        if (i < nx && j >= 1) { T e = E[(long)ny * i + j - 1]; dx += e * K0[6 * 8 + 6]; dy += e * K0[7 * 8 + 7]; }
+        if (i < nx && j < ny) { T e = E[(long)ny * i + j]; dx += e * K0[0]; dy += e * K0[1 * 8 + 1]; }
+        uint8_t f = fix[n];
+        D[2 * n] = (f & 1) ? (T)1 : dx;
+        D[2 * n + 1] = (f & 2) ? (T)1 : dy;
+    }
+}
+
+// =============================================================== PCG =====
+// Jacobi-PCG of ScalingCG (CG.h:419-453) as two launches per iteration.
+//   K1(k): beta from K2(k-1)'s partials (k > 0; convergence is tested first,
+//          as CG.h:443-447 does after iteration k-1), p = beta p + z fused
+//          into the operator gather, q = K p, partial p.q.
+//   K2(k): alpha = zr / p.q from K1's partials; x += alpha p; r -= alpha q;
+//          z = r ./ D; partials z.r and r.r.
+// Scalars that a later launch needs are written by workgroup 0 into slot
+// k&1, so no workgroup reads a value the same launch is writing.
+struct PcgDev {
+    double zr[2];
+    double bnorm;
+    double pq, alpha, beta, rnorm;
+    int iterations, status, done, pad;
+};
+
+__device__ inline void pcg_head(int k, const double* __restrict__ partB, int nbB, PcgDev* __restrict__ S, double eps,
+                                double* lds, double& beta, bool& stop) {
+    double v[2];
+    reduce_partials<2>(partB, nbB, lds, v);  // K2(k-1): (z.r, r.r); init: (z.r, b.b)
+    stop = false;
+    if (k == 0) {
+        beta = 0.0;
+        if (threadIdx.x == 0 && blockIdx.x == 0) {
+            S->zr[0] = v[0];
+            S->bnorm = sqrt(v[1]);
+            S->beta = 0.0;
+        }
+        return;
+    }
+    double bnorm = S->bnorm;
+    double zr_prev = S->zr[(k - 1) & 1];
+    double rnorm = sqrt(v[1]);
+    beta = v[0] / zr_prev;
+    if (rnorm < eps * bnorm) {
+        stop = true;
+        if (threadIdx.x == 0 && blockIdx.x == 0) {
+            S->rnorm = rnorm;
+            S->iterations = k;
+            S->status = 1;
+            S->done = 1;
+        }
+        return;
+    }
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        S->zr[k & 1] = v[0];
+        S->beta = beta;
+        S->rnorm = rnorm;
+    }
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_pcg_k1_q4(int nx, int ny, int k, const T* __restrict__ pold,
+                                                   const T* __restrict__ z, T* __restrict__ pnew, T* __restrict__ q,
+                                                   const T* __restrict__ E, const T* __restrict__ K0,
+                                                   const uint8_t* __restrict__ fix, const double* __restrict__ partB,
+                                                   int nbB, double* __restrict__ partA, PcgDev* __restrict__ S,
+                                                   double eps) {
+    __shared__ double lds[32];
+    if (S->done) return;
+    double beta;
+    bool stop;
+    pcg_head(k, partB, nbB, S, eps, lds, beta, stop);
+    if (stop) return;
+    const T tb = (T)beta;
+    const long nn = (long)(nx + 1) * (ny + 1);
+    double acc[1] = {0.0};
+    for (long n = blockIdx.x * (long)blockDim.x + threadIdx.x; n < nn; n += (long)gridDim.x * blockDim.x) {
+        int i = (int)(n / (ny + 1)), j = (int)(n - (long)i * (ny + 1));
+        Vec2<T> v[3][3];
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+#pragma unroll
+            for (int b = 0; b < 3; b++) {
+                int ii = i + a - 1, jj = j + b - 1;
+                if (ii >= 0 && ii <= nx && jj >= 0 && jj <= ny) {
+                    long m = (long)(ny + 1) * ii + jj;
+                    Vec2<T> po = load2(pold, m), zz = load2(z, m);
+                    v[a][b].x = tb * po.x + zz.x;  // xeaxpy: p = beta p + z (CG.h:42-49)
+                    v[a][b].y = tb * po.y + zz.y;
+                } else {
+                    v[a][b] = Vec2<T>{0, 0};
+                }
+            }
+        T yx, yy;
+        apply_q4_node(nx, ny, i, j, v, E, K0, yx, yy);
+        uint8_t f = fix[n];
+        if (f & 1) yx = 0;
+        if (f & 2) yy = 0;
+        pnew[2 * n] = v[1][1].x;
+        pnew[2 * n + 1] = v[1][1].y;
+        q[2 * n] = yx;
+        q[2 * n + 1] = yy;
+        acc[0] += (double)v[1][1].x * (double)yx;
+        acc[0] += (double)v[1][1].y * (double)yy;
+    }
+    store_partials<1>(acc, lds, partA);
+}
+
+// CSR flavour of K1 (free-DOF vectors): q_i = sum_j a_ij (beta pold_j + z_j),
+// each row summed in column order from 0 as CSR<T>::operator* does.
+__global__ void __launch_bounds__(256) k_pcg_k1_csr(int N, int k, const int32_t* __restrict__ indptr,
+                                                    const int32_t* __restrict__ indices,
+                                                    const double* __restrict__ data, const double* __restrict__ pold,
+                                                    const double* __restrict__ z, double* __restrict__ pnew,
+                                                    double* __restrict__ q, const double* __restrict__ partB, int nbB,
+                                                    double* __restrict__ partA, PcgDev* __restrict__ S, double eps) {
+    __shared__ double lds[32];
+    if (S->done) return;
+    double beta;
+    bool stop;
+    pcg_head(k, partB, nbB, S, eps, lds, beta, stop);
+    if (stop) return;
+    double acc[1] = {0.0};
+    for (long r = blockIdx.x * (long)blockDim.x + threadIdx.x; r < N; r += (long)gridDim.x * blockDim.x) {
+        double s = 0.0;
+        for (int jj = indptr[r]; jj < indptr[r + 1]; jj++) {
+            int c = indices[jj];
+            double pc = beta * pold[c] + z[c];
+            s += data[jj] * pc;
+        }
+        double pr = beta * pold[r] + z[r];
+        pnew[r] = pr;
+        q[r] = s;
+        acc[0] += pr * s;
+    }
+    store_partials<1>(acc, lds, partA);
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_pcg_k2(long n, int k, T* __restrict__ x, T* __restrict__ r, T* __restrict__ z,
+                                                const T* __restrict__ p, const T* __restrict__ q,
+                                                const T* __restrict__ D, const double* __restrict__ partA, int nbA,
+                                                double* __restrict__ partB, PcgDev* __restrict__ S) {
+    __shared__ double lds[32];
+    if (S->done) return;
+    double v[1];
+    reduce_partials<1>(partA, nbA, lds, v);
+    const double zr = S->zr[k & 1];
+    const double alpha = zr / v[0];  // CG.h:433
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        S->pq = v[0];
+        S->alpha = alpha;
+    }
+    const T ta = (T)alpha, tma = (T)(-alpha);
+    double acc[2] = {0.0, 0.0};
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        T xi = x[i] + ta * p[i];    // xexpay(x, alpha, p)
+        T ri = r[i] + tma * q[i];   // xexpay(r, -alpha, Ap)
+        T zi = ri / D[i];           // Scaling (CG.h:412)
+        x[i] = xi;
+        r[i] = ri;
+        z[i] = zi;
+        acc[0] += (double)zi * (double)ri;
+        acc[1] += (double)ri * (double)ri;
+    }
+    store_partials<2>(acc, lds, partB);
+}
+
+// x = 0, r = b, z = p0 = b ./ D, pold = 0; partials (z.r, b.b) for K1(0).
+template <class T>
+__global__ void __launch_bounds__(256) k_pcg_init(long n, const T* __restrict__ b, const T* __restrict__ D,
+                                                  T* __restrict__ x, T* __restrict__ r, T* __restrict__ z,
+                                                  T* __restrict__ pold, double* __restrict__ partB,
+                                                  PcgDev* __restrict__ S) {
+    __shared__ double lds[32];
+    double acc[2] = {0.0, 0.0};
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        T bi = b[i];
+        T ri = bi - (T)0;  // subtract(b, A*0) (CG.h:424)
+        T zi = ri / D[i];
+        x[i] = 0;
+        r[i] = ri;
+        z[i] = zi;
+        pold[i] = 0;
+        acc[0] += (double)zi * (double)ri;
+        acc[1] += (double)bi * (double)bi;
+    }
+    store_partials<2>(acc, lds, partB);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        S->done = 0;
+        S->status = 0;
+        S->iterations = 0;
+    }
+}
+
+// After the host's last K2: test convergence of the final iterate the way
+// K1 would, and record the itrmax exit (CG.h:451-452).
+__global__ void k_pcg_final(int k, const double* __restrict__ partB, int nbB, PcgDev* __restrict__ S, double eps) {
+    __shared__ double lds[32];
+    if (S->done) return;
+    double v[2];
+    reduce_partials<2>(partB, nbB, lds, v);
+    if (threadIdx.x == 0) {
+        double rnorm = sqrt(v[1]);
+        S->rnorm = rnorm;
+        S->iterations = k;
+        S->status = (k > 0 && rnorm < eps * S->bnorm) ? 1 : 2;
+        S->done = 1;
+    }
+}
+
+// ============================================================= CSR ========
+// Closed-form row lengths of the structured quad4 matrix: the free DOFs of
+// the 3x3 node block around the row's node (every pair of nodes in that block
+// shares an element).
+__global__ void k_csr_count_q4(int nx, int ny, const int32_t* __restrict__ n2g, const uint8_t* __restrict__ fix,
+                               int32_t* __restrict__ rowlen) {
+    const long nn = (long)(nx + 1) * (ny + 1);
+    for (long n = blockIdx.x * (long)blockDim.x + threadIdx.x; n < nn; n += (long)gridDim.x * blockDim.x) {
+        int i = (int)(n / (ny + 1)), j = (int)(n - (long)i * (ny + 1));
+        int cnt = 0;
+        for (int ii = i - 1; ii <= i + 1; ii++)
+            for (int jj = j - 1; jj <= j + 1; jj++) {
+                if (ii < 0 || ii > nx || jj < 0 || jj > ny) continue;
+                uint8_t f = fix[(long)(ny + 1) * ii + jj];
+                cnt += !(f & 1) + !(f & 2);
+            }
+        for (int d = 0; d < 2; d++) {
+            int g = n2g[2 * n + d];
+            if (g >= 0) rowlen[g] = cnt;
+        }
+    }
+}
+
+// Values: for row (node n, dof d) and column (node m, dof d'), the sum over
+// the elements containing both nodes, in ascending element id, of
+// Ke_e(2 loc_n + d, 2 loc_m + d'), starting from 0.0 + first term, exactly as
+// LILCSR set/get accumulates (Assembling.h:54-56, LILCSR.h:92-114).
+__device__ inline int loc_q4(int ei, int ej, int ni, int nj) {
+    // local index of node (ni,nj) in element (ei,ej): BL 0, BR 1, TR 2, TL 3
+    int di = ni - ei, dj = nj - ej;
+    return dj == 0 ? di : 3 - di;
+}
+
+__global__ void k_csr_fill_q4(int nx, int ny, const int32_t* __restrict__ n2g, const int32_t* __restrict__ indptr,
+                              const double* __restrict__ ke_all, int32_t* __restrict__ indices,
+                              double* __restrict__ data) {
+    const long nn = (long)(nx + 1) * (ny + 1);
+    for (long n = blockIdx.x * (long)blockDim.x + threadIdx.x; n < nn; n += (long)gridDim.x * blockDim.x) {
+        int i = (int)(n / (ny + 1)), j = (int)(n - (long)i * (ny + 1));
+        for (int d = 0; d < 2; d++) {
+            int row = n2g[2 * n + d];
+            if (row < 0) continue;
+            int pos = indptr[row];
+            for (int ii = i - 1; ii <= i + 1; ii++)
+                for (int jj = j - 1; jj <= j + 1; jj++) {
+                    if (ii < 0 || ii > nx || jj < 0 || jj > ny) continue;
+                    long m = (long)(ny + 1) * ii + jj;
+                    for (int d2 = 0; d2 < 2; d2++) {
+                        int col = n2g[2 * m + d2];
+                        if (col < 0) continue;
+                        double v = 0.0;
+                        // elements containing n and m, ascending id (i-major)
+                        int ei0 = max(max(i, ii) - 1, 0), ei1 = min(min(i, ii), nx - 1);
+                        int ej0 = max(max(j, jj) - 1, 0), ej1 = min(min(j, jj), ny - 1);
+                        for (int ei = ei0; ei <= ei1; ei++)
+                            for (int ej = ej0; ej <= ej1; ej++) {
+                                const double* Ke = ke_all + ((long)ny * ei + ej) * 64;
+                                int a = 2 * loc_q4(ei, ej, i, j) + d, b = 2 * loc_q4(ei, ej, ii, jj) + d2;
+                                v = v + Ke[a * 8 + b];
+                            }
+                        indices[pos] = col;
+                        data[pos] = v;
+                        pos++;
+                    }
+                }
+        }
+    }
+}
+
+// CSR<T>::operator* (CSR.h:108-122): row sums from 0 in column order.
+__global__ void k_csr_spmv(int N, const int32_t* __restrict__ indptr, const int32_t* __restrict__ indices,
+                           const double* __restrict__ data, const double* __restrict__ x, double* __restrict__ y) {
+    for (long r = blockIdx.x * (long)blockDim.x + threadIdx.x; r < N; r += (long)gridDim.x * blockDim.x) {
+        double s = 0.0;
+        for (int jj = indptr[r]; jj < indptr[r + 1]; jj++) s += data[jj] * x[indices[jj]];
+        y[r] = s;
+    }
+}
+
+// GetDiagonal via CSR::get (CG.h:397-404, CSR.h:154-167).
+__global__ void k_csr_diag(int N, const int32_t* __restrict__ indptr, const int32_t* __restrict__ indices,
+                           const double* __restrict__ data, double* __restrict__ D) {
+    for (long r = blockIdx.x * (long)blockDim.x + threadIdx.x; r < N; r += (long)gridDim.x * blockDim.x) {
+        int lo = indptr[r], hi = indptr[r + 1], end = hi;
+        while (lo < hi) {
+            int mid = lo + (hi - lo) / 2;
+            if (indices[mid] < r) lo = mid + 1; else hi = mid;
+        }
+        D[r] = (lo < end && indices[lo] == r) ? data[lo] : 0.0;
+    }
+}
+
+// Exclusive scan of row lengths into indptr (single workgroup, sequential
+// chunks; runs once per assembly).
+__global__ void k_scan(const int32_t* __restrict__ len, int32_t* __restrict__ out, int n) {
+    __shared__ int32_t sums[1024];
+    const int t = threadIdx.x, nt = blockDim.x;
+    long chunk = (n + nt - 1) / nt;
+    long s0 = t * chunk, s1 = min((long)n, s0 + chunk);
+    int32_t s = 0;
+    for (long i = s0; i < s1; i++) s += len[i];
+    sums[t] = s;
+    __syncthreads();
+    if (t == 0) {
+        int32_t run = 0;
+        for (int k = 0; k < nt; k++) { int32_t v = sums[k]; sums[k] = run; run += v; }
+        out[n] = run;
+    }
+    __syncthreads();
+    int32_t run = sums[t];
+    for (long i = s0; i < s1; i++) { out[i] = run; run += len[i]; }
+}
+
+// ==================================================== element matrices ====
+// PlaneStrainStiffness<T, ShapeFunction4Square, Gauss4Square>
+// (PlaneStrain.h:19-58) in the reference's operation order: node
+// coordinates lx*(i/(double)nx) (SquareMesh.h:66), dXdr = dNdr X, cofactor
+// inverse divided by the determinant, B^T D B J t w0 w1 accumulated per
+// Gauss point.  One thread per element.
+__device__ inline void mm_dev(const double* A, const double* B, double* C, int m, int k, int n) {
+    for (int i = 0; i < m; i++)
+        for (int j = 0; j < n; j++) {
+            double v = 0.0;
+            for (int q = 0; q < k; q++) v += A[i * k + q] * B[q * n + j];
+            C[i * n + j] = v;
+        }
+}
+
+__device__ void ke_q4_ref(const double* X, double E, double nu, double t, double* Ke) {
+    const double a = 1.0 / sqrt(3.0);
+    const double m = -1.0 / sqrt(3.0);
+    const double gp[4][2] = {{m, m}, {a, m}, {m, a}, {a, a}};
+    double D[9] = {1.0 - nu, nu, 0.0, nu, 1.0 - nu, 0.0, 0.0, 0.0, 0.5 * (1.0 - 2.0 * nu)};
+    double c = E / ((1.0 - 2.0 * nu) * (1.0 + nu));
+    for (int k = 0; k < 9; k++) D[k] *= c;
+    for (int k = 0; k < 64; k++) Ke[k] = 0.0;
+    for (int g = 0; g < 4; g++) {
+        double r0 = gp[g][0], r1 = gp[g][1];
+        double dN[8] = {-0.25 * (1.0 - r1), 0.25 * (1.0 - r1), 0.25 * (1.0 + r1), -0.25 * (1.0 + r1),
+                        -0.25 * (1.0 - r0), -0.25 * (1.0 + r0), 0.25 * (1.0 + r0), 0.25 * (1.0 - r0)};
+        double J2[4], inv[4], dNdX[8], B[24], Bt[24], BtD[24], M[64];
+        mm_dev(dN, X, J2, 2, 4, 2);
+        double det = J2[0] * J2[3] - J2[1] * J2[2];
+        inv[0] = J2[3] / det;
+        inv[1] = (-J2[1]) / det;
+        inv[2] = (-J2[2]) / det;
+        inv[3] = J2[0] / det;
+        mm_dev(inv, dN, dNdX, 2, 2, 4);
+        for (int n = 0; n < 4; n++) {
+            B[2 * n] = dNdX[n];         B[2 * n + 1] = 0.0;
+            B[8 + 2 * n] = 0.0;         B[8 + 2 * n + 1] = dNdX[4 + n];
+            B[16 + 2 * n] = dNdX[4 + n]; B[16 + 2 * n + 1] = dNdX[n];
+        }
+        for (int r = 0; r < 8; r++)
+            for (int s = 0; s < 3; s++) Bt[r * 3 + s] = B[s * 8 + r];
+        mm_dev(Bt, D, BtD, 8, 3, 3);
+        mm_dev(BtD, B, M, 8, 3, 8);
+        for (int k = 0; k < 64; k++) Ke[k] += M[k] * det * t * 1.0 * 1.0;
+    }
+}
+
+__global__ void k_ke_all_q4(int nx, int ny, double lx, double ly, const double* __restrict__ E, double nu, double t,
+                            double* __restrict__ ke_all) {
+    const long ne = (long)nx * ny;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < ne; e += (long)gridDim.x * blockDim.x) {
+        int i = (int)(e / ny), j = (int)(e - (long)i * ny);
+        double x0 = lx * (i / (double)nx), x1 = lx * ((i + 1) / (double)nx);
+        double y0 = ly * (j / (double)ny), y1 = ly * ((j + 1) / (double)ny);
+        double X[8] = {x0, y0, x1, y0, x1, y1, x0, y1};
+        double Ke[64];
+        ke_q4_ref(X, E[e], nu, t, Ke);
+        double* out = ke_all + e * 64;
+        for (int k = 0; k < 64; k++) out[k] = Ke[k];
+    }
+}
+
+// ============================================================ filter ======
+// Element centroids as CenterOfGravity computes them (General.h:70-77):
+// ((((0 + x_BL) + x_BR) + x_TR) + x_TL) / 4.
+__global__ void k_centroids_q4(int nx, int ny, double lx, double ly, double* __restrict__ cg) {
+    const long ne = (long)nx * ny;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < ne; e += (long)gridDim.x * blockDim.x) {
+        int i = (int)(e / ny), j = (int)(e - (long)i * ny);
+        double x0 = lx * (i / (double)nx), x1 = lx * ((i + 1) / (double)nx);
+        double y0 = ly * (j / (double)ny), y1 = ly * ((j + 1) / (double)ny);
+        double cx = 0.0, cy = 0.0;
+        cx += x0; cy += y0;
+        cx += x1; cy += y0;
+        cx += x1; cy += y1;
+        cx += x0; cy += y1;
+        cg[2 * e] = cx / 4.0;
+        cg[2 * e + 1] = cy / 4.0;
+    }
+}
+
+// Filter weight of the pair (i, j) as the sample driver computes it
+// (sample_optimize_density_oc.cpp:55-57, Vector::Norm at Vector.h:297-303).
+__device__ inline bool filter_w(const double* __restrict__ cg, long a, long b, double R, double& w) {
+    double v0 = cg[2 * a] - cg[2 * b], v1 = cg[2 * a + 1] - cg[2 * b + 1];
+    double s = 0.0;
+    s += v0 * v0;
+    s += v1 * v1;
+    double d = sqrt(s);
+    if (!(d <= R)) return false;
+    w = (R - d) / R;
+    return true;
+}
+
+// mode 0: rho_i = sum_j w_ij s_j / sum_j w_ij  (DensityFilter.h:45-56)
+// mode 1: dfds_i = sum_j dfdrho_j w_ij / sum_j w_ij (DensityFilter.h:60-71)
+// Neighbours visited in ascending element id inside a +-win window.
+__device__ inline double filter_one(int nx, int ny, int win, double R, const double* __restrict__ cg,
+                                    const double* __restrict__ in, long e) {
+    int i = (int)(e / ny), j = (int)(e - (long)i * ny);
+    double acc = 0.0, wsum = 0.0;
+    int i0 = max(i - win, 0), i1 = min(i + win, nx - 1), j0 = max(j - win, 0), j1 = min(j + win, ny - 1);
+    for (int ii = i0; ii <= i1; ii++)
+        for (int jj = j0; jj <= j1; jj++) {
+            long m = (long)ny * ii + jj;
+            double w;
+            if (filter_w(cg, e, m, R, w)) {
+                acc += w * in[m];
+                wsum += w;
+            }
+        }
+    return acc / wsum;
+}
+
+__global__ void __launch_bounds__(256) k_filter_q4(int nx, int ny, int win, double R, const double* __restrict__ cg,
+                                                   const double* __restrict__ in, double* __restrict__ out) {
+    const long ne = (long)nx * ny;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < ne; e += (long)gridDim.x * blockDim.x)
+        out[e] = filter_one(nx, ny, win, R, cg, in, e);
+}
+
+// g = sum_i scale1 rho_i / (vf ne) - scale1 (sample...oc.cpp:99-105); the
+// per-element terms are summed in a fixed tree order.
+__global__ void __launch_bounds__(256) k_volume_partials(long ne, const double* __restrict__ rho, double vf,
+                                                         double scale1, double* __restrict__ part) {
+    __shared__ double lds[32];
+    double acc[1] = {0.0};
+    const double den = vf * (double)ne;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < ne; e += (long)gridDim.x * blockDim.x)
+        acc[0] += scale1 * rho[e] / den;
+    store_partials<1>(acc, lds, part);
+}
+
+__global__ void k_reduce_final(const double* __restrict__ part, int nparts, int K, double scale, double shift,
+                               double* __restrict__ out) {
+    __shared__ double lds[64];
+    for (int k = 0; k < K; k++) {
+        double v[1] = {0.0};
+        for (int i = threadIdx.x; i < nparts; i += blockDim.x) v[0] += part[i * K + k];
+        block_sum<1>(v, lds);
+        if (threadIdx.x == 0) out[k] = scale * v[0] - shift;
+    }
+}
+
+// ============================================ compliance / sensitivity ====
+// Per element c_e = u_e^T K0 u_e (Matrix*Vector then Vector*Vector order),
+// dfdrho_e = -scale0 p (E1 - E0) rho^(p-1) c_e (sample...oc.cpp:161) and the
+// compliance terms E_e c_e (u^T K u = sum_e E_e c_e with fixed DOFs at 0).
+__global__ void __launch_bounds__(256) k_comp_sens_q4(int nx, int ny, const double* __restrict__ u,
+                                                      const double* __restrict__ E, const double* __restrict__ rho,
+                                                      const double* __restrict__ K0, double coef, double pm1,
+                                                      double* __restrict__ dfdrho, double* __restrict__ part) {
+    __shared__ double lds[32];
+    const long ne = (long)nx * ny;
+    double acc[1] = {0.0};
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < ne; e += (long)gridDim.x * blockDim.x) {
+        int i = (int)(e / ny), j = (int)(e - (long)i * ny);
+        long nbl = (long)(ny + 1) * i + j, nbr = nbl + (ny + 1);
+        double ue[8] = {u[2 * nbl], u[2 * nbl + 1], u[2 * nbr], u[2 * nbr + 1],
+                        u[2 * (nbr + 1)], u[2 * (nbr + 1) + 1], u[2 * (nbl + 1)], u[2 * (nbl + 1) + 1]};
+        double c = 0.0;
+        for (int a = 0; a < 8; a++) {
+            double ka = 0.0;
+            for (int b = 0; b < 8; b++) ka += K0[a * 8 + b] * ue[b];
+            c += ue[a] * ka;
+        }
+        dfdrho[e] = coef * simp_pow(rho[e], pm1) * c;
+        acc[0] += E[e] * c;
+    }
+    store_partials<1>(acc, lds, part);
+}
+
+// ================================================================ OC =====
+// OC<T>::UpdateVariables (OC.h:76-107).  One bisection step is two launches:
+//   k_oc_xplus(k): decide step k-1 from its volume partials (g > 0 -> l0 =
+//     lambda else l1 = lambda), stop when (l1-l0)/(l1+l0) <= lambdaeps, else
+//     lambda = (l1+l0)/2 and x+ = clamp(x (-dfdx/(dgdx lambda))^iota);
+//   k_oc_volume(k): filter x+ and sum the closure's volume terms.
+// The bracket of step k lives in slot k of the history arrays.
+struct OcDev {
+    double l0i, l1i;  // initial bracket (lambdamin, lambdamax), set by the host
+    double l0[2], l1[2], lambda[2];
+    double g_last, lambda_last;
+    int steps, done, status, pad;
+};
+
+__global__ void __launch_bounds__(256) k_oc_xplus(long ne, int k, const double* __restrict__ x,
+                                                  const double* __restrict__ dfdx, const double* __restrict__ dgdx,
+                                                  double iota, double move, double leps, double scale1,
+                                                  const double* __restrict__ part, int nparts,
+                                                  double* __restrict__ xplus, OcDev* __restrict__ S) {
+    __shared__ double lds[32];
+    if (S->done) return;
+    double l0, l1;
+    if (k == 0) {
+        l0 = S->l0i;
+        l1 = S->l1i;
+    } else {
+        double v[1];
+        reduce_partials<1>(part, nparts, lds, v);
+        double g = v[0] - 1.0 * scale1;
+        l0 = S->l0[(k - 1) & 1];
+        l1 = S->l1[(k - 1) & 1];
+        double lam = S->lambda[(k - 1) & 1];
+        if (g > 0.0) l0 = lam; else l1 = lam;
+        if (threadIdx.x == 0 && blockIdx.x == 0) S->g_last = g;
+    }
+    if (!((l1 - l0) / (l1 + l0) > leps)) {
+        if (threadIdx.x == 0 && blockIdx.x == 0) {
+            S->steps = k;
+            S->done = 1;
+        }
+        return;
+    }
+    const double lambda = 0.5 * (l1 + l0);
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        S->l0[k & 1] = l0;
+        S->l1[k & 1] = l1;
+        S->lambda[k & 1] = lambda;
+        S->lambda_last = lambda;
+        S->steps = k + 1;
+    }
+    const bool half = (iota == 0.5);
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < ne; i += (long)gridDim.x * blockDim.x) {
+        double xi = x[i];
+        double base = -dfdx[i] / (dgdx[i] * lambda);
+        double v = (half ? sqrt(base) : pow(base, iota)) * xi;
+        double lo = std_max0((1.0 - move) * xi), hi = std_min1((1.0 + move) * xi);
+        if (v < lo) v = lo;
+        else if (v > hi) v = hi;
+        xplus[i] = v;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_oc_volume(int nx, int ny, int win, double R, const double* __restrict__ cg,
+                                                   const double* __restrict__ xplus, double vf, double scale1,
+                                                   double* __restrict__ part, const OcDev* __restrict__ S) {
+    __shared__ double lds[32];
+    if (S->done) return;
+    const long ne = (long)nx * ny;
+    const double den = vf * (double)ne;
+    double acc[1] = {0.0};
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < ne; e += (long)gridDim.x * blockDim.x)
+        acc[0] += scale1 * filter_one(nx, ny, win, R, cg, xplus, e) / den;
+    store_partials<1>(acc, lds, part);
+}
+
+// ===================================================== true residual =====
+// r = b - K x (nodal, FP64) with partials of r.r; used by the refinement
+// loop and to report the true residual of every solve.
+__global__ void __launch_bounds__(256) k_residual_q4(int nx, int ny, const double* __restrict__ b,
+                                                     const double* __restrict__ x, double* __restrict__ r,
+                                                     const double* __restrict__ E, const double* __restrict__ K0,
+                                                     const uint8_t* __restrict__ fix, double* __restrict__ part) {
+    __shared__ double lds[32];
+    const long nn = (long)(nx + 1) * (ny + 1);
+    double acc[1] = {0.0};
+    for (long n = blockIdx.x * (long)blockDim.x + threadIdx.x; n < nn; n += (long)gridDim.x * blockDim.x) {
+        int i = (int)(n / (ny + 1)), j = (int)(n - (long)i * (ny + 1));
+        Vec2<double> v[3][3];
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+#pragma unroll
+            for (int bb = 0; bb < 3; bb++) {
+                int ii = i + a - 1, jj = j + bb - 1;
+                if (ii >= 0 && ii <= nx && jj >= 0 && jj <= ny) v[a][bb] = load2(x, (long)(ny + 1) * ii + jj);
+                else v[a][bb] = Vec2<double>{0, 0};
+            }
+        double yx, yy;
+        apply_q4_node(nx, ny, i, j, v, E, K0, yx, yy);
+        uint8_t f = fix[n];
+        double rx = (f & 1) ? 0.0 : b[2 * n] - yx;
+        double ry = (f & 2) ? 0.0 : b[2 * n + 1] - yy;
+        r[2 * n] = rx;
+        r[2 * n + 1] = ry;
+        acc[0] += rx * rx;
+        acc[0] += ry * ry;
+    }
+    store_partials<1>(acc, lds, part);
+}
+
+__global__ void k_axpy_f32(long n, const float* __restrict__ d, double* __restrict__ x) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        x[i] = x[i] + (double)d[i];
+}
+
+// One element's PlaneStrainStiffness on the device (check path).
+__global__ void k_ke_one_q4(int nx, int ny, double lx, double ly, int e, double E, double nu, double t,
+                            double* __restrict__ Ke) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    int i = e / ny, j = e - i * ny;
+    double x0 = lx * (i / (double)nx), x1 = lx * ((i + 1) / (double)nx);
+    double y0 = ly * (j / (double)ny), y1 = ly * ((j + 1) / (double)ny);
+    double X[8] = {x0, y0, x1, y0, x1, y1, x0, y1};
+    ke_q4_ref(X, E, nu, t, Ke);
+}
+
+__global__ void __launch_bounds__(256) k_dot_partials(long n, const double* __restrict__ a, const double* __restrict__ b,
+                                                      double* __restrict__ part) {
+    __shared__ double lds[32];
+    double acc[1] = {0.0};
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        acc[0] += a[i] * b[i];
+    store_partials<1>(acc, lds, part);
+}
+
+}  // namespace pf

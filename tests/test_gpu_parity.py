@@ -1,0 +1,164 @@
+"""GPU parity of the HIP path against the reference's golden vectors and the
+CPU oracle, called through the C ABI.
+
+Bars: bit-exact for DOF numbering, CSR structure, CSR values, the CSR
+product and the device element matrix; relative 1e-14 for the matrix-free
+FP64 operator against the assembled one; the reference's own tolerances for
+solves (displacement 1e-8, compliance 1e-9) and for the loop (density after
+50 OC iterations 1e-6 max-abs).
+"""
+import math
+
+import numpy as np
+import pytest
+
+from tests import golden_io as gi
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def pa():
+    import pansfem2_amd
+    return pansfem2_amd
+
+
+def host_E(rho, E1=1.0, E0=1e-9, p=3.0):
+    # the driver's E (sample_optimize_density_oc.cpp:123), evaluated with the host libm
+    return np.array([E1 * math.pow(r, p) + E0 * (1.0 - math.pow(r, p)) for r in rho])
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))
+
+
+@pytest.mark.parametrize("case,nx,ny", [("simp_4x2_oc", 4, 2), ("simp_8x4_oc", 8, 4), ("simp_120x40_oc", 120, 40)])
+def test_dofmap_csr_bit_exact(pa, case, nx, ny):
+    g = gi.load(case)
+    ctx = pa.cantilever2d(nx, ny)
+    assert gi.sha(ctx.dofmap()) == gi.digest(case, "dofmap")
+    assert gi.sha(ctx.element_stiffness(0, 1.0).reshape(-1)) == gi.digest(case, "ke_e0")
+    ctx.set_moduli(host_E(g["rho0"]))
+    ctx.assemble_csr()
+    ip, ix, dv = ctx.csr()
+    assert gi.sha(ip) == gi.digest(case, "csr_indptr")
+    assert gi.sha(ix) == gi.digest(case, "csr_indices")
+    assert gi.sha(dv) == gi.digest(case, "csr_data")
+    assert np.array_equal(ctx.load(), g["F"])
+
+
+def test_element_matrix_bit_exact(pa):
+    ctx = pa.cantilever2d(120, 40)
+    g = gi.manifest()["simp_120x40_oc"]["arrays"]["ke_e0"]["sha256"]
+    assert gi.sha(ctx.element_stiffness(0, 1.0).reshape(-1)) == g
+
+
+@pytest.mark.parametrize("case", ["operator_16x8", "operator_64x32"])
+def test_operator_and_solve(pa, case):
+    g = gi.load(case)
+    m = gi.meta(case)
+    ctx = pa.cantilever2d(m["nx"], m["ny"])
+    ctx.set_moduli(host_E(g["rho"]))
+    ctx.assemble_csr()
+    ip, ix, dv = ctx.csr()
+    assert gi.sha(dv) == gi.digest(case, "csr_data")
+    # CSR<T>::operator* bit for bit
+    y_csr = ctx.apply_K(g["x"], pa.PF_OP_CSR64)
+    assert np.array_equal(y_csr, g["y"])
+    # matrix-free FP64 operator (device moduli from rho) against the assembled one
+    ctx.set_density(g["rho"])
+    y_ebe = ctx.apply_K(g["x"], pa.PF_OP_EBE64)
+    assert rel(y_ebe, g["y"]) < 1e-14
+    y32 = ctx.apply_K(g["x"], pa.PF_OP_EBE32)
+    assert rel(y32, g["y"]) < 1e-5
+    # ScalingCG to 1e-10: displacement within 1e-8 of the reference's
+    for op in (pa.PF_OP_EBE64, pa.PF_OP_CSR64):
+        if op == pa.PF_OP_CSR64:
+            ctx.set_moduli(host_E(g["rho"]))
+            ctx.assemble_csr()
+        u, st = ctx.pcg(g["F"], op=op, eps=1e-10)
+        assert st["status"] == 1
+        assert rel(u, g["u"]) < 1e-8, (op, rel(u, g["u"]), st)
+        assert st["true_residual"] < 1e-9
+
+
+def test_filter_bit_exact(pa):
+    from oracle import pyoracle as po
+    nx, ny = 120, 40
+    ctx = pa.cantilever2d(nx, ny)
+    xy, conn = po.grid2d(nx, ny)
+    fp, fi, fw = po.filter_build(nx, ny, po.centroids(conn, xy))
+    rng = np.random.default_rng(11)
+    s = rng.uniform(0, 1, nx * ny)
+    assert np.array_equal(ctx.filter(s, pa.PF_FILTER_VARIABLES), po.filter_apply(fp, fi, fw, s))
+    d = rng.standard_normal(nx * ny)
+    assert np.array_equal(ctx.filter(d, pa.PF_FILTER_SENSITIVITIES), po.filter_sens(fp, fi, fw, d))
+
+
+def test_compliance_sensitivity(pa):
+    case = "simp_120x40_oc"
+    g = gi.load(case)
+    ctx = pa.cantilever2d(120, 40)
+    ctx.set_density(g["rho0"])
+    f, dfdrho = ctx.compliance_sens(g["u0_free"])
+    f0 = gi_f0(case)
+    assert abs(f - f0) / f0 < 1e-12
+    assert np.max(np.abs(dfdrho - g["dfdrho0"])) < 1e-12 * np.max(np.abs(g["dfdrho0"]))
+
+
+def gi_f0(case):
+    from oracle import pyoracle as po
+    return float(po.simp2d(120, 40, 1)["f"][0])
+
+
+def test_oc_update_matches_oracle(pa):
+    from oracle import pyoracle as po
+    case = "simp_120x40_oc"
+    g = gi.load(case)
+    nx, ny = 120, 40
+    ctx = pa.cantilever2d(nx, ny)
+    xy, conn = po.grid2d(nx, ny)
+    fp, fi, fw = po.filter_build(nx, ny, po.centroids(conn, xy))
+    s0 = np.full(nx * ny, 0.5)
+    dgds = po.filter_sens(fp, fi, fw, np.full(nx * ny, 1.0 / (0.5 * nx * ny)))
+    x_ref, lam_ref, dec = po.oc_update(s0, g["dfds0"], dgds, fp, fi, fw)
+    x, lam, steps = ctx.oc_update(s0, g["dfds0"], dgds)
+    assert steps == len(dec)
+    assert lam == lam_ref
+    assert np.max(np.abs(x - x_ref)) < 1e-12
+
+
+def test_simp_loop_c1_against_reference(pa):
+    """50 OC iterations at 120x40: compliance history and final density."""
+    case = "simp_120x40_oc"
+    g = gi.load(case)
+    ctx = pa.cantilever2d(120, 40)
+    r = ctx.simp_run(50)
+    from oracle import pyoracle as po
+    ref = po.simp2d(120, 40, 50)
+    assert gi.sha(ref["f"]) == gi.digest(case, "hist_f")
+    assert np.max(np.abs(r["f"] - ref["f"]) / ref["f"]) < 1e-9
+    assert np.max(np.abs(r["rho"] - g["rho_final"])) < 1e-6
+    assert np.max(np.abs(r["s"] - g["s_final"])) < 1e-6
+    assert np.max(np.abs(r["cg_iters"] - ref["cg_iters"])) <= 3
+
+
+def test_operator_symmetry_and_residual_large(pa):
+    """1024x512: size-independent properties (x^T K y = y^T K x, PCG true residual)."""
+    nx, ny = 1024, 512
+    ctx = pa.cantilever2d(nx, ny)
+    rng = np.random.default_rng(20261201)
+    rho = rng.uniform(1e-3, 1.0, nx * ny)
+    ctx.set_density(rho)
+    x = rng.uniform(-1, 1, ctx.N)
+    y = rng.uniform(-1, 1, ctx.N)
+    Kx = ctx.apply_K(x)
+    Ky = ctx.apply_K(y)
+    assert abs(x @ Ky - y @ Kx) / abs(x @ Ky) < 1e-12
+    assert x @ Kx > 0
+    ctx.set_moduli(host_E(rho))
+    ctx.assemble_csr()
+    assert ctx.nnz == 18874360
+    Kc = ctx.apply_K(x, pa.PF_OP_CSR64)
+    ctx.set_density(rho)
+    assert rel(Kx, Kc) < 1e-14
