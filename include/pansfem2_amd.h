@@ -204,6 +204,16 @@ int pf_simp_run(pf_ctx* ctx, const pf_simp_params* prm, int iters, double* s, do
  * 0.7, 1.2, 1e-6)). */
 void pf_simp_default_params(pf_simp_params* prm);
 
+/* ---- measurement -------------------------------------------------------- */
+/* Average device duration (HIP events on the context's stream, `reps`
+ * back-to-back launches each) of the two PCG iteration kernels of operator
+ * `op` on the current moduli: ms[0] = operator kernel (p update + K p +
+ * p.q), ms[1] = vector-update kernel (x, r, z and the z.r, r.r partials),
+ * ms[2] = density filter, ms[3] = one OC bisection step (two launches).
+ * Diagnostic only (no counterpart in the reference); it overwrites the
+ * solver's work vectors. */
+int pf_profile_kernels(pf_ctx* ctx, int op, int reps, double* ms);
+
 /* ---- learned preconditioner -------------------------------------------- */
 /* 4-level 3x3 convolutional network (BF16 weights and activations, FP32
  * accumulation) mapping (rho, r_x, r_y) to a displacement correction, used

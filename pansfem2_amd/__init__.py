@@ -104,6 +104,7 @@ def lib():
             "pf_mma_update": ([c_void_p, _f64p, c_double, _f64p, c_double, _f64p, P(c_int)], c_int),
             "pf_simp_run": ([c_void_p, P(SimpParams), c_int, _f64p, c_void_p, P(SimpHistory)], c_int),
             "pf_simp_default_params": ([P(SimpParams)], None),
+            "pf_profile_kernels": ([c_void_p, c_int, c_int, _f64p], c_int),
             "pf_precond_init": ([c_void_p, c_int, ctypes.c_uint64], c_int),
             "pf_precond_save": ([c_void_p, c_void_p, P(c_int64)], c_int),
             "pf_precond_load": ([c_void_p, _f32p, c_int64], c_int),
@@ -297,6 +298,13 @@ class Context:
         steps = ctypes.c_int(0)
         self._ck(lib().pf_mma_update(self._h, x, f, _f64(dfdx), g, _f64(dgdx), ctypes.byref(steps)))
         return x, steps.value
+
+    def profile_kernels(self, op=PF_OP_EBE64, reps=50):
+        """Average device ms of the PCG operator kernel, the PCG vector kernel,
+        the filter and one OC bisection step (HIP events on the library stream)."""
+        ms = np.zeros(4)
+        self._ck(lib().pf_profile_kernels(self._h, op, reps, ms))
+        return dict(k1_ms=ms[0], k2_ms=ms[1], filter_ms=ms[2], oc_step_ms=ms[3])
 
     def simp_run(self, iters, params=None, s0=0.5):
         """The device-resident SIMP loop (sample_optimize_density_oc.cpp:83-208)."""

@@ -919,6 +919,89 @@ int pf_simp_run(pf_ctx* ctx, const pf_simp_params* prm, int iters, double* s, do
     return sync(ctx);
 }
 
+int pf_profile_kernels(pf_ctx* ctx, int op, int reps, double* ms) {
+    if (!ctx || !ms || reps < 1) return PF_ERR_ARG;
+    PF_TRY(require_ready(ctx, true));
+    PF_CHECK(op == PF_OP_EBE64 || op == PF_OP_EBE32, PF_ERR_ARG, "pf_profile_kernels: EbE operators only");
+    PF_CHECK(is2d(ctx), PF_ERR_UNSUPPORTED, "pf_profile_kernels: quad4 grids only in this build");
+    PF_TRY(set_device(ctx));
+    const bool f32 = op == PF_OP_EBE32;
+    PcgDev* S = (PcgDev*)ctx->d_pcg;
+    double* partA = ctx->red.partials;
+    double* partB = ctx->red.partials + 2 * kMaxPartials;
+    const unsigned g1 = gsize(ctx->nn), g2 = gsize(ctx->nv);
+    // a defined state: b = load, x = 0, z = b/D, p = 0; launches use k = 0 (no
+    // convergence test) so every repetition does the full work
+    PF_TRY(diag_nodal(ctx, f32));
+    if (f32) {
+        k_convert<double, float><<<gsize(ctx->nv), kBlock, 0, ctx->stream>>>(ctx->nv, ctx->d_load, ctx->f_b);
+        k_pcg_init<float><<<g2, kBlock, 0, ctx->stream>>>(ctx->nv, ctx->f_b, ctx->f_D, ctx->f_x, ctx->f_r, ctx->f_z,
+                                                         ctx->f_p0, partB, S);
+    } else {
+        k_pcg_init<double><<<g2, kBlock, 0, ctx->stream>>>(ctx->nv, ctx->d_load, ctx->d_D, ctx->d_x, ctx->d_r,
+                                                          ctx->d_z, ctx->d_p0, partB, S);
+    }
+    PF_TRY(check_launch(ctx));
+    hipEvent_t e0, e1;
+    PF_HIP(hipEventCreate(&e0));
+    PF_HIP(hipEventCreate(&e1));
+    auto timed = [&](auto&& launch) -> double {
+        launch();  // warm
+        hipEventRecord(e0, ctx->stream);
+        for (int i = 0; i < reps; i++) launch();
+        hipEventRecord(e1, ctx->stream);
+        hipEventSynchronize(e1);
+        float t = 0.f;
+        hipEventElapsedTime(&t, e0, e1);
+        return (double)t / reps;
+    };
+    if (f32) {
+        ms[0] = timed([&] {
+            k_pcg_k1_q4<float><<<g1, kBlock, 0, ctx->stream>>>(ctx->nx, ctx->ny, 0, ctx->f_p0, ctx->f_z, ctx->f_p1,
+                                                             ctx->f_q, ctx->d_Ef, ctx->d_K0f, ctx->d_fix, partB,
+                                                             (int)g2, partA, S, 0.0);
+        });
+        ms[1] = timed([&] {
+            k_pcg_k2<float><<<g2, kBlock, 0, ctx->stream>>>(ctx->nv, 0, ctx->f_x, ctx->f_r, ctx->f_z, ctx->f_p1,
+                                                          ctx->f_q, ctx->f_D, partA, (int)g1, partB, S);
+        });
+    } else {
+        ms[0] = timed([&] {
+            k_pcg_k1_q4<double><<<g1, kBlock, 0, ctx->stream>>>(ctx->nx, ctx->ny, 0, ctx->d_p0, ctx->d_z, ctx->d_p1,
+                                                              ctx->d_q, ctx->d_E, ctx->d_K0, ctx->d_fix, partB,
+                                                              (int)g2, partA, S, 0.0);
+        });
+        ms[1] = timed([&] {
+            k_pcg_k2<double><<<g2, kBlock, 0, ctx->stream>>>(ctx->nv, 0, ctx->d_x, ctx->d_r, ctx->d_z, ctx->d_p1,
+                                                           ctx->d_q, ctx->d_D, partA, (int)g1, partB, S);
+        });
+    }
+    ms[2] = ms[3] = 0.0;
+    if (ctx->have_filter) {
+        ms[2] = timed([&] {
+            k_filter_q4<<<gsize(ctx->ne), kBlock, 0, ctx->stream>>>(ctx->nx, ctx->ny, ctx->win, ctx->R, ctx->d_cg,
+                                                                   ctx->d_rho, ctx->d_e5);
+        });
+        OcDev init{};
+        init.l0i = 0.0;
+        init.l1i = 1e9;
+        hipMemcpyAsync(ctx->d_oc, &init, sizeof(OcDev), hipMemcpyHostToDevice, ctx->stream);
+        const unsigned ge = gsize(ctx->ne);
+        k_fill<double><<<ge, kBlock, 0, ctx->stream>>>(ctx->ne, ctx->d_e1, -1.0);
+        k_fill<double><<<ge, kBlock, 0, ctx->stream>>>(ctx->ne, ctx->d_e2, 1.0);
+        ms[3] = timed([&] {
+            k_oc_xplus<<<ge, kBlock, 0, ctx->stream>>>(ctx->ne, 0, ctx->d_rho, ctx->d_e1, ctx->d_e2, 0.5, 0.2, 1e-3,
+                                                      1.0, ctx->red.partials, (int)ge, ctx->d_e4, (OcDev*)ctx->d_oc);
+            k_oc_volume<<<ge, kBlock, 0, ctx->stream>>>(ctx->nx, ctx->ny, ctx->win, ctx->R, ctx->d_cg, ctx->d_e4, 0.5,
+                                                       1.0, ctx->red.partials, (OcDev*)ctx->d_oc);
+        });
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    PF_TRY(check_launch(ctx));
+    return sync(ctx);
+}
+
 int pf_precond_init(pf_ctx* ctx, int channels, uint64_t seed) {
     if (!ctx) return PF_ERR_ARG;
     ctx->err = "learned preconditioner is not available in this build";
