@@ -2,6 +2,7 @@
 // filter, OC and the device-resident SIMP loop.  Declarations and the
 // reference routine each entry point replaces: include/pansfem2_amd.h.
 #include "pf_kernels.hip"
+#include "pf_mma.hip"
 
 #include <algorithm>
 #include <chrono>
@@ -19,6 +20,19 @@ int grid_cap() {
         const char* s = std::getenv("PF_GRID_CAP");
         int v = s ? std::atoi(s) : 1024;
         return std::max(64, std::min(v, kMaxPartials));
+    }();
+    return cap;
+}
+
+constexpr int kTileJ = 64, kTileI = 8;  // nodes per LDS tile of the quad4 PCG operator kernel
+
+// workgroups of the streaming PCG vector kernel (few, so that the operator
+// kernel's redundant read of its partials stays small)
+int k2_cap() {
+    static int cap = [] {
+        const char* s = std::getenv("PF_K2_GRID");
+        int v = s ? std::atoi(s) : 512;
+        return std::max(32, std::min(v, kMaxPartials));
     }();
     return cap;
 }
@@ -192,8 +206,10 @@ int pcg_device(pf_ctx* ctx, bool csr, const T* b, T* x, const T* D, T* r, T* z, 
     PcgDev* S = (PcgDev*)ctx->d_pcg;
     double* partA = ctx->red.partials;
     double* partB = ctx->red.partials + 2 * kMaxPartials;
-    const unsigned g1 = csr ? gsize(ctx->N) : gsize(ctx->nn);
-    const unsigned g2 = gsize(n);
+    const int tilesJ = (ctx->ny + 1 + kTileJ - 1) / kTileJ;
+    const unsigned g1 = csr ? gsize(ctx->N) : (unsigned)(tilesJ * ((ctx->nx + 1 + kTileI - 1) / kTileI));
+    const unsigned g2 = std::min<unsigned>(gsize(n), (unsigned)k2_cap());
+    PF_CHECK(g1 <= (unsigned)kMaxPartials, PF_ERR_ARG, "grid too large for the reduction scratch");
     k_pcg_init<T><<<g2, kBlock, 0, ctx->stream>>>(n, b, D, x, r, z, p0, partB, S);
     PF_TRY(check_launch(ctx));
     const int chunk = 32;
@@ -217,8 +233,8 @@ int pcg_device(pf_ctx* ctx, bool csr, const T* b, T* x, const T* D, T* r, T* z, 
             } else if (is2d(ctx)) {
                 const T* Ev = std::is_same<T, float>::value ? (const T*)ctx->d_Ef : (const T*)ctx->d_E;
                 const T* Kv = std::is_same<T, float>::value ? (const T*)ctx->d_K0f : (const T*)ctx->d_K0;
-                k_pcg_k1_q4<T><<<g1, kBlock, 0, ctx->stream>>>(ctx->nx, ctx->ny, k, pold, z, pnew, q, Ev, Kv,
-                                                             ctx->d_fix, partB, (int)g2, partA, S, eps);
+                k_pcg_k1_q4_tile<T, kTileJ, kTileI><<<g1, kBlock, 0, ctx->stream>>>(
+                    ctx->nx, ctx->ny, tilesJ, k, pold, z, pnew, q, Ev, Kv, ctx->d_fix, partB, (int)g2, partA, S, eps);
             }
             k_pcg_k2<T><<<g2, kBlock, 0, ctx->stream>>>(n, k, x, r, z, pnew, q, D, partA, (int)g1, partB, S);
         }
@@ -460,6 +476,213 @@ int comp_sens_device(pf_ctx* ctx, const double* u, double scale0, double* dfdrho
     return rc;
 }
 
+// ---------------------------------------------------------------- MMA -----
+}  // namespace
+
+struct MmaState {
+    pf_mma_params prm{};
+    int k = 0;
+    double previousvalue = 0.0;
+    long n = 0;
+    double *xkm1 = nullptr, *xkm2 = nullptr, *L = nullptr, *U = nullptr, *alpha = nullptr, *beta = nullptr;
+    double *p0 = nullptr, *q0 = nullptr, *p = nullptr, *q = nullptr, *x = nullptr, *gsi = nullptr, *ita = nullptr;
+    double *G = nullptr, *Dx = nullptr, *dlx = nullptr, *dx = nullptr, *dgsi = nullptr, *dita = nullptr;
+    double *xp = nullptr, *gp = nullptr, *ip = nullptr;
+    std::vector<double> hpart;
+    std::vector<double**> buffers() {
+        return {&xkm1, &xkm2, &L, &U, &alpha, &beta, &p0, &q0, &p, &q, &x,
+                &gsi, &ita, &G, &Dx, &dlx, &dx, &dgsi, &dita, &xp, &gp, &ip};
+    }
+};
+
+namespace {
+
+void mma_free(MmaState* M) {
+    if (!M) return;
+    for (double** b : M->buffers()) dfree(*b);
+    delete M;
+}
+
+// Fixed-order host sum of K-wide workgroup partials.
+int host_partials(pf_ctx* ctx, const double* part, int nparts, int K, double* out, bool is_max = false) {
+    MmaState* M = ctx->mma;
+    M->hpart.resize((size_t)nparts * K);
+    PF_HIP(hipMemcpyAsync(M->hpart.data(), part, sizeof(double) * nparts * K, hipMemcpyDeviceToHost, ctx->stream));
+    PF_TRY(sync(ctx));
+    for (int k = 0; k < K; k++) {
+        double s = 0.0;
+        for (int i = 0; i < nparts; i++) {
+            double v = M->hpart[(size_t)i * K + k];
+            if (is_max) s = (s < v) ? v : s;
+            else s += v;
+        }
+        out[k] = s;
+    }
+    return PF_OK;
+}
+
+// MMA<T>::solvels for the 2x2 system (MMA.h:464-509).
+void solve2(double A[2][2], double b[2], double x[2]) {
+    double pivot = std::fabs(A[0][0]);
+    if (pivot < std::fabs(A[1][0])) {
+        std::swap(b[0], b[1]);
+        std::swap(A[0][0], A[1][0]);
+        std::swap(A[0][1], A[1][1]);
+    }
+    A[1][1] -= A[0][1] * A[1][0] / A[0][0];
+    b[1] -= b[0] * A[1][0] / A[0][0];
+    x[1] = b[1];
+    x[1] /= A[1][1];
+    x[0] = b[0];
+    x[0] -= x[1] * A[0][1];
+    x[0] /= A[0][0];
+}
+
+double max5(double a, double b, double c, double d, double e) {
+    double r = a;
+    if (r < b) r = b;
+    if (r < c) r = c;
+    if (r < d) r = d;
+    if (r < e) r = e;
+    return r;
+}
+
+// MMA<T>::UpdateVariables (MMA.h:116-419) for m = 1 on device arrays.
+int mma_update_device(pf_ctx* ctx, double* xk, double f, const double* dfdx, double g, const double* dgdx,
+                      int* steps_out) {
+    MmaState* M = ctx->mma;
+    PF_CHECK(M, PF_ERR_STATE, "MMA not initialised: call pf_mma_init first");
+    const long n = M->n;
+    const pf_mma_params& P = M->prm;
+    const double a = P.a, c = P.c, d = P.d, a0 = P.a0;
+    const unsigned gg = gsize(n);
+    double* part = ctx->red.partials;
+    k_mma_prep<<<gg, kBlock, 0, ctx->stream>>>(n, M->k < 2, xk, M->xkm1, M->xkm2, M->L, M->U, P.xmin, P.xmax,
+                                              P.asyinit, P.asydecr, P.asyincr, P.albefa, P.move, P.raa0, dfdx, dgdx,
+                                              M->alpha, M->beta, M->p0, M->q0, M->p, M->q, part);
+    PF_TRY(check_launch(ctx));
+    double Sb;
+    PF_TRY(host_partials(ctx, part, (int)gg, 1, &Sb));
+    const double b = -g + Sb;
+    double eps = 1.0, y = 1.0, z = 1.0, zeta = 1.0, lam = 1.0, s = 1.0;
+    double mu = (1.0 < 0.5 * c) ? 0.5 * c : 1.0;
+    k_mma_start<<<gg, kBlock, 0, ctx->stream>>>(n, M->alpha, M->beta, M->x, M->gsi, M->ita);
+    PF_TRY(check_launch(ctx));
+    auto kkt = [&](double Ksum, double gsum, double yv, double zv, double lv, double mv, double zetav, double sv) {
+        double norm = Ksum;
+        double t;
+        t = c + d * yv - lv - mv; norm += t * t;
+        t = gsum - a * zv - yv + sv - b; norm += t * t;
+        t = mv * yv - eps; norm += t * t;
+        t = lv * sv - eps; norm += t * t;
+        t = a0 - zetav - (0.0 + lv * a); norm += t * t;
+        t = zetav * zv - eps; norm += t * t;
+        return std::sqrt(norm);
+    };
+    int l = 0;
+    const int max_newton = P.max_newton > 0 ? P.max_newton : 500;
+    for (; eps > 1.0e-7; l++) {
+        if (l >= max_newton) {
+            ctx->err = "MMA interior-point loop reached max_newton";
+            if (steps_out) *steps_out = l;
+            return PF_NOT_CONVERGED;
+        }
+        k_mma_newton_a<<<gg, kBlock, 0, ctx->stream>>>(n, lam, eps, M->x, M->gsi, M->ita, M->alpha, M->beta, M->L,
+                                                      M->U, M->p0, M->q0, M->p, M->q, M->G, M->Dx, M->dlx, part);
+        PF_TRY(check_launch(ctx));
+        double S[4];
+        PF_TRY(host_partials(ctx, part, (int)gg, 4, S));
+        double Dy = d + mu / y;
+        double Dl = s / lam;
+        double dly = c + d * y - lam - eps / y;
+        double dlz = a0 - eps / z - (0.0 + lam * a);
+        double dll = -a * z - y - b + eps / lam;
+        dll += S[0];
+        double Dlly = Dl + 1.0 / Dy;
+        double dlly = dll + dly / Dy;
+        double A[2][2] = {{0.0 + S[1], a}, {a, -zeta / z}};
+        A[0][0] += Dlly;
+        double B[2] = {dlly - S[2], dlz};
+        double sol[2];
+        solve2(A, B, sol);
+        double dlam = sol[0], dz = sol[1];
+        k_mma_newton_b<<<gg, kBlock, 0, ctx->stream>>>(n, dlam, eps, M->x, M->gsi, M->ita, M->alpha, M->beta, M->G,
+                                                      M->Dx, M->dlx, M->dx, M->dgsi, M->dita, part);
+        PF_TRY(check_launch(ctx));
+        double txmax;
+        PF_TRY(host_partials(ctx, part, (int)gg, 1, &txmax, true));
+        double dy = dlam / Dy - dly / Dy;
+        double dmu = -mu * dy / y - mu + eps / y;
+        double ds = -s * dlam / lam - s + eps / lam;
+        double dzeta = -zeta * dz / z - zeta + eps / z;
+        double tymax = 0.0;
+        {
+            double t = -1.01 * dy / y, t2 = -1.01 * dlam / lam, t3 = -1.01 * dmu / mu, t4 = -1.01 * ds / s;
+            if (t < t2) t = t2;
+            if (t < t3) t = t3;
+            if (t < t4) t = t4;
+            if (tymax < t) tymax = t;
+        }
+        double tau = 1.0 / max5(1.0, txmax, tymax, -1.01 * dz / z, -1.01 * dzeta / zeta);
+        double deltawl = kkt(S[3], S[0], y, z, lam, mu, zeta, s);
+        double yp = y, lp = lam, mp = mu, sp = s, zp = z, zetap = zeta, dw1 = 0.0;
+        for (int ll = 0; ll < 50; ll++) {
+            yp = y + tau * dy;
+            lp = lam + tau * dlam;
+            mp = mu + tau * dmu;
+            sp = s + tau * ds;
+            zp = z + tau * dz;
+            zetap = zeta + tau * dzeta;
+            k_mma_trial<<<gg, kBlock, 0, ctx->stream>>>(n, tau, lp, eps, M->x, M->gsi, M->ita, M->dx, M->dgsi,
+                                                       M->dita, M->alpha, M->beta, M->L, M->U, M->p0, M->q0, M->p,
+                                                       M->q, M->xp, M->gp, M->ip, part);
+            PF_TRY(check_launch(ctx));
+            double T[2];
+            PF_TRY(host_partials(ctx, part, (int)gg, 2, T));
+            dw1 = kkt(T[0], T[1], yp, zp, lp, mp, zetap, sp);
+            if (dw1 < deltawl) break;
+            tau *= 0.5;
+        }
+        std::swap(M->x, M->xp);
+        std::swap(M->gsi, M->gp);
+        std::swap(M->ita, M->ip);
+        y = yp; z = zp; lam = lp; mu = mp; zeta = zetap; s = sp;
+        if (dw1 < 0.9 * eps) eps *= 0.1;
+        else eps *= 1.0;
+    }
+    M->previousvalue = f;
+    M->k++;
+    const size_t eb = sizeof(double) * n;
+    PF_HIP(hipMemcpyAsync(M->xkm2, M->xkm1, eb, hipMemcpyDeviceToDevice, ctx->stream));
+    PF_HIP(hipMemcpyAsync(M->xkm1, xk, eb, hipMemcpyDeviceToDevice, ctx->stream));
+    PF_HIP(hipMemcpyAsync(xk, M->x, eb, hipMemcpyDeviceToDevice, ctx->stream));
+    PF_TRY(sync(ctx));
+    if (steps_out) *steps_out = l;
+    return PF_OK;
+}
+
+int mma_init_device(pf_ctx* ctx, const pf_mma_params* prm) {
+    PF_CHECK(ctx->dim > 0, PF_ERR_STATE, "no grid");
+    PF_CHECK(prm && prm->m == 1, PF_ERR_UNSUPPORTED, "MMA on the device supports one constraint (m = 1)");
+    PF_CHECK(prm->xmax > prm->xmin, PF_ERR_ARG, "MMA: xmax must exceed xmin");
+    if (!ctx->mma || ctx->mma->n != ctx->ne) {
+        mma_free(ctx->mma);
+        ctx->mma = new MmaState();
+        ctx->mma->n = ctx->ne;
+        for (double** b : ctx->mma->buffers()) PF_TRY(dalloc(ctx, b, ctx->ne));
+    }
+    MmaState* M = ctx->mma;
+    M->prm = *prm;
+    M->k = 0;
+    M->previousvalue = 0.0;
+    const size_t eb = sizeof(double) * ctx->ne;
+    PF_HIP(hipMemsetAsync(M->xkm1, 0, eb, ctx->stream));
+    PF_HIP(hipMemsetAsync(M->xkm2, 0, eb, ctx->stream));
+    PF_HIP(hipMemsetAsync(M->L, 0, eb, ctx->stream));
+    PF_HIP(hipMemsetAsync(M->U, 0, eb, ctx->stream));
+    return sync(ctx);
+}
+
 }  // namespace
 
 // =================================================================== ABI ==
@@ -501,6 +724,8 @@ int pf_destroy(pf_ctx* ctx) {
     hipSetDevice(ctx->device);
     if (ctx->stream) hipStreamSynchronize(ctx->stream);
     free_grid(ctx);
+    mma_free(ctx->mma);
+    ctx->mma = nullptr;
     dfree(ctx->d_K0);
     dfree(ctx->d_K0f);
     dfree(ctx->red.partials);
@@ -816,16 +1041,25 @@ int pf_oc_update(pf_ctx* ctx, double* x, const double* dfdx, const double* dgdx,
 }
 
 int pf_mma_init(pf_ctx* ctx, const pf_mma_params* prm) {
-    if (!ctx) return PF_ERR_ARG;
-    ctx->err = "MMA is not available in this build";
-    return PF_ERR_UNSUPPORTED;
+    if (!ctx || !prm) return PF_ERR_ARG;
+    PF_TRY(set_device(ctx));
+    return mma_init_device(ctx, prm);
 }
 
 int pf_mma_update(pf_ctx* ctx, double* x, double f, const double* dfdx, double g, const double* dgdx,
                   int* newton_steps) {
-    if (!ctx) return PF_ERR_ARG;
-    ctx->err = "MMA is not available in this build";
-    return PF_ERR_UNSUPPORTED;
+    if (!ctx || !x || !dfdx || !dgdx) return PF_ERR_ARG;
+    PF_CHECK(ctx->mma, PF_ERR_STATE, "MMA not initialised: call pf_mma_init first");
+    PF_TRY(set_device(ctx));
+    size_t bytes = sizeof(double) * ctx->ne;
+    PF_HIP(hipMemcpyAsync(ctx->d_e0, x, bytes, hipMemcpyHostToDevice, ctx->stream));
+    PF_HIP(hipMemcpyAsync(ctx->d_e1, dfdx, bytes, hipMemcpyHostToDevice, ctx->stream));
+    PF_HIP(hipMemcpyAsync(ctx->d_e2, dgdx, bytes, hipMemcpyHostToDevice, ctx->stream));
+    int rc = mma_update_device(ctx, ctx->d_e0, f, ctx->d_e1, g, ctx->d_e2, newton_steps);
+    if (rc < 0) return rc;
+    PF_HIP(hipMemcpyAsync(x, ctx->d_e0, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    PF_TRY(sync(ctx));
+    return rc;
 }
 
 void pf_simp_default_params(pf_simp_params* p) {
@@ -847,7 +1081,6 @@ int pf_simp_run(pf_ctx* ctx, const pf_simp_params* prm, int iters, double* s, do
     if (!ctx || !prm || !s) return PF_ERR_ARG;
     PF_TRY(require_ready(ctx, false));
     PF_CHECK(iters >= 0, PF_ERR_ARG, "iters must be non-negative");
-    PF_CHECK(!prm->use_mma, PF_ERR_UNSUPPORTED, "MMA is not available in this build");
     PF_CHECK(!prm->learned, PF_ERR_UNSUPPORTED, "learned preconditioner is not available in this build");
     PF_TRY(set_device(ctx));
     ctx->E1 = prm->E1; ctx->E0 = prm->E0; ctx->p = prm->p; ctx->nu = prm->nu; ctx->t = prm->t;
@@ -863,6 +1096,7 @@ int pf_simp_run(pf_ctx* ctx, const pf_simp_params* prm, int iters, double* s, do
     pf_oc_params oc = prm->oc;
     oc.vf = prm->vf;
     oc.scale1 = prm->scale1;
+    if (prm->use_mma) PF_TRY(mma_init_device(ctx, &prm->mma));
     double prev_f = 0.0;
     int it = 0;
     for (; it < iters; it++) {
@@ -886,13 +1120,20 @@ int pf_simp_run(pf_ctx* ctx, const pf_simp_params* prm, int iters, double* s, do
         double f;
         PF_TRY(comp_sens_device(ctx, ctx->d_x, prm->scale0, d_dfdrho, &f));
         PF_TRY(filter_device(ctx, d_dfdrho, d_dfds));
-        bool conv = std::fabs(f - prev_f) / (f + prev_f) < 1.0e-5;  // OC::IsConvergence (OC.h:67-73)
+        // IsConvergence of OC (OC.h:67-73, eps 1e-5) / MMA (MMA.h:107-113, epsvalue)
+        const double ceps = prm->use_mma ? prm->mma.epsvalue : 1.0e-5;
+        bool conv = std::fabs(f - prev_f) / (f + prev_f) < ceps;
         double lam = 0.0;
         int steps = 0;
         if (!(prm->stop_on_convergence && conv)) {
-            int orc = oc_device(ctx, &oc, d_s, d_dfds, d_dgds, d_xplus, &lam, &steps);
-            if (orc < 0) return orc;
-            PF_HIP(hipMemcpyAsync(d_s, d_xplus, eb, hipMemcpyDeviceToDevice, ctx->stream));
+            if (prm->use_mma) {
+                int mrc = mma_update_device(ctx, d_s, f, d_dfds, g, d_dgds, &steps);
+                if (mrc != PF_OK) return mrc < 0 ? mrc : PF_NOT_CONVERGED;
+            } else {
+                int orc = oc_device(ctx, &oc, d_s, d_dfds, d_dgds, d_xplus, &lam, &steps);
+                if (orc < 0) return orc;
+                PF_HIP(hipMemcpyAsync(d_s, d_xplus, eb, hipMemcpyDeviceToDevice, ctx->stream));
+            }
         }
         PF_TRY(sync(ctx));
         double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -929,7 +1170,10 @@ int pf_profile_kernels(pf_ctx* ctx, int op, int reps, double* ms) {
     PcgDev* S = (PcgDev*)ctx->d_pcg;
     double* partA = ctx->red.partials;
     double* partB = ctx->red.partials + 2 * kMaxPartials;
-    const unsigned g1 = gsize(ctx->nn), g2 = gsize(ctx->nv);
+    const int tilesJ = (ctx->ny + 1 + kTileJ - 1) / kTileJ;
+    const unsigned g1 = (unsigned)(tilesJ * ((ctx->nx + 1 + kTileI - 1) / kTileI));
+    const unsigned g2 = std::min<unsigned>(gsize(ctx->nv), (unsigned)k2_cap());
+    PF_CHECK(g1 <= (unsigned)kMaxPartials, PF_ERR_ARG, "grid too large for the reduction scratch");
     // a defined state: b = load, x = 0, z = b/D, p = 0; launches use k = 0 (no
     // convergence test) so every repetition does the full work
     PF_TRY(diag_nodal(ctx, f32));
@@ -957,9 +1201,9 @@ int pf_profile_kernels(pf_ctx* ctx, int op, int reps, double* ms) {
     };
     if (f32) {
         ms[0] = timed([&] {
-            k_pcg_k1_q4<float><<<g1, kBlock, 0, ctx->stream>>>(ctx->nx, ctx->ny, 0, ctx->f_p0, ctx->f_z, ctx->f_p1,
-                                                             ctx->f_q, ctx->d_Ef, ctx->d_K0f, ctx->d_fix, partB,
-                                                             (int)g2, partA, S, 0.0);
+            k_pcg_k1_q4_tile<float, kTileJ, kTileI><<<g1, kBlock, 0, ctx->stream>>>(
+                ctx->nx, ctx->ny, tilesJ, 0, ctx->f_p0, ctx->f_z, ctx->f_p1, ctx->f_q, ctx->d_Ef, ctx->d_K0f, ctx->d_fix,
+                partB, (int)g2, partA, S, 0.0);
         });
         ms[1] = timed([&] {
             k_pcg_k2<float><<<g2, kBlock, 0, ctx->stream>>>(ctx->nv, 0, ctx->f_x, ctx->f_r, ctx->f_z, ctx->f_p1,
@@ -967,9 +1211,9 @@ int pf_profile_kernels(pf_ctx* ctx, int op, int reps, double* ms) {
         });
     } else {
         ms[0] = timed([&] {
-            k_pcg_k1_q4<double><<<g1, kBlock, 0, ctx->stream>>>(ctx->nx, ctx->ny, 0, ctx->d_p0, ctx->d_z, ctx->d_p1,
-                                                              ctx->d_q, ctx->d_E, ctx->d_K0, ctx->d_fix, partB,
-                                                              (int)g2, partA, S, 0.0);
+            k_pcg_k1_q4_tile<double, kTileJ, kTileI><<<g1, kBlock, 0, ctx->stream>>>(
+                ctx->nx, ctx->ny, tilesJ, 0, ctx->d_p0, ctx->d_z, ctx->d_p1, ctx->d_q, ctx->d_E, ctx->d_K0, ctx->d_fix,
+                partB, (int)g2, partA, S, 0.0);
         });
         ms[1] = timed([&] {
             k_pcg_k2<double><<<g2, kBlock, 0, ctx->stream>>>(ctx->nv, 0, ctx->d_x, ctx->d_r, ctx->d_z, ctx->d_p1,

@@ -59,7 +59,7 @@ __global__ void k_convert(long n, const A* __restrict__ a, B* __restrict__ b) {
 //
 // v[a][b] holds the nodal pair of node (i+a-1, j+b-1).
 template <class T>
-struct Vec2 { T x, y; };
+struct alignas(2 * sizeof(T)) Vec2 { T x, y; };
 
 template <class T>
 __device__ inline void elem_rows_q4(const T* __restrict__ K0, int loc, const Vec2<T> n0, const Vec2<T> n1,
@@ -104,12 +104,15 @@ __device__ inline void apply_q4_node(int nx, int ny, int i, int j, const Vec2<T>
     yy = ay;
 }
 
+// One node's pair as a single 16-byte (FP64) or 8-byte (FP32) access.
 template <class T>
 __device__ inline Vec2<T> load2(const T* __restrict__ a, long n) {
-    Vec2<T> r;
-    r.x = a[2 * n];
-    r.y = a[2 * n + 1];
-    return r;
+    return reinterpret_cast<const Vec2<T>*>(a)[n];
+}
+
+template <class T>
+__device__ inline void store2(T* __restrict__ a, long n, Vec2<T> v) {
+    reinterpret_cast<Vec2<T>*>(a)[n] = v;
 }
 
 // y = K x (nodal vectors), fixed DOFs of y set to zero when mask_fixed.
@@ -259,6 +262,87 @@ __global__ void __launch_bounds__(256) k_pcg_k1_q4(int nx, int ny, int k, const 
     store_partials<1>(acc, lds, partA);
 }
 
+// LDS-tiled K1 for quad4 grids.  A workgroup owns a TI x TJ tile of nodes
+// (TJ along j, the contiguous direction).  It stages p_new = beta p_old + z
+// for the tile and a one-node halo in LDS (each halo value computed once),
+// plus the moduli of the tile's elements, then every thread gathers its 3x3
+// neighbourhood from LDS.  Global traffic per node: p_old, z, E read once
+// (plus the halo), p_new and q written once.
+template <class T, int TJ, int TI>
+__global__ void __launch_bounds__(256) k_pcg_k1_q4_tile(int nx, int ny, int tilesJ, int k, const T* __restrict__ pold,
+                                                        const T* __restrict__ z, T* __restrict__ pnew,
+                                                        T* __restrict__ q, const T* __restrict__ E,
+                                                        const T* __restrict__ K0, const uint8_t* __restrict__ fix,
+                                                        const double* __restrict__ partB, int nbB,
+                                                        double* __restrict__ partA, PcgDev* __restrict__ S,
+                                                        double eps) {
+    __shared__ double lds[32];
+    __shared__ Vec2<T> sp[TI + 2][TJ + 2];
+    __shared__ T se[TI + 1][TJ + 1];
+    if (S->done) return;
+    double beta;
+    bool stop;
+    pcg_head(k, partB, nbB, S, eps, lds, beta, stop);
+    if (stop) return;
+    const T tb = (T)beta;
+    const int j0 = (blockIdx.x % tilesJ) * TJ, i0 = (blockIdx.x / tilesJ) * TI;
+    for (int h = threadIdx.x; h < (TI + 2) * (TJ + 2); h += blockDim.x) {
+        int a = h / (TJ + 2), b = h - a * (TJ + 2);
+        int ii = i0 - 1 + a, jj = j0 - 1 + b;
+        Vec2<T> v{0, 0};
+        if (ii >= 0 && ii <= nx && jj >= 0 && jj <= ny) {
+            long m = (long)(ny + 1) * ii + jj;
+            Vec2<T> po = load2(pold, m), zz = load2(z, m);
+            v.x = tb * po.x + zz.x;  // xeaxpy: p = beta p + z (CG.h:42-49)
+            v.y = tb * po.y + zz.y;
+        }
+        sp[a][b] = v;
+    }
+    for (int h = threadIdx.x; h < (TI + 1) * (TJ + 1); h += blockDim.x) {
+        int a = h / (TJ + 1), b = h - a * (TJ + 1);
+        int ei = i0 - 1 + a, ej = j0 - 1 + b;
+        se[a][b] = (ei >= 0 && ei < nx && ej >= 0 && ej < ny) ? E[(long)ny * ei + ej] : (T)0;
+    }
+    __syncthreads();
+    double acc[1] = {0.0};
+    const int c = threadIdx.x % TJ;
+    for (int r = threadIdx.x / TJ; r < TI; r += blockDim.x / TJ) {
+        const int i = i0 + r, j = j0 + c;
+        if (i > nx || j > ny) continue;
+        T ax = 0, ay = 0, ex, ey;
+        if (i >= 1 && j >= 1) {  // element (i-1, j-1): this node is TR
+            elem_rows_q4(K0, 2, sp[r][c], sp[r + 1][c], sp[r + 1][c + 1], sp[r][c + 1], ex, ey);
+            T Ee = se[r][c];
+            ax += Ee * ex; ay += Ee * ey;
+        }
+        if (i >= 1 && j < ny) {  // element (i-1, j): BR
+            elem_rows_q4(K0, 1, sp[r][c + 1], sp[r + 1][c + 1], sp[r + 1][c + 2], sp[r][c + 2], ex, ey);
+            T Ee = se[r][c + 1];
+            ax += Ee * ex; ay += Ee * ey;
+        }
+        if (i < nx && j >= 1) {  // element (i, j-1): TL
+            elem_rows_q4(K0, 3, sp[r + 1][c], sp[r + 2][c], sp[r + 2][c + 1], sp[r + 1][c + 1], ex, ey);
+            T Ee = se[r + 1][c];
+            ax += Ee * ex; ay += Ee * ey;
+        }
+        if (i < nx && j < ny) {  // element (i, j): BL
+            elem_rows_q4(K0, 0, sp[r + 1][c + 1], sp[r + 2][c + 1], sp[r + 2][c + 2], sp[r + 1][c + 2], ex, ey);
+            T Ee = se[r + 1][c + 1];
+            ax += Ee * ex; ay += Ee * ey;
+        }
+        const long n = (long)(ny + 1) * i + j;
+        uint8_t f = fix[n];
+        if (f & 1) ax = 0;
+        if (f & 2) ay = 0;
+        Vec2<T> pv = sp[r + 1][c + 1];
+        store2(pnew, n, pv);
+        store2(q, n, Vec2<T>{ax, ay});
+        acc[0] += (double)pv.x * (double)ax;
+        acc[0] += (double)pv.y * (double)ay;
+    }
+    store_partials<1>(acc, lds, partA);
+}
+
 // CSR flavour of K1 (free-DOF vectors): q_i = sum_j a_ij (beta pold_j + z_j),
 // each row summed in column order from 0 as CSR<T>::operator* does.
 __global__ void __launch_bounds__(256) k_pcg_k1_csr(int N, int k, const int32_t* __restrict__ indptr,
@@ -306,10 +390,29 @@ __global__ void __launch_bounds__(256) k_pcg_k2(long n, int k, T* __restrict__ x
     }
     const T ta = (T)alpha, tma = (T)(-alpha);
     double acc[2] = {0.0, 0.0};
-    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-        T xi = x[i] + ta * p[i];    // xexpay(x, alpha, p)
-        T ri = r[i] + tma * q[i];   // xexpay(r, -alpha, Ap)
-        T zi = ri / D[i];           // Scaling (CG.h:412)
+    // pairs of entries per access (n is even for nodal vectors); an odd tail
+    // (free-numbered CSR vectors) is handled by the scalar loop below
+    const long n2 = n / 2;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n2; i += (long)gridDim.x * blockDim.x) {
+        Vec2<T> xi = load2(x, i), ri = load2(r, i), pi = load2(p, i), qi = load2(q, i), di = load2(D, i);
+        xi.x = xi.x + ta * pi.x;   // xexpay(x, alpha, p)
+        xi.y = xi.y + ta * pi.y;
+        ri.x = ri.x + tma * qi.x;  // xexpay(r, -alpha, Ap)
+        ri.y = ri.y + tma * qi.y;
+        Vec2<T> zi{ri.x / di.x, ri.y / di.y};  // Scaling (CG.h:412)
+        store2(x, i, xi);
+        store2(r, i, ri);
+        store2(z, i, zi);
+        acc[0] += (double)zi.x * (double)ri.x;
+        acc[0] += (double)zi.y * (double)ri.y;
+        acc[1] += (double)ri.x * (double)ri.x;
+        acc[1] += (double)ri.y * (double)ri.y;
+    }
+    if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) {
+        long i = n - 1;
+        T xi = x[i] + ta * p[i];
+        T ri = r[i] + tma * q[i];
+        T zi = ri / D[i];
         x[i] = xi;
         r[i] = ri;
         z[i] = zi;

@@ -166,3 +166,39 @@ def test_operator_symmetry_and_residual_large(pa):
     # element matrices of the assembled path carry the grid's coordinate rounding (up to
     # ~1e-16 relative per entry); with cancellation in K x this stays below 1e-13
     assert rel(Kx, Kc) < 1e-13
+
+
+def test_mma_cantilever_against_reference(pa):
+    """src/Optimize/Solver/test_MMA.cpp through the device MMA (n = 5 as a 5x1 grid)."""
+    g = gi.load("mma_cantilever")["traj"].reshape(-1, 7)
+    ctx = pa.Context(0)
+    ctx.grid2d(5, 1)
+    ctx.mma_init(m=1, a0=1.0, a=0.0, c=1000.0, d=1.0, xmin=1.0, xmax=10.0, raa0=1e-5, albefa=0.1, move=0.5,
+                 asyinit=0.5, asydecr=0.7, asyincr=1.2, epsvalue=1e-5)
+    C1, cc = 0.0624, np.array([61.0, 37.0, 19.0, 7.0, 1.0])
+    s = np.full(5, 5.0)
+    rows = []
+    for k in range(len(g)):
+        obj = C1 * (s[0] + s[1] + s[2] + s[3] + s[4])
+        con = sum(c * math.pow(v, -3.0) for c, v in zip(cc, s)) - 1.0
+        rows.append([obj, con] + list(s))
+        dcon = np.array([-3.0 * c * math.pow(v, -4.0) for c, v in zip(cc, s)])
+        s, steps = ctx.mma_update(s, obj, np.full(5, C1), con, dcon)
+    rows = np.array(rows)
+    # n = 5 sums are evaluated in a different order than the reference's serial
+    # loops; the trajectory agrees to rounding
+    assert np.max(np.abs(rows - g) / np.maximum(np.abs(g), 1.0)) < 1e-10
+
+
+def test_simp_mma_loop_c1_against_reference(pa):
+    """50 MMA iterations at 120x40 (sample_optimize_density_mma.cpp loop)."""
+    case = "simp_120x40_mma"
+    g = gi.load(case)
+    ctx = pa.cantilever2d(120, 40)
+    r = ctx.simp_run(50, pa.simp_default_params(use_mma=1))
+    from oracle import pyoracle as po
+    ref = po.simp2d(120, 40, 50, po.simp_params(use_mma=1))
+    assert gi.sha(ref["f"]) == gi.digest(case, "hist_f")
+    assert np.max(np.abs(r["f"] - ref["f"]) / ref["f"]) < 1e-9
+    assert np.max(np.abs(r["s"] - g["s_final"])) < 1e-6
+    assert np.max(np.abs(r["rho"] - g["rho_final"])) < 1e-6
