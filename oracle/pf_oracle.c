@@ -381,6 +381,30 @@ long orc_filter_build(int nx, int ny, const double* cg, double R, int brute, int
     return k;
 }
 
+long orc_filter_build_brute(int ne, int dim, const double* cg, double R, int* ptr, int** idx, double** w) {
+    /* the sample driver's all-pairs search (sample_optimize_density_oc.cpp:53-60)
+     * for any dimension: Norm() sums pow(v, 2.0) per component (Vector.h:297-303) */
+    long cap = (long)ne * 32, k = 0;
+    int* I = (int*)malloc(sizeof(int) * (size_t)cap);
+    double* W = (double*)malloc(sizeof(double) * (size_t)cap);
+    ptr[0] = 0;
+    for (int i = 0; i < ne; i++) {
+        for (int j = 0; j < ne; j++) {
+            double s = 0.0;
+            for (int d = 0; d < dim; d++) s += pow2(cg[(long)i * dim + d] - cg[(long)j * dim + d]);
+            double dist = sqrt(s);
+            if (dist <= R) {
+                if (k == cap) { cap *= 2; I = (int*)realloc(I, sizeof(int) * cap); W = (double*)realloc(W, sizeof(double) * cap); }
+                I[k] = j; W[k] = (R - dist) / R; k++;
+            }
+        }
+        ptr[i + 1] = (int)k;
+    }
+    *idx = I;
+    *w = W;
+    return k;
+}
+
 void orc_filter_apply(int ne, const int* ptr, const int* idx, const double* w, const double* s, double* rho) {
     /* DensityFilter.h:45-56 */
     for (int i = 0; i < ne; i++) {

@@ -66,6 +66,8 @@ void orc_centroids(int ne, int nen, int dim, const int* conn, const double* node
  * (same candidates that can pass, same order).  ptr has ne+1 entries; idx/w
  * are malloc'd (orc_free). Returns the total entry count. */
 long orc_filter_build(int nx, int ny, const double* cg, double R, int brute, int* ptr, int** idx, double** w);
+/* All-pairs neighbour search for any dimension (dim = 2 or 3), cg is ne x dim. */
+long orc_filter_build_brute(int ne, int dim, const double* cg, double R, int* ptr, int** idx, double** w);
 /* DensityFilter<T>::GetFilteredVariables (src/Optimize/Filter/DensityFilter.h:45-56). */
 void orc_filter_apply(int ne, const int* ptr, const int* idx, const double* w, const double* s, double* rho);
 /* DensityFilter<T>::GetFilteredSensitivitis (DensityFilter.h:60-71): row-normalised. */

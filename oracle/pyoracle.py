@@ -49,6 +49,8 @@ def lib():
         L.orc_centroids.argtypes = [c_int, c_int, c_int, _i32p, _f64p, _f64p]
         L.orc_filter_build.argtypes = [c_int, c_int, _f64p, c_double, c_int, _i32p, pp_i, pp_d]
         L.orc_filter_build.restype = c_long
+        L.orc_filter_build_brute.argtypes = [c_int, c_int, _f64p, c_double, _i32p, pp_i, pp_d]
+        L.orc_filter_build_brute.restype = c_long
         L.orc_filter_apply.argtypes = [c_int, _i32p, _i32p, _f64p, _f64p, _f64p]
         L.orc_filter_sens.argtypes = [c_int, _i32p, _i32p, _f64p, _f64p, _f64p]
         L.orc_oc_update.argtypes = [c_int, _f64p, _f64p, _f64p, c_double, c_double, c_double, c_double, c_double,
@@ -166,6 +168,81 @@ def filter_build(nx, ny, cg, R=1.5, brute=False):
     L.orc_free(ctypes.cast(ix, ctypes.c_void_p))
     L.orc_free(ctypes.cast(w, ctypes.c_void_p))
     return ptr, idx, ww
+
+
+def filter_build_brute(cg, R=1.5):
+    ne, dim = cg.shape
+    ptr = np.zeros(ne + 1, np.int32)
+    ix = ctypes.POINTER(ctypes.c_int)()
+    w = ctypes.POINTER(ctypes.c_double)()
+    L = lib()
+    nent = L.orc_filter_build_brute(ne, dim, np.ascontiguousarray(cg, np.float64), R, ptr, ctypes.byref(ix),
+                                    ctypes.byref(w))
+    idx = np.ctypeslib.as_array(ix, (max(nent, 1),))[:nent].copy()
+    ww = np.ctypeslib.as_array(w, (max(nent, 1),))[:nent].copy()
+    L.orc_free(ctypes.cast(ix, ctypes.c_void_p))
+    L.orc_free(ctypes.cast(w, ctypes.c_void_p))
+    return ptr, idx, ww
+
+
+def grid3d(nx, ny, nz, lx=None, ly=None, lz=None):
+    """Structured hex8 grid in the library's numbering: node (i,j,k) ->
+    (ny+1)(nz+1) i + (nz+1) j + k, element -> ny nz i + nz j + k, nodes in
+    ShapeFunction8Cubic order; coordinates l*(i/(double)n) as SquareMesh."""
+    lx = float(nx) if lx is None else lx
+    ly = float(ny) if ly is None else ly
+    lz = float(nz) if lz is None else lz
+    I, J, K = np.meshgrid(np.arange(nx + 1), np.arange(ny + 1), np.arange(nz + 1), indexing="ij")
+    xyz = np.stack([lx * (I / float(nx)), ly * (J / float(ny)), lz * (K / float(nz))], -1).reshape(-1, 3)
+    node = lambda i, j, k: (ny + 1) * (nz + 1) * i + (nz + 1) * j + k
+    conn = np.zeros((nx * ny * nz, 8), np.int32)
+    e = 0
+    for i in range(nx):
+        for j in range(ny):
+            for k in range(nz):
+                conn[e] = [node(i, j, k), node(i + 1, j, k), node(i + 1, j + 1, k), node(i, j + 1, k),
+                           node(i, j, k + 1), node(i + 1, j, k + 1), node(i + 1, j + 1, k + 1), node(i, j + 1, k + 1)]
+                e += 1
+    return xyz, conn
+
+
+def simp3d(nx, ny, nz, iters, loads, E1=1.0, E0=1e-9, nu=0.3, p=3.0, vf=0.3, R=1.5, cg_eps=1e-10, s0=None):
+    """The SIMP loop (sample_optimize_density_oc.cpp:83-208) composed from the
+    oracle's routines on a hex8 grid with the x = 0 face clamped; `loads` is a
+    list of (node, dof, value).  Small grids only (all-pairs filter search)."""
+    xyz, conn = grid3d(nx, ny, nz)
+    nn, ne = xyz.shape[0], conn.shape[0]
+    fixed = np.arange((ny + 1) * (nz + 1))
+    N, n2g = dofmap(nn, 3, np.repeat(fixed, 3), np.tile(np.arange(3), len(fixed)))
+    fp, fi, fw = filter_build_brute(centroids(conn, xyz), R)
+    ke1 = np.stack([ke_h8(xyz[c], 1.0, nu) for c in conn])
+    s = np.full(ne, vf) if s0 is None else np.array(s0, np.float64)
+    hist = []
+    dgds = filter_sens(fp, fi, fw, np.full(ne, 1.0 / (vf * ne)))
+    for k in range(iters):
+        rho = filter_apply(fp, fi, fw, s)
+        g = 0.0
+        for r in rho:
+            g += 1.0 * r / (vf * ne)
+        g -= 1.0
+        E = [E1 * np.power(r, p) + E0 * (1.0 - np.power(r, p)) for r in rho]
+        ke = np.stack([ke_h8(xyz[c], e, nu) for c, e in zip(conn, E)])
+        indptr, indices, data, F = assemble_csr(N, conn, n2g, ke, 3)
+        for (nd, d, v) in loads:
+            if n2g[3 * nd + d] >= 0:
+                F[n2g[3 * nd + d]] += v
+        u, it = scaling_cg(indptr, indices, data, F, eps=cg_eps)
+        un = np.where(n2g >= 0, u[np.maximum(n2g, 0)], 0.0).reshape(-1, 3)
+        f = 0.0
+        dfdrho = np.zeros(ne)
+        for e in range(ne):
+            ue = un[conn[e]].reshape(-1)
+            f += float(ue @ (ke[e] @ ue))
+            dfdrho[e] = -p * (E1 - E0) * rho[e] ** (p - 1.0) * float(ue @ (ke1[e] @ ue))
+        dfds = filter_sens(fp, fi, fw, dfdrho)
+        hist.append((f, g, it))
+        s, lam, dec = oc_update(s, dfds, dgds, fp, fi, fw, vf=vf)
+    return dict(s=s, rho=filter_apply(fp, fi, fw, s), hist=hist, n2g=n2g, N=N)
 
 
 def filter_apply(ptr, idx, w, s):

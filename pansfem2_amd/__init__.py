@@ -334,3 +334,27 @@ def cantilever2d(nx, ny, device=0, ctx=None, E1=1.0, E0=1e-9, p=3.0, nu=0.3, t=1
     ctx.set_element_matrix(ctx.element_stiffness(0, 1.0, nu, t))
     ctx.filter_setup(R)
     return ctx
+
+
+def line_load_nodes3d(nx, ny, nz, total=-1.0):
+    """Unit line load along the bottom edge (y = 0) of the x = max face,
+    split consistently over the edge's nz + 1 nodes (end nodes half)."""
+    nodes = np.array([(ny + 1) * (nz + 1) * nx + k for k in range(nz + 1)], np.int32)
+    vals = np.full(nz + 1, total / nz)
+    vals[0] *= 0.5
+    vals[-1] *= 0.5
+    return nodes, np.ones(nz + 1, np.int32), vals
+
+
+def cantilever3d(nx, ny, nz, device=0, ctx=None, E1=1.0, E0=1e-9, p=3.0, nu=0.3, R=1.5):
+    """Hex8 SIMP grid: x = 0 face clamped, unit line load on the bottom edge
+    of the x = max face (-y), K0 = element 0 at E = 1."""
+    ctx = ctx or Context(device)
+    ctx.grid3d(nx, ny, nz, float(nx), float(ny), float(nz))
+    face = np.arange((ny + 1) * (nz + 1), dtype=np.int32)
+    ctx.set_dirichlet(np.repeat(face, 3), np.tile(np.arange(3, dtype=np.int32), len(face)))
+    ctx.set_point_loads(*line_load_nodes3d(nx, ny, nz))
+    ctx.set_material(E1, E0, p, nu, 1.0)
+    ctx.set_element_matrix(ctx.element_stiffness(0, 1.0, nu))
+    ctx.filter_setup(R)
+    return ctx

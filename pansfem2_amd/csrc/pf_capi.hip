@@ -3,6 +3,7 @@
 // reference routine each entry point replaces: include/pansfem2_amd.h.
 #include "pf_kernels.hip"
 #include "pf_mma.hip"
+#include "pf_hex8.hip"
 
 #include <algorithm>
 #include <chrono>
@@ -77,6 +78,7 @@ int check_launch(pf_ctx* ctx) {
 }
 
 bool is2d(pf_ctx* ctx) { return ctx->dim == 2; }
+G3 g3(pf_ctx* ctx) { return G3{ctx->nx, ctx->ny, ctx->nz}; }
 
 void free_grid(pf_ctx* ctx) {
     dfree(ctx->d_n2g); dfree(ctx->d_g2n); dfree(ctx->d_fix);
@@ -162,8 +164,14 @@ int apply_nodal(pf_ctx* ctx, bool f32, const void* x, void* y, bool mask) {
                                                                          mask);
         return check_launch(ctx);
     }
-    ctx->err = "3D operator not available";
-    return PF_ERR_UNSUPPORTED;
+    if (f32)
+        k_apply_h8<float><<<gsize(ctx->nn), kBlock, 0, ctx->stream>>>(g3(ctx), (const float*)x, (float*)y, ctx->d_Ef,
+                                                                    ctx->d_K0f, ctx->d_fix, mask, nullptr, nullptr);
+    else
+        k_apply_h8<double><<<gsize(ctx->nn), kBlock, 0, ctx->stream>>>(g3(ctx), (const double*)x, (double*)y,
+                                                                     ctx->d_E, ctx->d_K0, ctx->d_fix, mask, nullptr,
+                                                                     nullptr);
+    return check_launch(ctx);
 }
 
 int diag_nodal(pf_ctx* ctx, bool f32) {
@@ -176,8 +184,13 @@ int diag_nodal(pf_ctx* ctx, bool f32) {
                                                                         ctx->d_fix, ctx->d_D);
         return check_launch(ctx);
     }
-    ctx->err = "3D operator not available";
-    return PF_ERR_UNSUPPORTED;
+    if (f32)
+        k_diag_h8<float><<<gsize(ctx->nn), kBlock, 0, ctx->stream>>>(g3(ctx), ctx->d_Ef, ctx->d_K0f, ctx->d_fix,
+                                                                   ctx->f_D);
+    else
+        k_diag_h8<double><<<gsize(ctx->nn), kBlock, 0, ctx->stream>>>(g3(ctx), ctx->d_E, ctx->d_K0, ctx->d_fix,
+                                                                    ctx->d_D);
+    return check_launch(ctx);
 }
 
 int to_nodal(pf_ctx* ctx, const double* xf_dev, double* xn_dev) {
@@ -207,7 +220,9 @@ int pcg_device(pf_ctx* ctx, bool csr, const T* b, T* x, const T* D, T* r, T* z, 
     double* partA = ctx->red.partials;
     double* partB = ctx->red.partials + 2 * kMaxPartials;
     const int tilesJ = (ctx->ny + 1 + kTileJ - 1) / kTileJ;
-    const unsigned g1 = csr ? gsize(ctx->N) : (unsigned)(tilesJ * ((ctx->nx + 1 + kTileI - 1) / kTileI));
+    const bool h8 = !csr && !is2d(ctx);
+    const unsigned g1 = csr ? gsize(ctx->N)
+                            : (h8 ? gsize(ctx->nn) : (unsigned)(tilesJ * ((ctx->nx + 1 + kTileI - 1) / kTileI)));
     const unsigned g2 = std::min<unsigned>(gsize(n), (unsigned)k2_cap());
     PF_CHECK(g1 <= (unsigned)kMaxPartials, PF_ERR_ARG, "grid too large for the reduction scratch");
     k_pcg_init<T><<<g2, kBlock, 0, ctx->stream>>>(n, b, D, x, r, z, p0, partB, S);
@@ -226,6 +241,15 @@ int pcg_device(pf_ctx* ctx, bool csr, const T* b, T* x, const T* D, T* r, T* z, 
         for (; k < kend; k++) {
             T* pold = (k & 1) ? p1 : p0;
             T* pnew = (k & 1) ? p0 : p1;
+            if (h8) {
+                // three launches: p = beta p + z in place (head + convergence), q = K p, update
+                const T* Ev = std::is_same<T, float>::value ? (const T*)ctx->d_Ef : (const T*)ctx->d_E;
+                const T* Kv = std::is_same<T, float>::value ? (const T*)ctx->d_K0f : (const T*)ctx->d_K0;
+                k_pcg_k3<T><<<g2, kBlock, 0, ctx->stream>>>(n, k, p0, z, partB, (int)g2, S, eps);
+                k_pcg_k1_h8<T><<<g1, kBlock, 0, ctx->stream>>>(g3(ctx), p0, q, Ev, Kv, ctx->d_fix, partA, S);
+                k_pcg_k2<T><<<g2, kBlock, 0, ctx->stream>>>(n, k, x, r, z, p0, q, D, partA, (int)g1, partB, S);
+                continue;
+            }
             if (csr) {
                 k_pcg_k1_csr<<<g1, kBlock, 0, ctx->stream>>>(ctx->N, k, ctx->d_indptr, ctx->d_indices, ctx->d_data,
                                                             (const double*)pold, (const double*)z, (double*)pnew,
@@ -284,12 +308,12 @@ double reduce_host(pf_ctx* ctx, const double* part, int nparts, double scale, do
 // True FP64 residual ||b - K x|| of nodal vectors (EbE operator).
 int true_residual(pf_ctx* ctx, const double* b, const double* x, double* r, double* out_norm) {
     const unsigned g = gsize(ctx->nn);
-    if (!is2d(ctx)) {
-        ctx->err = "3D residual not available";
-        return PF_ERR_UNSUPPORTED;
-    }
-    k_residual_q4<<<g, kBlock, 0, ctx->stream>>>(ctx->nx, ctx->ny, b, x, r, ctx->d_E, ctx->d_K0, ctx->d_fix,
-                                                ctx->red.partials);
+    if (is2d(ctx))
+        k_residual_q4<<<g, kBlock, 0, ctx->stream>>>(ctx->nx, ctx->ny, b, x, r, ctx->d_E, ctx->d_K0, ctx->d_fix,
+                                                    ctx->red.partials);
+    else
+        k_apply_h8<double><<<g, kBlock, 0, ctx->stream>>>(g3(ctx), x, r, ctx->d_E, ctx->d_K0, ctx->d_fix, 1, b,
+                                                        ctx->red.partials);
     PF_TRY(check_launch(ctx));
     int rc;
     double rr = reduce_host(ctx, ctx->red.partials, (int)g, 1.0, 0.0, &rc);
@@ -302,6 +326,85 @@ double norm_nodal(pf_ctx* ctx, const double* a, int* rc) {
     const unsigned g = gsize(ctx->nv);
     k_dot_partials<<<g, kBlock, 0, ctx->stream>>>(ctx->nv, a, a, ctx->red.partials);
     return std::sqrt(reduce_host(ctx, ctx->red.partials, (int)g, 1.0, 0.0, rc));
+}
+
+// Mixed precision: FP32 matrix-free Jacobi-PCG (FP32 vectors, FP32 operator,
+// FP64 dot accumulation) whose residual is replaced by the FP64 true residual
+// b - K x (x accumulated in FP64) each time the FP32 recursive residual has
+// dropped by `delta` since the last replacement: iterative refinement that
+// keeps the Krylov space (reliable updates) instead of restarting CG.
+int pcg_mixed(pf_ctx* ctx, const double* b, double* x, double eps, int itrmax, double delta, pf_solve_stats& st) {
+    const long n = ctx->nv;
+    PcgDev* S = (PcgDev*)ctx->d_pcg;
+    double* partA = ctx->red.partials;
+    double* partB = ctx->red.partials + 2 * kMaxPartials;
+    const int tilesJ = (ctx->ny + 1 + kTileJ - 1) / kTileJ;
+    const bool h8 = !is2d(ctx);
+    const unsigned g1 = h8 ? gsize(ctx->nn) : (unsigned)(tilesJ * ((ctx->nx + 1 + kTileI - 1) / kTileI));
+    const unsigned g2 = std::min<unsigned>(gsize(n), (unsigned)k2_cap());
+    PF_CHECK(g1 <= (unsigned)kMaxPartials, PF_ERR_ARG, "grid too large for the reduction scratch");
+    PF_TRY(diag_nodal(ctx, true));
+    int rc;
+    double bn = norm_nodal(ctx, b, &rc);
+    PF_TRY(rc);
+    PF_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, ctx->stream));
+    k_convert<double, float><<<gsize(n), kBlock, 0, ctx->stream>>>(n, b, ctx->f_b);
+    k_pcg_init<float><<<g2, kBlock, 0, ctx->stream>>>(n, ctx->f_b, ctx->f_D, ctx->f_x, ctx->f_r, ctx->f_z, ctx->f_p0,
+                                                     partB, S);
+    PF_TRY(check_launch(ctx));
+    const int chunk = 64;
+    int k = 0, replacements = 0, status = 2;
+    double rlast = bn, tn = bn;
+    volatile PcgDev* H = (volatile PcgDev*)ctx->h_pcg;
+    while (k < itrmax) {
+        int kend = std::min(itrmax, k + chunk);
+        for (; k < kend; k++) {
+            if (h8) {
+                k_pcg_k3<float><<<g2, kBlock, 0, ctx->stream>>>(n, k, ctx->f_p0, ctx->f_z, partB, (int)g2, S, 0.0);
+                k_pcg_k1_h8<float><<<g1, kBlock, 0, ctx->stream>>>(g3(ctx), ctx->f_p0, ctx->f_q, ctx->d_Ef, ctx->d_K0f,
+                                                                 ctx->d_fix, partA, S);
+                k_pcg_k2<float><<<g2, kBlock, 0, ctx->stream>>>(n, k, ctx->f_x, ctx->f_r, ctx->f_z, ctx->f_p0,
+                                                              ctx->f_q, ctx->f_D, partA, (int)g1, partB, S);
+                continue;
+            }
+            float* pold = (k & 1) ? ctx->f_p1 : ctx->f_p0;
+            float* pnew = (k & 1) ? ctx->f_p0 : ctx->f_p1;
+            k_pcg_k1_q4_tile<float, kTileJ, kTileI><<<g1, kBlock, 0, ctx->stream>>>(
+                ctx->nx, ctx->ny, tilesJ, k, pold, ctx->f_z, pnew, ctx->f_q, ctx->d_Ef, ctx->d_K0f, ctx->d_fix, partB,
+                (int)g2, partA, S, 0.0);
+            k_pcg_k2<float><<<g2, kBlock, 0, ctx->stream>>>(n, k, ctx->f_x, ctx->f_r, ctx->f_z, pnew, ctx->f_q,
+                                                          ctx->f_D, partA, (int)g1, partB, S);
+        }
+        PF_TRY(check_launch(ctx));
+        k_pcg_final<<<1, kBlock, 0, ctx->stream>>>(k, partB, (int)g2, S, 0.0);  // recursive ||r|| of the last K2
+        PF_HIP(hipMemcpyAsync((void*)H, S, sizeof(PcgDev), hipMemcpyDeviceToHost, ctx->stream));
+        PF_TRY(sync(ctx));
+        const double rrec = H->rnorm;
+        // k_pcg_final marked the solve done; clear it for the next chunk
+        PF_HIP(hipMemsetAsync(&S->done, 0, sizeof(int), ctx->stream));
+        if (!std::isfinite(rrec)) {
+            status = 3;
+            break;
+        }
+        if (rrec <= delta * rlast || rrec <= eps * bn || k >= itrmax) {
+            k_axpy_f32<<<gsize(n), kBlock, 0, ctx->stream>>>(n, ctx->f_x, x);
+            PF_TRY(true_residual(ctx, b, x, ctx->d_t0, &tn));
+            replacements++;
+            if (tn <= eps * bn) {
+                status = 1;
+                break;
+            }
+            k_mixed_replace<<<g2, kBlock, 0, ctx->stream>>>(n, ctx->d_t0, ctx->f_r, ctx->f_D, ctx->f_z, ctx->f_x,
+                                                           partB);
+            PF_TRY(check_launch(ctx));
+            rlast = tn;
+        }
+    }
+    st.iterations = k;
+    st.status = status;
+    st.outer = replacements;
+    st.rel_residual = bn > 0 ? tn / bn : 0.0;
+    return PF_OK;
 }
 
 // Solve K x = b for nodal device vectors b, x with the selected operator.
@@ -347,43 +450,12 @@ int solve_nodal(pf_ctx* ctx, int op, const double* b, double* x, double eps, int
         s.status = run.status;
         s.rel_residual = run.bnorm > 0 ? run.rnorm / run.bnorm : 0.0;
     } else if (op == PF_OP_EBE32) {
-        // FP32 EbE Jacobi-PCG inner solves inside FP64 iterative refinement:
-        // r = b - K x (FP64), solve K d = r in FP32 to inner_eps, x += d,
-        // until ||b - K x|| <= eps ||b||.
-        PF_TRY(diag_nodal(ctx, true));
-        int rc;
-        double bn = norm_nodal(ctx, b, &rc);
-        PF_TRY(rc);
-        PF_HIP(hipMemsetAsync(x, 0, sizeof(double) * ctx->nv, ctx->stream));
-        double rn = bn;
-        int outer = 0, total = 0, status = 2;
-        const int max_outer = 60;
-        double* r64 = ctx->d_t0;
-        while (outer < max_outer) {
-            PF_TRY(true_residual(ctx, b, x, r64, &rn));
-            if (bn == 0.0 || rn <= eps * bn) {
-                status = 1;
-                break;
-            }
-            if (total >= itrmax) break;
-            k_convert<double, float><<<gsize(ctx->nv), kBlock, 0, ctx->stream>>>(ctx->nv, r64, ctx->f_b);
-            PcgRun run;
-            PF_TRY(pcg_device<float>(ctx, false, ctx->f_b, ctx->f_x, ctx->f_D, ctx->f_r, ctx->f_z, ctx->f_p0,
-                                     ctx->f_p1, ctx->f_q, inner_eps, std::max(1, itrmax - total), run));
-            total += run.iterations > 0 ? run.iterations : itrmax;
-            k_axpy_f32<<<gsize(ctx->nv), kBlock, 0, ctx->stream>>>(ctx->nv, ctx->f_x, x);
-            PF_TRY(check_launch(ctx));
-            outer++;
-        }
-        s.iterations = total;
-        s.status = status;
-        s.outer = outer;
-        s.rel_residual = bn > 0 ? rn / bn : 0.0;
+        PF_TRY(pcg_mixed(ctx, b, x, eps, itrmax, 0.1, s));
     } else {
         ctx->err = "unknown operator";
         return PF_ERR_ARG;
     }
-    if (is2d(ctx)) {
+    {
         double trn;
         int rc;
         PF_TRY(true_residual(ctx, b, x, ctx->d_t0, &trn));
@@ -418,8 +490,12 @@ int oc_device(pf_ctx* ctx, const pf_oc_params* prm, const double* x, const doubl
         for (; k < kend; k++) {
             k_oc_xplus<<<g, kBlock, 0, ctx->stream>>>(ctx->ne, k, x, dfdx, dgdx, prm->iota, prm->movelimit,
                                                      prm->lambdaeps, prm->scale1, ctx->red.partials, (int)g, xplus, S);
-            k_oc_volume<<<g, kBlock, 0, ctx->stream>>>(ctx->nx, ctx->ny, ctx->win, ctx->R, ctx->d_cg, xplus, prm->vf,
-                                                      prm->scale1, ctx->red.partials, S);
+            if (is2d(ctx))
+                k_oc_volume<<<g, kBlock, 0, ctx->stream>>>(ctx->nx, ctx->ny, ctx->win, ctx->R, ctx->d_cg, xplus,
+                                                          prm->vf, prm->scale1, ctx->red.partials, S);
+            else
+                k_oc_volume_h8<<<g, kBlock, 0, ctx->stream>>>(g3(ctx), ctx->win, ctx->R, ctx->d_cg, xplus, prm->vf,
+                                                             prm->scale1, ctx->red.partials, S);
         }
         PF_TRY(check_launch(ctx));
         PF_HIP(hipMemcpyAsync((void*)H, S, sizeof(OcDev), hipMemcpyDeviceToHost, ctx->stream));
@@ -444,11 +520,11 @@ int oc_device(pf_ctx* ctx, const pf_oc_params* prm, const double* x, const doubl
 
 int filter_device(pf_ctx* ctx, const double* in, double* out) {
     PF_CHECK(ctx->have_filter, PF_ERR_STATE, "no filter: call pf_filter_setup first");
-    if (!is2d(ctx)) {
-        ctx->err = "3D filter not available";
-        return PF_ERR_UNSUPPORTED;
-    }
-    k_filter_q4<<<gsize(ctx->ne), kBlock, 0, ctx->stream>>>(ctx->nx, ctx->ny, ctx->win, ctx->R, ctx->d_cg, in, out);
+    if (is2d(ctx))
+        k_filter_q4<<<gsize(ctx->ne), kBlock, 0, ctx->stream>>>(ctx->nx, ctx->ny, ctx->win, ctx->R, ctx->d_cg, in,
+                                                               out);
+    else
+        k_filter_h8<<<gsize(ctx->ne), kBlock, 0, ctx->stream>>>(g3(ctx), ctx->win, ctx->R, ctx->d_cg, in, out);
     return check_launch(ctx);
 }
 
@@ -462,14 +538,14 @@ int moduli_from_rho(pf_ctx* ctx) {
 
 // f = scale0 sum_e E_e u_e^T K0 u_e and dfdrho (device, nodal u).
 int comp_sens_device(pf_ctx* ctx, const double* u, double scale0, double* dfdrho, double* f) {
-    if (!is2d(ctx)) {
-        ctx->err = "3D compliance not available";
-        return PF_ERR_UNSUPPORTED;
-    }
     const double coef = ((-scale0) * ctx->p) * ((-ctx->E0) + ctx->E1);
     const unsigned g = gsize(ctx->ne);
-    k_comp_sens_q4<<<g, kBlock, 0, ctx->stream>>>(ctx->nx, ctx->ny, u, ctx->d_E, ctx->d_rho, ctx->d_K0, coef,
-                                                 ctx->p - 1.0, dfdrho, ctx->red.partials);
+    if (is2d(ctx))
+        k_comp_sens_q4<<<g, kBlock, 0, ctx->stream>>>(ctx->nx, ctx->ny, u, ctx->d_E, ctx->d_rho, ctx->d_K0, coef,
+                                                     ctx->p - 1.0, dfdrho, ctx->red.partials);
+    else
+        k_comp_sens_h8<<<g, kBlock, 0, ctx->stream>>>(g3(ctx), u, ctx->d_E, ctx->d_rho, ctx->d_K0, coef, ctx->p - 1.0,
+                                                     dfdrho, ctx->red.partials);
     PF_TRY(check_launch(ctx));
     int rc;
     *f = reduce_host(ctx, ctx->red.partials, (int)g, scale0, 0.0, &rc);
@@ -765,8 +841,24 @@ int pf_grid2d(pf_ctx* ctx, int nx, int ny, double lx, double ly) {
 
 int pf_grid3d(pf_ctx* ctx, int nx, int ny, int nz, double lx, double ly, double lz) {
     if (!ctx) return PF_ERR_ARG;
-    ctx->err = "pf_grid3d: hex8 grids are not available in this build";
-    return PF_ERR_UNSUPPORTED;
+    PF_CHECK(nx >= 1 && ny >= 1 && nz >= 1 && lx > 0 && ly > 0 && lz > 0, PF_ERR_ARG,
+             "pf_grid3d: nx, ny, nz >= 1 and lx, ly, lz > 0 required");
+    PF_CHECK((long)(nx + 1) * (ny + 1) * (nz + 1) * 3 < (1L << 31), PF_ERR_ARG,
+             "pf_grid3d: grid exceeds int32 DOF numbering");
+    PF_TRY(set_device(ctx));
+    free_grid(ctx);
+    ctx->dim = 3;
+    ctx->nx = nx; ctx->ny = ny; ctx->nz = nz;
+    ctx->lx = lx; ctx->ly = ly; ctx->lz = lz;
+    ctx->nn = (long)(nx + 1) * (ny + 1) * (nz + 1);
+    ctx->ne = (long)nx * ny * nz;
+    ctx->ndpn = 3;
+    ctx->nd = 24;
+    ctx->nv = ctx->nn * 3;
+    PF_TRY(alloc_grid(ctx));
+    k_centroids_h8<<<gsize(ctx->ne), kBlock, 0, ctx->stream>>>(g3(ctx), lx, ly, lz, ctx->d_cg);
+    PF_TRY(check_launch(ctx));
+    return sync(ctx);
 }
 
 int pf_set_dirichlet(pf_ctx* ctx, const int32_t* node, const int32_t* dof, int nfixed, int32_t* n_free) {
@@ -833,15 +925,19 @@ int pf_set_element_matrix(pf_ctx* ctx, const double* K0, int nd) {
 
 int pf_element_stiffness(pf_ctx* ctx, int e, double E, double nu, double t, double* Ke) {
     if (!ctx || !Ke) return PF_ERR_ARG;
-    PF_CHECK(ctx->dim == 2, PF_ERR_UNSUPPORTED, "pf_element_stiffness: quad4 grids only in this build");
+    PF_CHECK(ctx->dim == 2 || ctx->dim == 3, PF_ERR_STATE, "no grid");
     PF_CHECK(e >= 0 && e < ctx->ne, PF_ERR_ARG, "element id out of range");
     PF_TRY(set_device(ctx));
+    const int nd2 = ctx->nd * ctx->nd;
     double* dK;
-    PF_HIP(hipMalloc((void**)&dK, sizeof(double) * 64 * 1));
+    PF_HIP(hipMalloc((void**)&dK, sizeof(double) * nd2));
     // compute only element e: one workgroup, one active thread
-    k_ke_one_q4<<<1, 64, 0, ctx->stream>>>(ctx->nx, ctx->ny, ctx->lx, ctx->ly, e, E, nu, t, dK);
+    if (is2d(ctx))
+        k_ke_one_q4<<<1, 64, 0, ctx->stream>>>(ctx->nx, ctx->ny, ctx->lx, ctx->ly, e, E, nu, t, dK);
+    else
+        k_ke_one_h8<<<1, 64, 0, ctx->stream>>>(g3(ctx), ctx->lx, ctx->ly, ctx->lz, e, E, nu, dK);
     hipError_t le = hipGetLastError();
-    hipError_t ce = hipMemcpyAsync(Ke, dK, sizeof(double) * 64, hipMemcpyDeviceToHost, ctx->stream);
+    hipError_t ce = hipMemcpyAsync(Ke, dK, sizeof(double) * nd2, hipMemcpyDeviceToHost, ctx->stream);
     hipError_t se = hipStreamSynchronize(ctx->stream);
     hipFree(dK);
     PF_HIP(le);
@@ -884,13 +980,15 @@ int pf_set_moduli(pf_ctx* ctx, const double* E) {
 int pf_assemble_csr(pf_ctx* ctx, int64_t* nnz) {
     if (!ctx) return PF_ERR_ARG;
     PF_TRY(require_ready(ctx, true));
-    PF_CHECK(is2d(ctx), PF_ERR_UNSUPPORTED, "pf_assemble_csr: quad4 grids only in this build");
     PF_TRY(set_device(ctx));
     const int N = ctx->N;
     PF_TRY(dalloc(ctx, &ctx->d_indptr, N + 1));
     int32_t* rowlen = nullptr;
     PF_TRY(dalloc(ctx, &rowlen, N));
-    k_csr_count_q4<<<gsize(ctx->nn), kBlock, 0, ctx->stream>>>(ctx->nx, ctx->ny, ctx->d_n2g, ctx->d_fix, rowlen);
+    if (is2d(ctx))
+        k_csr_count_q4<<<gsize(ctx->nn), kBlock, 0, ctx->stream>>>(ctx->nx, ctx->ny, ctx->d_n2g, ctx->d_fix, rowlen);
+    else
+        k_csr_count_h8<<<gsize(ctx->nn), kBlock, 0, ctx->stream>>>(g3(ctx), ctx->d_n2g, ctx->d_fix, rowlen);
     k_scan<<<1, 1024, 0, ctx->stream>>>(rowlen, ctx->d_indptr, N);
     int32_t total = 0;
     hipError_t e1 = hipGetLastError();
@@ -903,11 +1001,18 @@ int pf_assemble_csr(pf_ctx* ctx, int64_t* nnz) {
     ctx->nnz = total;
     PF_TRY(dalloc(ctx, &ctx->d_indices, std::max<long>(total, 1)));
     PF_TRY(dalloc(ctx, &ctx->d_data, std::max<long>(total, 1)));
-    PF_TRY(dalloc(ctx, &ctx->d_ke_all, ctx->ne * 64));
-    k_ke_all_q4<<<gsize(ctx->ne), kBlock, 0, ctx->stream>>>(ctx->nx, ctx->ny, ctx->lx, ctx->ly, ctx->d_E, ctx->nu,
-                                                           ctx->t, ctx->d_ke_all);
-    k_csr_fill_q4<<<gsize(ctx->nn), kBlock, 0, ctx->stream>>>(ctx->nx, ctx->ny, ctx->d_n2g, ctx->d_indptr,
-                                                             ctx->d_ke_all, ctx->d_indices, ctx->d_data);
+    PF_TRY(dalloc(ctx, &ctx->d_ke_all, ctx->ne * (long)ctx->nd * ctx->nd));
+    if (is2d(ctx)) {
+        k_ke_all_q4<<<gsize(ctx->ne), kBlock, 0, ctx->stream>>>(ctx->nx, ctx->ny, ctx->lx, ctx->ly, ctx->d_E, ctx->nu,
+                                                               ctx->t, ctx->d_ke_all);
+        k_csr_fill_q4<<<gsize(ctx->nn), kBlock, 0, ctx->stream>>>(ctx->nx, ctx->ny, ctx->d_n2g, ctx->d_indptr,
+                                                                 ctx->d_ke_all, ctx->d_indices, ctx->d_data);
+    } else {
+        k_ke_all_h8<<<gsize(ctx->ne), kBlock, 0, ctx->stream>>>(g3(ctx), ctx->lx, ctx->ly, ctx->lz, ctx->d_E, ctx->nu,
+                                                               ctx->d_ke_all);
+        k_csr_fill_h8<<<gsize(ctx->nn), kBlock, 0, ctx->stream>>>(g3(ctx), ctx->d_n2g, ctx->d_indptr, ctx->d_ke_all,
+                                                                 ctx->d_indices, ctx->d_data);
+    }
     PF_TRY(check_launch(ctx));
     PF_TRY(sync(ctx));
     dfree(ctx->d_ke_all);
@@ -1008,7 +1113,9 @@ int pf_filter_setup(pf_ctx* ctx, double R) {
     double h = std::min(ctx->lx / ctx->nx, ctx->ly / ctx->ny);
     if (ctx->dim == 3) h = std::min(h, ctx->lz / ctx->nz);
     ctx->R = R;
-    ctx->win = (int)std::ceil(R / h) + 1;
+    // offsets beyond floor(R/h) elements are farther than R (centroid spacing
+    // is h up to rounding of the node coordinates)
+    ctx->win = (int)std::floor(R / h * (1.0 + 1e-9));
     ctx->have_filter = true;
     return PF_OK;
 }

@@ -280,28 +280,45 @@ __global__ void __launch_bounds__(256) k_pcg_k1_q4_tile(int nx, int ny, int tile
     __shared__ Vec2<T> sp[TI + 2][TJ + 2];
     __shared__ T se[TI + 1][TJ + 1];
     if (S->done) return;
-    double beta;
-    bool stop;
-    pcg_head(k, partB, nbB, S, eps, lds, beta, stop);
-    if (stop) return;
-    const T tb = (T)beta;
     const int j0 = (blockIdx.x % tilesJ) * TJ, i0 = (blockIdx.x / tilesJ) * TI;
-    for (int h = threadIdx.x; h < (TI + 2) * (TJ + 2); h += blockDim.x) {
+    // issue the halo and moduli loads first: they do not depend on beta, so
+    // their latency overlaps the reduction of the previous launch's partials
+    constexpr int HALO = (TI + 2) * (TJ + 2);
+    constexpr int PER = (HALO + 255) / 256;
+    Vec2<T> po[PER], zz[PER];
+#pragma unroll
+    for (int t = 0; t < PER; t++) {
+        int h = threadIdx.x + t * 256;
         int a = h / (TJ + 2), b = h - a * (TJ + 2);
         int ii = i0 - 1 + a, jj = j0 - 1 + b;
-        Vec2<T> v{0, 0};
-        if (ii >= 0 && ii <= nx && jj >= 0 && jj <= ny) {
+        po[t] = Vec2<T>{0, 0};
+        zz[t] = Vec2<T>{0, 0};
+        if (h < HALO && ii >= 0 && ii <= nx && jj >= 0 && jj <= ny) {
             long m = (long)(ny + 1) * ii + jj;
-            Vec2<T> po = load2(pold, m), zz = load2(z, m);
-            v.x = tb * po.x + zz.x;  // xeaxpy: p = beta p + z (CG.h:42-49)
-            v.y = tb * po.y + zz.y;
+            po[t] = load2(pold, m);
+            zz[t] = load2(z, m);
         }
-        sp[a][b] = v;
     }
     for (int h = threadIdx.x; h < (TI + 1) * (TJ + 1); h += blockDim.x) {
         int a = h / (TJ + 1), b = h - a * (TJ + 1);
         int ei = i0 - 1 + a, ej = j0 - 1 + b;
         se[a][b] = (ei >= 0 && ei < nx && ej >= 0 && ej < ny) ? E[(long)ny * ei + ej] : (T)0;
+    }
+    double beta;
+    bool stop;
+    pcg_head(k, partB, nbB, S, eps, lds, beta, stop);
+    if (stop) return;
+    const T tb = (T)beta;
+#pragma unroll
+    for (int t = 0; t < PER; t++) {
+        int h = threadIdx.x + t * 256;
+        if (h < HALO) {
+            int a = h / (TJ + 2), b = h - a * (TJ + 2);
+            Vec2<T> v;
+            v.x = tb * po[t].x + zz[t].x;  // xeaxpy: p = beta p + z (CG.h:42-49)
+            v.y = tb * po[t].y + zz[t].y;
+            sp[a][b] = v;
+        }
     }
     __syncthreads();
     double acc[1] = {0.0};
@@ -380,6 +397,14 @@ __global__ void __launch_bounds__(256) k_pcg_k2(long n, int k, T* __restrict__ x
                                                 double* __restrict__ partB, PcgDev* __restrict__ S) {
     __shared__ double lds[32];
     if (S->done) return;
+    // first pair of every thread loaded before the partial reduction
+    const long n2 = n / 2;
+    const long stride = (long)gridDim.x * blockDim.x;
+    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    Vec2<T> xi{0, 0}, ri{0, 0}, pi{0, 0}, qi{0, 0}, di{1, 1};
+    if (i < n2) {
+        xi = load2(x, i); ri = load2(r, i); pi = load2(p, i); qi = load2(q, i); di = load2(D, i);
+    }
     double v[1];
     reduce_partials<1>(partA, nbA, lds, v);
     const double zr = S->zr[k & 1];
@@ -391,10 +416,8 @@ __global__ void __launch_bounds__(256) k_pcg_k2(long n, int k, T* __restrict__ x
     const T ta = (T)alpha, tma = (T)(-alpha);
     double acc[2] = {0.0, 0.0};
     // pairs of entries per access (n is even for nodal vectors); an odd tail
-    // (free-numbered CSR vectors) is handled by the scalar loop below
-    const long n2 = n / 2;
-    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n2; i += (long)gridDim.x * blockDim.x) {
-        Vec2<T> xi = load2(x, i), ri = load2(r, i), pi = load2(p, i), qi = load2(q, i), di = load2(D, i);
+    // (free-numbered CSR vectors) is handled by the scalar step below
+    while (i < n2) {
         xi.x = xi.x + ta * pi.x;   // xexpay(x, alpha, p)
         xi.y = xi.y + ta * pi.y;
         ri.x = ri.x + tma * qi.x;  // xexpay(r, -alpha, Ap)
@@ -407,17 +430,21 @@ __global__ void __launch_bounds__(256) k_pcg_k2(long n, int k, T* __restrict__ x
         acc[0] += (double)zi.y * (double)ri.y;
         acc[1] += (double)ri.x * (double)ri.x;
         acc[1] += (double)ri.y * (double)ri.y;
+        i += stride;
+        if (i < n2) {
+            xi = load2(x, i); ri = load2(r, i); pi = load2(p, i); qi = load2(q, i); di = load2(D, i);
+        }
     }
     if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) {
-        long i = n - 1;
-        T xi = x[i] + ta * p[i];
-        T ri = r[i] + tma * q[i];
-        T zi = ri / D[i];
-        x[i] = xi;
-        r[i] = ri;
-        z[i] = zi;
-        acc[0] += (double)zi * (double)ri;
-        acc[1] += (double)ri * (double)ri;
+        long t = n - 1;
+        T xt = x[t] + ta * p[t];
+        T rt = r[t] + tma * q[t];
+        T zt = rt / D[t];
+        x[t] = xt;
+        r[t] = rt;
+        z[t] = zt;
+        acc[0] += (double)zt * (double)rt;
+        acc[1] += (double)rt * (double)rt;
     }
     store_partials<2>(acc, lds, partB);
 }
@@ -447,6 +474,26 @@ __global__ void __launch_bounds__(256) k_pcg_init(long n, const T* __restrict__ 
         S->status = 0;
         S->iterations = 0;
     }
+}
+
+// Residual replacement of the mixed-precision solve: r32 = (float) r64,
+// z = r32 ./ D32, the FP32 correction accumulator cleared, and the (z.r, r.r)
+// partials that the next K1 reads rewritten for the new residual.
+__global__ void __launch_bounds__(256) k_mixed_replace(long n, const double* __restrict__ r64, float* __restrict__ r32,
+                                                       const float* __restrict__ D, float* __restrict__ z,
+                                                       float* __restrict__ xacc, double* __restrict__ partB) {
+    __shared__ double lds[32];
+    double acc[2] = {0.0, 0.0};
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        float ri = (float)r64[i];
+        float zi = ri / D[i];
+        r32[i] = ri;
+        z[i] = zi;
+        xacc[i] = 0.0f;
+        acc[0] += (double)zi * (double)ri;
+        acc[1] += (double)ri * (double)ri;
+    }
+    store_partials<2>(acc, lds, partB);
 }
 
 // After the host's last K2: test convergence of the final iterate the way
