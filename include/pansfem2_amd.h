@@ -41,7 +41,7 @@ enum {
 };
 
 /* operators for pf_apply_K / pf_pcg */
-enum { PF_OP_CSR64 = 0, PF_OP_EBE64 = 1, PF_OP_EBE32 = 2 };
+enum { PF_OP_CSR64 = 0, PF_OP_EBE64 = 1, PF_OP_EBE32 = 2, PF_OP_LEARNED = 3 };
 /* filter modes for pf_filter */
 enum { PF_FILTER_VARIABLES = 0, PF_FILTER_SENSITIVITIES = 1 };
 
@@ -83,6 +83,9 @@ typedef struct pf_simp_params {
     int stop_on_convergence;       /* honour OC/MMA IsConvergence (OC.h:67-73) */
     pf_oc_params oc;
     pf_mma_params mma;
+    int collect;                   /* > 0: keep a training pair every `collect` CG chunks (32 iterations)
+                                      of each solve (needs pf_precond_init) */
+    int collect_max;               /* cap on pairs taken from one solve */
 } pf_simp_params;
 
 typedef struct pf_simp_history {
@@ -215,12 +218,36 @@ void pf_simp_default_params(pf_simp_params* prm);
 int pf_profile_kernels(pf_ctx* ctx, int op, int reps, double* ms);
 
 /* ---- learned preconditioner -------------------------------------------- */
-/* 4-level 3x3 convolutional network (BF16 weights and activations, FP32
- * accumulation) mapping (rho, r_x, r_y) to a displacement correction, used
- * inside flexible CG.  No counterpart in the reference. */
+/* 4-level 3x3 convolutional network on the nodal grid of a quad4 problem
+ * (BF16 weights and activations, FP32 accumulation on MFMA) mapping the
+ * density and the residual to a correction: M(r) = (r + s y(rho, r/s)) ./ D,
+ * s the RMS of r, D the Jacobi diagonal.  Used inside flexible CG
+ * (PF_OP_LEARNED) with a Jacobi-PCG fallback when it stalls.  No counterpart
+ * in the reference.  pf_precond_init builds it for the current grid with a
+ * seeded initialisation (the correction starts near zero, i.e. at Jacobi). */
 int pf_precond_init(pf_ctx* ctx, int channels, uint64_t seed);
+/* Number of network parameters (FP32 master copy). */
+int pf_precond_num_params(pf_ctx* ctx, int64_t* count);
 int pf_precond_save(pf_ctx* ctx, float* params, int64_t* count);
 int pf_precond_load(pf_ctx* ctx, const float* params, int64_t count);
+/* Training pairs from a plain Jacobi-PCG solve of the current system
+ * (current moduli and loads): every `every` chunks of 32 iterations the
+ * residual r_k and the iterate x_k are kept; after convergence e_k = x* - x_k.
+ * *count receives the dataset size. */
+int pf_precond_collect(pf_ctx* ctx, int every, int max_pairs, int* count);
+/* Synthetic pairs (K v, v) for `count` random smooth displacement fields v
+ * (seeded multi-scale lattice noise) at the current moduli. */
+int pf_precond_collect_random(pf_ctx* ctx, int count, uint64_t seed, int* total);
+/* Adam on the collected pairs, loss ||M(r) - e||_K^2 / ||e||_K^2. */
+int pf_precond_train(pf_ctx* ctx, int steps, double lr, uint64_t seed, double* loss_first, double* loss_last);
+/* d = M(r) for a free-DOF residual with the current moduli. */
+int pf_precond_apply(pf_ctx* ctx, const double* r_free, double* d_free);
+/* Raw network pass on an input tensor x0 (H x W x 16, H = nx+1, W = ny+1,
+ * FP32 values rounded to BF16) -> y (H x W x 16, FP32), and the parameter
+ * gradient of sum(y * dy) (grad in pf_precond_save order).  For checks
+ * against a CPU restatement of the network. */
+int pf_precond_forward_raw(pf_ctx* ctx, const float* x0, float* y);
+int pf_precond_backward_raw(pf_ctx* ctx, const float* dy, float* grad);
 
 #ifdef __cplusplus
 }

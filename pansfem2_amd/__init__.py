@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libpansfem2_amd.so")
 
 PF_OK, PF_NOT_CONVERGED = 0, 1
-PF_OP_CSR64, PF_OP_EBE64, PF_OP_EBE32 = 0, 1, 2
+PF_OP_CSR64, PF_OP_EBE64, PF_OP_EBE32, PF_OP_LEARNED = 0, 1, 2, 3
 PF_FILTER_VARIABLES, PF_FILTER_SENSITIVITIES = 0, 1
 
 _i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
@@ -53,7 +53,8 @@ class SimpParams(ctypes.Structure):
                                                "cg_eps")] + \
                [("cg_itrmax", ctypes.c_int), ("use_mma", ctypes.c_int), ("solver", ctypes.c_int),
                 ("mixed", ctypes.c_int), ("inner_eps", ctypes.c_double), ("learned", ctypes.c_int),
-                ("stop_on_convergence", ctypes.c_int), ("oc", OcParams), ("mma", MmaParams)]
+                ("stop_on_convergence", ctypes.c_int), ("oc", OcParams), ("mma", MmaParams),
+                ("collect", ctypes.c_int), ("collect_max", ctypes.c_int)]
 
 
 class SimpHistory(ctypes.Structure):
@@ -106,8 +107,15 @@ def lib():
             "pf_simp_default_params": ([P(SimpParams)], None),
             "pf_profile_kernels": ([c_void_p, c_int, c_int, _f64p], c_int),
             "pf_precond_init": ([c_void_p, c_int, ctypes.c_uint64], c_int),
+            "pf_precond_num_params": ([c_void_p, P(c_int64)], c_int),
             "pf_precond_save": ([c_void_p, c_void_p, P(c_int64)], c_int),
             "pf_precond_load": ([c_void_p, _f32p, c_int64], c_int),
+            "pf_precond_collect": ([c_void_p, c_int, c_int, P(c_int)], c_int),
+            "pf_precond_collect_random": ([c_void_p, c_int, ctypes.c_uint64, P(c_int)], c_int),
+            "pf_precond_train": ([c_void_p, c_int, c_double, ctypes.c_uint64, P(c_double), P(c_double)], c_int),
+            "pf_precond_apply": ([c_void_p, _f64p, _f64p], c_int),
+            "pf_precond_forward_raw": ([c_void_p, _f32p, _f32p], c_int),
+            "pf_precond_backward_raw": ([c_void_p, _f32p, _f32p], c_int),
         }
         for name, (args, res) in sig.items():
             fn = getattr(L, name)
@@ -305,6 +313,57 @@ class Context:
         ms = np.zeros(4)
         self._ck(lib().pf_profile_kernels(self._h, op, reps, ms))
         return dict(k1_ms=ms[0], k2_ms=ms[1], filter_ms=ms[2], oc_step_ms=ms[3])
+
+    # ---- learned preconditioner ----------------------------------------------
+    def precond_init(self, seed=20261204):
+        self._ck(lib().pf_precond_init(self._h, 16, seed))
+
+    def precond_num_params(self):
+        n = ctypes.c_int64(0)
+        self._ck(lib().pf_precond_num_params(self._h, ctypes.byref(n)))
+        return n.value
+
+    def precond_save(self):
+        out = np.zeros(self.precond_num_params(), np.float32)
+        n = ctypes.c_int64(0)
+        self._ck(lib().pf_precond_save(self._h, out.ctypes.data, ctypes.byref(n)))
+        return out
+
+    def precond_load(self, params):
+        params = np.ascontiguousarray(params, np.float32)
+        self._ck(lib().pf_precond_load(self._h, params, len(params)))
+
+    def precond_collect(self, every=8, max_pairs=32):
+        n = ctypes.c_int(0)
+        self._ck(lib().pf_precond_collect(self._h, every, max_pairs, ctypes.byref(n)))
+        return n.value
+
+    def precond_collect_random(self, count, seed=1):
+        n = ctypes.c_int(0)
+        self._ck(lib().pf_precond_collect_random(self._h, count, seed, ctypes.byref(n)))
+        return n.value
+
+    def precond_train(self, steps, lr=1e-3, seed=1):
+        a, b = ctypes.c_double(0), ctypes.c_double(0)
+        self._ck(lib().pf_precond_train(self._h, steps, lr, seed, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def precond_apply(self, r):
+        d = np.zeros(self.N)
+        self._ck(lib().pf_precond_apply(self._h, _f64(r), d))
+        return d
+
+    def precond_forward_raw(self, x0):
+        x0 = np.ascontiguousarray(x0, np.float32).reshape(-1)
+        y = np.zeros_like(x0)
+        self._ck(lib().pf_precond_forward_raw(self._h, x0, y))
+        return y
+
+    def precond_backward_raw(self, dy):
+        dy = np.ascontiguousarray(dy, np.float32).reshape(-1)
+        g = np.zeros(self.precond_num_params(), np.float32)
+        self._ck(lib().pf_precond_backward_raw(self._h, dy, g))
+        return g
 
     def simp_run(self, iters, params=None, s0=0.5):
         """The device-resident SIMP loop (sample_optimize_density_oc.cpp:83-208)."""

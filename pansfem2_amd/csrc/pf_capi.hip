@@ -4,6 +4,7 @@
 #include "pf_kernels.hip"
 #include "pf_mma.hip"
 #include "pf_hex8.hip"
+#include "pf_learned.hip"
 
 #include <algorithm>
 #include <chrono>
@@ -78,6 +79,7 @@ int check_launch(pf_ctx* ctx) {
 }
 
 bool is2d(pf_ctx* ctx) { return ctx->dim == 2; }
+void release_net(pf_ctx* ctx);
 G3 g3(pf_ctx* ctx) { return G3{ctx->nx, ctx->ny, ctx->nz}; }
 
 void free_grid(pf_ctx* ctx) {
@@ -94,6 +96,7 @@ void free_grid(pf_ctx* ctx) {
     ctx->N = 0;
     ctx->have_E = false;
     ctx->have_filter = false;
+    release_net(ctx);
 }
 
 int alloc_grid(pf_ctx* ctx) {
@@ -212,9 +215,17 @@ struct PcgRun {
 
 // Jacobi-PCG on device vectors: nodal (EbE, n = nv) or free-numbered (CSR,
 // n = N).  b, x, D are of the matching layout; x receives the solution.
+// Optional capture of (x_k) during a solve: every `every` chunks the iterate
+// is copied (stream-ordered) into slot `count` of `xs` (nv doubles each).
+struct SnapHook {
+    int every = 0, max = 0, count = 0;
+    double* xs = nullptr;
+    std::vector<int> iters;
+};
+
 template <class T>
 int pcg_device(pf_ctx* ctx, bool csr, const T* b, T* x, const T* D, T* r, T* z, T* p0, T* p1, T* q, double eps,
-               int itrmax, PcgRun& out) {
+               int itrmax, PcgRun& out, SnapHook* snap = nullptr) {
     const long n = csr ? ctx->N : ctx->nv;
     PcgDev* S = (PcgDev*)ctx->d_pcg;
     double* partA = ctx->red.partials;
@@ -263,6 +274,12 @@ int pcg_device(pf_ctx* ctx, bool csr, const T* b, T* x, const T* D, T* r, T* z, 
             k_pcg_k2<T><<<g2, kBlock, 0, ctx->stream>>>(n, k, x, r, z, pnew, q, D, partA, (int)g1, partB, S);
         }
         PF_TRY(check_launch(ctx));
+        if (snap && std::is_same<T, double>::value && snap->count < snap->max && (k / chunk) % snap->every == 0) {
+            PF_HIP(hipMemcpyAsync(snap->xs + (size_t)snap->count * n, x, sizeof(double) * n, hipMemcpyDeviceToDevice,
+                                  ctx->stream));
+            snap->iters.push_back(k);
+            snap->count++;
+        }
         PF_HIP(hipMemcpyAsync((void*)&hflag[slot], &S->done, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
         PF_HIP(hipEventRecord(ev[slot], ctx->stream));
         return PF_OK;
@@ -407,6 +424,610 @@ int pcg_mixed(pf_ctx* ctx, const double* b, double* x, double eps, int itrmax, d
     return PF_OK;
 }
 
+
+// ----------------------------------------------------- learned network ----
+}  // namespace
+
+struct NetLayer {
+    int cin = 0, cout = 0, level = 0, relu = 1;
+    float *W = nullptr, *m = nullptr, *v = nullptr, *g = nullptr;
+    bf16 *Wb = nullptr, *Wtb = nullptr;
+    long size() const { return 9L * cin * cout; }
+};
+
+struct LearnedNet {
+    int H[4] = {0}, Wd[4] = {0};
+    std::vector<NetLayer> L;  // c0a c0b c1a c1b c2a c2b c3a c3b d2 d1 d0 out
+    // activations (level, channels) and their gradients
+    bf16 *x0, *a0, *e0, *x1, *a1, *e1, *x2, *a2, *e2, *x3, *a3, *b3, *u2, *d2, *u1, *d1, *u0, *d0;
+    bf16 *gy, *gd0, *gu0, *gd1, *gu1, *gd2, *gu2, *gb3, *ga3, *gx3, *ge2, *ga2, *gx2, *ge1, *ga1, *gx1, *ge0, *ga0;
+    float* y = nullptr;
+    float* wpart = nullptr;
+    long wpart_n = 0;
+    // dataset (FP32): residual, error (nodal) and element densities per pair
+    int cap = 0, nsamp = 0;
+    float *ds_r = nullptr, *ds_e = nullptr, *ds_rho = nullptr;
+    double* snap_x = nullptr;
+    int snap_cap = 0;
+    // FP64 scratch (nodal)
+    double *w = nullptr, *Kw = nullptr, *gd = nullptr, *rr = nullptr, *ee = nullptr, *dd = nullptr;
+    int adam_t = 0;
+    std::vector<void*> allocs;
+    bool trained = false;
+    // flexible CG: stored directions and their operator images
+    int fcg_m = 0;
+    std::vector<double*> fP, fQ;
+    double* mpart = nullptr;  // multi-dot partials (grid x 32)
+};
+
+namespace {
+
+const int kNetLayers = 12;
+int moduli_from_rho(pf_ctx* ctx);
+
+long lvl_px(LearnedNet* N, int lv) { return (long)N->H[lv] * N->Wd[lv]; }
+
+void net_free(LearnedNet* N) {
+    if (!N) return;
+    for (void* p : N->allocs) hipFree(p);
+    delete N;
+}
+
+void release_net(pf_ctx* ctx) {
+    net_free(ctx->net);
+    ctx->net = nullptr;
+}
+
+template <class T>
+int net_alloc(pf_ctx* ctx, LearnedNet* N, T** p, long n) {
+    PF_HIP(hipMalloc((void**)p, sizeof(T) * (size_t)std::max(1L, n)));
+    N->allocs.push_back((void*)*p);
+    PF_HIP(hipMemsetAsync(*p, 0, sizeof(T) * (size_t)std::max(1L, n), ctx->stream));
+    return PF_OK;
+}
+
+uint64_t splitmix(uint64_t& s) {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+double urand(uint64_t& s) { return (double)(splitmix(s) >> 11) * (1.0 / 9007199254740992.0); }
+
+int net_refresh_bf16(pf_ctx* ctx, LearnedNet* N) {
+    for (NetLayer& l : N->L) {
+        k_weights_bf16<<<nblocks(l.size()), kBlock, 0, ctx->stream>>>(l.cout, l.cin, l.W, l.Wb, l.Wtb);
+    }
+    return check_launch(ctx);
+}
+
+int net_build(pf_ctx* ctx, uint64_t seed) {
+    PF_CHECK(is2d(ctx), PF_ERR_UNSUPPORTED, "learned preconditioner: quad4 grids only");
+    net_free(ctx->net);
+    LearnedNet* N = new LearnedNet();
+    ctx->net = N;
+    N->H[0] = ctx->nx + 1;
+    N->Wd[0] = ctx->ny + 1;
+    for (int lv = 1; lv < 4; lv++) {
+        N->H[lv] = (N->H[lv - 1] + 1) / 2;
+        N->Wd[lv] = (N->Wd[lv - 1] + 1) / 2;
+    }
+    const int spec[kNetLayers][4] = {  // cin, cout, level, relu
+        {16, 16, 0, 1}, {16, 16, 0, 1}, {16, 32, 1, 1}, {32, 32, 1, 1}, {32, 32, 2, 1}, {32, 32, 2, 1},
+        {32, 64, 3, 1}, {64, 32, 3, 1}, {32, 32, 2, 1}, {32, 16, 1, 1}, {16, 16, 0, 1}, {16, 16, 0, 0}};
+    uint64_t rs = seed ? seed : 20261204ull;
+    long maxw = 0;
+    for (int i = 0; i < kNetLayers; i++) {
+        NetLayer l;
+        l.cin = spec[i][0];
+        l.cout = spec[i][1];
+        l.level = spec[i][2];
+        l.relu = spec[i][3];
+        long n = l.size();
+        maxw = std::max(maxw, n);
+        PF_TRY(net_alloc(ctx, N, &l.W, n));
+        PF_TRY(net_alloc(ctx, N, &l.m, n));
+        PF_TRY(net_alloc(ctx, N, &l.v, n));
+        PF_TRY(net_alloc(ctx, N, &l.g, n));
+        PF_TRY(net_alloc(ctx, N, &l.Wb, n));
+        PF_TRY(net_alloc(ctx, N, &l.Wtb, n));
+        // He-uniform initialisation; the output layer starts near zero so that
+        // the preconditioner starts at Jacobi
+        std::vector<float> w(n);
+        double a = std::sqrt(6.0 / (9.0 * l.cin)) * (i == kNetLayers - 1 ? 0.01 : 1.0);
+        for (long q = 0; q < n; q++) w[q] = (float)((2.0 * urand(rs) - 1.0) * a);
+        PF_HIP(hipMemcpyAsync(l.W, w.data(), sizeof(float) * n, hipMemcpyHostToDevice, ctx->stream));
+        PF_TRY(sync(ctx));
+        N->L.push_back(l);
+    }
+    PF_TRY(net_refresh_bf16(ctx, N));
+    const long p0 = lvl_px(N, 0), p1 = lvl_px(N, 1), p2 = lvl_px(N, 2), p3 = lvl_px(N, 3);
+    struct A { bf16** p; long n; } acts[] = {
+        {&N->x0, p0 * 16}, {&N->a0, p0 * 16}, {&N->e0, p0 * 16}, {&N->x1, p1 * 16}, {&N->a1, p1 * 32},
+        {&N->e1, p1 * 32}, {&N->x2, p2 * 32}, {&N->a2, p2 * 32}, {&N->e2, p2 * 32}, {&N->x3, p3 * 32},
+        {&N->a3, p3 * 64}, {&N->b3, p3 * 32}, {&N->u2, p2 * 32}, {&N->d2, p2 * 32}, {&N->u1, p1 * 32},
+        {&N->d1, p1 * 16}, {&N->u0, p0 * 16}, {&N->d0, p0 * 16},
+        {&N->gy, p0 * 16}, {&N->gd0, p0 * 16}, {&N->gu0, p0 * 16}, {&N->gd1, p1 * 16}, {&N->gu1, p1 * 32},
+        {&N->gd2, p2 * 32}, {&N->gu2, p2 * 32}, {&N->gb3, p3 * 32}, {&N->ga3, p3 * 64}, {&N->gx3, p3 * 32},
+        {&N->ge2, p2 * 32}, {&N->ga2, p2 * 32}, {&N->gx2, p2 * 32}, {&N->ge1, p1 * 32}, {&N->ga1, p1 * 32},
+        {&N->gx1, p1 * 16}, {&N->ge0, p0 * 16}, {&N->ga0, p0 * 16}};
+    for (auto& a : acts) PF_TRY(net_alloc(ctx, N, a.p, a.n));
+    PF_TRY(net_alloc(ctx, N, &N->y, p0 * 16));
+    const int ppb = 4096;
+    N->wpart_n = ((p0 + ppb - 1) / ppb) * maxw;
+    PF_TRY(net_alloc(ctx, N, &N->wpart, N->wpart_n));
+    const long nv = ctx->nv;
+    PF_TRY(net_alloc(ctx, N, &N->mpart, 256L * 32));
+    double** sc[] = {&N->w, &N->Kw, &N->gd, &N->rr, &N->ee, &N->dd};
+    for (double** q : sc) PF_TRY(net_alloc(ctx, N, q, nv));
+    return sync(ctx);
+}
+
+template <int CI, int CO>
+void conv_fwd_t(pf_ctx* ctx, int H, int W, const bf16* X, const bf16* Wt, bf16* Y, float* Yf, int relu) {
+    dim3 grid(H, (W + 63) / 64);
+    k_conv_fwd<CI, CO><<<grid, 256, 0, ctx->stream>>>(H, W, X, Wt, Y, Yf, relu);
+}
+
+template <int CI, int CO>
+void conv_wgrad_t(pf_ctx* ctx, LearnedNet* N, int H, int W, const bf16* X, const bf16* dZ, float* dW) {
+    const int ppb = 4096;
+    const long npx = (long)H * W;
+    const int nb = (int)((npx + ppb - 1) / ppb);
+    k_conv_wgrad<CI, CO><<<nb, 256, 0, ctx->stream>>>(H, W, X, dZ, N->wpart, ppb);
+    const int n = 9 * CI * CO;
+    k_wgrad_reduce<<<nblocks(n), kBlock, 0, ctx->stream>>>(N->wpart, nb, n, dW);
+}
+
+// dispatch on (cin, cout) in {16, 32, 64}
+int conv_fwd(pf_ctx* ctx, int cin, int cout, int H, int W, const bf16* X, const bf16* Wt, bf16* Y, float* Yf,
+             int relu) {
+#define PF_CONV_CASE(a, b) \
+    if (cin == a && cout == b) { conv_fwd_t<a, b>(ctx, H, W, X, Wt, Y, Yf, relu); return check_launch(ctx); }
+    PF_CONV_CASE(16, 16) PF_CONV_CASE(16, 32) PF_CONV_CASE(32, 16) PF_CONV_CASE(32, 32) PF_CONV_CASE(32, 64)
+    PF_CONV_CASE(64, 32) PF_CONV_CASE(16, 64) PF_CONV_CASE(64, 16) PF_CONV_CASE(64, 64)
+#undef PF_CONV_CASE
+    ctx->err = "conv: unsupported channel counts";
+    return PF_ERR_UNSUPPORTED;
+}
+
+int conv_wgrad(pf_ctx* ctx, int cin, int cout, int H, int W, const bf16* X, const bf16* dZ, float* dW) {
+    LearnedNet* N = ctx->net;
+#define PF_WG_CASE(a, b) \
+    if (cin == a && cout == b) { conv_wgrad_t<a, b>(ctx, N, H, W, X, dZ, dW); return check_launch(ctx); }
+    PF_WG_CASE(16, 16) PF_WG_CASE(16, 32) PF_WG_CASE(32, 16) PF_WG_CASE(32, 32) PF_WG_CASE(32, 64) PF_WG_CASE(64, 32)
+#undef PF_WG_CASE
+    ctx->err = "wgrad: unsupported channel counts";
+    return PF_ERR_UNSUPPORTED;
+}
+
+int net_forward(pf_ctx* ctx) {
+    LearnedNet* N = ctx->net;
+    auto& L = N->L;
+    auto lay = [&](int li, const bf16* X, bf16* Y, float* Yf) {
+        NetLayer& l = L[li];
+        return conv_fwd(ctx, l.cin, l.cout, N->H[l.level], N->Wd[l.level], X, l.Wb, Y, Yf, l.relu);
+    };
+    auto pool = [&](int lv, int C, const bf16* in, bf16* out) {
+        long n = lvl_px(N, lv + 1) * C;
+        k_pool2<<<gsize(n), kBlock, 0, ctx->stream>>>(N->H[lv], N->Wd[lv], C, in, out);
+    };
+    auto upadd = [&](int lv, int C, const bf16* coarse, const bf16* skip, bf16* out) {
+        long n = lvl_px(N, lv) * C;
+        k_up2_add<<<gsize(n), kBlock, 0, ctx->stream>>>(N->H[lv], N->Wd[lv], C, coarse, skip, out);
+    };
+    PF_TRY(lay(0, N->x0, N->a0, nullptr));
+    PF_TRY(lay(1, N->a0, N->e0, nullptr));
+    pool(0, 16, N->e0, N->x1);
+    PF_TRY(lay(2, N->x1, N->a1, nullptr));
+    PF_TRY(lay(3, N->a1, N->e1, nullptr));
+    pool(1, 32, N->e1, N->x2);
+    PF_TRY(lay(4, N->x2, N->a2, nullptr));
+    PF_TRY(lay(5, N->a2, N->e2, nullptr));
+    pool(2, 32, N->e2, N->x3);
+    PF_TRY(lay(6, N->x3, N->a3, nullptr));
+    PF_TRY(lay(7, N->a3, N->b3, nullptr));
+    upadd(2, 32, N->b3, N->e2, N->u2);
+    PF_TRY(lay(8, N->u2, N->d2, nullptr));
+    upadd(1, 32, N->d2, N->e1, N->u1);
+    PF_TRY(lay(9, N->u1, N->d1, nullptr));
+    upadd(0, 16, N->d1, N->e0, N->u0);
+    PF_TRY(lay(10, N->u0, N->d0, nullptr));
+    PF_TRY(lay(11, N->d0, nullptr, N->y));
+    return check_launch(ctx);
+}
+
+// gradients of all layer weights for the output gradient N->gy
+int net_backward(pf_ctx* ctx) {
+    LearnedNet* N = ctx->net;
+    auto& L = N->L;
+    // layer li: output gradient gz (already masked by ReLU), input X -> dW, and
+    // the input gradient gx (if non-null)
+    auto back = [&](int li, const bf16* X, bf16* gz, bf16* gx) -> int {
+        NetLayer& l = L[li];
+        const int H = N->H[l.level], W = N->Wd[l.level];
+        PF_TRY(conv_wgrad(ctx, l.cin, l.cout, H, W, X, gz, l.g));
+        if (gx) PF_TRY(conv_fwd(ctx, l.cout, l.cin, H, W, gz, l.Wtb, gx, nullptr, 0));
+        return PF_OK;
+    };
+    auto relu_bwd = [&](int lv, int C, const bf16* Y, bf16* g) {
+        long n = lvl_px(N, lv) * C;
+        k_relu_bwd<<<gsize(n), kBlock, 0, ctx->stream>>>(n, Y, g);
+    };
+    auto up_bwd = [&](int lv, int C, const bf16* gfine, bf16* gcoarse) {
+        long n = lvl_px(N, lv + 1) * C;
+        k_up2_bwd<<<gsize(n), kBlock, 0, ctx->stream>>>(N->H[lv], N->Wd[lv], C, gfine, gcoarse);
+    };
+    auto pool_bwd_add = [&](int lv, int C, const bf16* gcoarse, bf16* gfine) {
+        long n = lvl_px(N, lv) * C;
+        k_pool2_bwd_add<<<gsize(n), kBlock, 0, ctx->stream>>>(N->H[lv], N->Wd[lv], C, gcoarse, gfine);
+    };
+    auto copy = [&](long n, const bf16* a, bf16* b) { k_copy_bf16<<<gsize(n), kBlock, 0, ctx->stream>>>(n, a, b); };
+    const long p0 = lvl_px(N, 0), p1 = lvl_px(N, 1), p2 = lvl_px(N, 2);
+    PF_TRY(back(11, N->d0, N->gy, N->gd0));
+    relu_bwd(0, 16, N->d0, N->gd0);
+    PF_TRY(back(10, N->u0, N->gd0, N->gu0));
+    up_bwd(0, 16, N->gu0, N->gd1);
+    copy(p0 * 16, N->gu0, N->ge0);  // skip branch of u0 = up(d1) + e0
+    relu_bwd(1, 16, N->d1, N->gd1);
+    PF_TRY(back(9, N->u1, N->gd1, N->gu1));
+    up_bwd(1, 32, N->gu1, N->gd2);
+    copy(p1 * 32, N->gu1, N->ge1);
+    relu_bwd(2, 32, N->d2, N->gd2);
+    PF_TRY(back(8, N->u2, N->gd2, N->gu2));
+    up_bwd(2, 32, N->gu2, N->gb3);
+    copy(p2 * 32, N->gu2, N->ge2);
+    relu_bwd(3, 32, N->b3, N->gb3);
+    PF_TRY(back(7, N->a3, N->gb3, N->ga3));
+    relu_bwd(3, 64, N->a3, N->ga3);
+    PF_TRY(back(6, N->x3, N->ga3, N->gx3));
+    pool_bwd_add(2, 32, N->gx3, N->ge2);
+    relu_bwd(2, 32, N->e2, N->ge2);
+    PF_TRY(back(5, N->a2, N->ge2, N->ga2));
+    relu_bwd(2, 32, N->a2, N->ga2);
+    PF_TRY(back(4, N->x2, N->ga2, N->gx2));
+    pool_bwd_add(1, 32, N->gx2, N->ge1);
+    relu_bwd(1, 32, N->e1, N->ge1);
+    PF_TRY(back(3, N->a1, N->ge1, N->ga1));
+    relu_bwd(1, 32, N->a1, N->ga1);
+    PF_TRY(back(2, N->x1, N->ga1, N->gx1));
+    pool_bwd_add(0, 16, N->gx1, N->ge0);
+    relu_bwd(0, 16, N->e0, N->ge0);
+    PF_TRY(back(1, N->a0, N->ge0, N->ga0));
+    relu_bwd(0, 16, N->a0, N->ga0);
+    PF_TRY(back(0, N->x0, N->ga0, nullptr));
+    return check_launch(ctx);
+}
+
+double dot_nodal(pf_ctx* ctx, const double* a, const double* b, int* rc);
+int optimal_pair(pf_ctx* ctx, const double* r, double* a, double* c);
+
+// M(r) for a nodal FP64 residual r with the current moduli (D, the FP64
+// Jacobi diagonal, in ctx->d_D):  t = r ./ D,  y = network field for
+// (rho, r / s),  d = a t + c y  with (a, c) minimising the energy norm of the
+// error of d over span{t, y}:  [t.Kt t.Ky; t.Ky y.Ky] (a, c) = (t.r, y.r)
+// (K e = r makes this computable without e).  d is never worse, in that
+// norm, than the best-scaled Jacobi step.  N->w holds y, N->Kw K y, N->dd t
+// and N->gd K t on return.
+int net_apply(pf_ctx* ctx, const double* rho, const double* r, double* d, double* rs_out, double rnorm = -1.0) {
+    LearnedNet* N = ctx->net;
+    const long nv = ctx->nv;
+    int rc = PF_OK;
+    const double rn = rnorm >= 0.0 ? rnorm : norm_nodal(ctx, r, &rc);
+    PF_TRY(rc);
+    const double rs = rn > 0 ? rn / std::sqrt((double)std::max(1, ctx->N)) : 1.0;
+    k_net_input<<<gsize(ctx->nn), kBlock, 0, ctx->stream>>>(ctx->nx, ctx->ny, rho, r, rs, N->x0);
+    PF_TRY(net_forward(ctx));
+    k_net_y_nodal<<<gsize(ctx->nn), kBlock, 0, ctx->stream>>>(ctx->nn, N->y, ctx->d_fix, N->w);
+    k_div_d<<<gsize(nv), kBlock, 0, ctx->stream>>>(nv, r, ctx->d_D, N->dd);   // t = r ./ D
+    PF_TRY(apply_nodal(ctx, false, N->dd, N->gd, true));                     // K t
+    PF_TRY(apply_nodal(ctx, false, N->w, N->Kw, true));                      // K y
+    double a = 1.0, c = 0.0;
+    PF_TRY(optimal_pair(ctx, r, &a, &c));
+    k_lincomb_d<<<gsize(nv), kBlock, 0, ctx->stream>>>(nv, a, N->dd, c, N->w, d);
+    if (rs_out) *rs_out = rs;
+    return check_launch(ctx);
+}
+
+double dot_nodal(pf_ctx* ctx, const double* a, const double* b, int* rc) {
+    const unsigned g = gsize(ctx->nv);
+    k_dot_partials<<<g, kBlock, 0, ctx->stream>>>(ctx->nv, a, b, ctx->red.partials);
+    return reduce_host(ctx, ctx->red.partials, (int)g, 1.0, 0.0, rc);
+}
+
+// (a, c) minimising ||a t + c y - e||_K with t = N->dd, y = N->w, K t = N->gd,
+// K y = N->Kw and K e = r; falls back to the best-scaled t alone when the
+// 2x2 Gram matrix is singular.
+// dots of `a` with up to 32 vectors in one pass, summed on the host in a
+// fixed order
+int multidot(pf_ctx* ctx, const double* a, const double* const* b, int nb, double* out) {
+    LearnedNet* N = ctx->net;
+    PtrPack P{};
+    for (int j = 0; j < nb; j++) P.p[j] = b[j];
+    const unsigned g = std::min<unsigned>(gsize(ctx->nv), 256u);
+    k_multidot<<<g, kBlock, 0, ctx->stream>>>(ctx->nv, nb, a, P, N->mpart);
+    PF_TRY(check_launch(ctx));
+    std::vector<double> h((size_t)g * nb);
+    PF_HIP(hipMemcpyAsync(h.data(), N->mpart, sizeof(double) * g * nb, hipMemcpyDeviceToHost, ctx->stream));
+    PF_TRY(sync(ctx));
+    for (int j = 0; j < nb; j++) {
+        double s = 0.0;
+        for (unsigned q = 0; q < g; q++) s += h[(size_t)q * nb + j];
+        out[j] = s;
+    }
+    return PF_OK;
+}
+
+int optimal_pair(pf_ctx* ctx, const double* r, double* a, double* c) {
+    LearnedNet* N = ctx->net;
+    double d1[3], d2[2];
+    const double* b1[3] = {N->gd, N->Kw, r};
+    PF_TRY(multidot(ctx, N->dd, b1, 3, d1));   // t.Kt, t.Ky, t.r
+    const double* b2[2] = {N->Kw, r};
+    PF_TRY(multidot(ctx, N->w, b2, 2, d2));    // y.Ky, y.r
+    const double tkt = d1[0], tky = d1[1], tr = d1[2], yky = d2[0], yr = d2[1];
+    const double det = tkt * yky - tky * tky;
+    if (std::isfinite(det) && det > 1e-12 * tkt * yky && yky > 0.0) {
+        *a = (yky * tr - tky * yr) / det;
+        *c = (tkt * yr - tky * tr) / det;
+    } else {
+        *a = tkt > 0.0 ? tr / tkt : 1.0;
+        *c = 0.0;
+    }
+    return PF_OK;
+}
+
+int net_dataset_alloc(pf_ctx* ctx) {
+    LearnedNet* N = ctx->net;
+    if (N->cap > 0) return PF_OK;
+    const long nv = ctx->nv, ne = ctx->ne;
+    // dataset capacity: bounded by memory (about 10 bytes per DOF per pair)
+    N->cap = (int)std::max(16L, std::min(1024L, (8L << 30) / (10L * nv)));
+    PF_TRY(net_alloc(ctx, N, &N->ds_r, (long)N->cap * nv));
+    PF_TRY(net_alloc(ctx, N, &N->ds_e, (long)N->cap * nv));
+    PF_TRY(net_alloc(ctx, N, &N->ds_rho, (long)N->cap * ne));
+    N->snap_cap = 64;
+    PF_TRY(net_alloc(ctx, N, &N->snap_x, (long)N->snap_cap * nv));
+    return PF_OK;
+}
+
+
+
+
+
+// Solve the current system with plain FP64 Jacobi-PCG, keeping snapshots of
+// x_k; then store (true residual b - K x_k, x* - x_k, rho) pairs.
+int net_collect(pf_ctx* ctx, const double* b, int every, int max_pairs, int* count, PcgRun* run_out = nullptr) {
+    LearnedNet* N = ctx->net;
+    PF_CHECK(N, PF_ERR_STATE, "learned preconditioner not initialised: call pf_precond_init first");
+    PF_TRY(net_dataset_alloc(ctx));
+    const long nv = ctx->nv;
+    const long ne = ctx->ne;
+    SnapHook hook;
+    hook.every = std::max(1, every);
+    hook.max = std::min(N->snap_cap, std::max(0, max_pairs));
+    hook.xs = N->snap_x;
+    PF_TRY(diag_nodal(ctx, false));
+    PcgRun run;
+    PF_TRY(pcg_device<double>(ctx, false, b, ctx->d_x, ctx->d_D, ctx->d_r, ctx->d_z, ctx->d_p0, ctx->d_p1, ctx->d_q,
+                              1e-10, 100000, run, &hook));
+    int added = 0;
+    for (int s = 0; s < hook.count; s++) {
+        if (hook.iters[s] >= run.iterations) continue;  // converged iterate: e = 0
+        if (N->nsamp >= N->cap) break;
+        // e = x* - x_k and r = K e: an exactly consistent pair (b - K x_k
+        // would carry the final residual of x*, which is not small next to a
+        // late r_k)
+        const double* xs = N->snap_x + (size_t)s * nv;
+        const int slot = N->nsamp++;
+        k_sub_scale<<<gsize(nv), kBlock, 0, ctx->stream>>>(nv, ctx->d_x, xs, 1.0, ctx->d_t1);
+        PF_TRY(apply_nodal(ctx, false, ctx->d_t1, ctx->d_t0, true));
+        k_pair_store<<<gsize(nv), kBlock, 0, ctx->stream>>>(nv, ctx->d_t0, ctx->d_t1, N->ds_r + (size_t)slot * nv,
+                                                           N->ds_e + (size_t)slot * nv);
+        k_d2f<<<gsize(ne), kBlock, 0, ctx->stream>>>(ne, ctx->d_rho, N->ds_rho + (size_t)slot * ne);
+        added++;
+    }
+    PF_TRY(check_launch(ctx));
+    PF_TRY(sync(ctx));
+    if (count) *count = N->nsamp;
+    if (run_out) *run_out = run;
+    return PF_OK;
+}
+
+// One Adam step on pair `s`; returns the loss of that pair,
+//   L = min_{a,c} ||a t + c y - e||_K^2 / ||e||_K^2   (t = r ./ D, y = network),
+// whose gradient at the optimal (a, c) is  dL/dy = 2 c (a K t + c K y - r) / e.r.
+int net_train_step(pf_ctx* ctx, int s, float lr, double* loss) {
+    LearnedNet* N = ctx->net;
+    const long nv = ctx->nv, ne = ctx->ne;
+    k_f2d<<<gsize(ne), kBlock, 0, ctx->stream>>>(ne, N->ds_rho + (size_t)s * ne, ctx->d_rho);
+    PF_TRY(moduli_from_rho(ctx));
+    PF_TRY(diag_nodal(ctx, false));
+    k_f2d<<<gsize(nv), kBlock, 0, ctx->stream>>>(nv, N->ds_r + (size_t)s * nv, N->rr);
+    k_f2d<<<gsize(nv), kBlock, 0, ctx->stream>>>(nv, N->ds_e + (size_t)s * nv, N->ee);
+    int rc;
+    const double rn = norm_nodal(ctx, N->rr, &rc);
+    PF_TRY(rc);
+    const double rs = rn > 0 ? rn / std::sqrt((double)std::max(1, ctx->N)) : 1.0;
+    k_net_input<<<gsize(ctx->nn), kBlock, 0, ctx->stream>>>(ctx->nx, ctx->ny, ctx->d_rho, N->rr, rs, N->x0);
+    PF_TRY(net_forward(ctx));
+    k_net_y_nodal<<<gsize(ctx->nn), kBlock, 0, ctx->stream>>>(ctx->nn, N->y, ctx->d_fix, N->w);   // y
+    PF_TRY(apply_nodal(ctx, false, N->w, N->Kw, true));                                           // K y
+    k_div_d<<<gsize(nv), kBlock, 0, ctx->stream>>>(nv, N->rr, ctx->d_D, N->dd);                   // t
+    PF_TRY(apply_nodal(ctx, false, N->dd, N->gd, true));                                          // K t
+    const double er = dot_nodal(ctx, N->ee, N->rr, &rc);
+    PF_TRY(rc);
+    double a, c;
+    PF_TRY(optimal_pair(ctx, N->rr, &a, &c));
+    if (!(er > 0.0) || !std::isfinite(a + c)) {
+        *loss = NAN;
+        return PF_OK;
+    }
+    // v = a K t + c K y - r (= K (a t + c y - e));  L = (a t + c y - e).v / e.r
+    k_lincomb_d<<<gsize(nv), kBlock, 0, ctx->stream>>>(nv, a, N->gd, c, N->Kw, ctx->d_t1);
+    k_axpy_d<<<gsize(nv), kBlock, 0, ctx->stream>>>(nv, -1.0, N->rr, ctx->d_t1);
+    k_lincomb_d<<<gsize(nv), kBlock, 0, ctx->stream>>>(nv, a, N->dd, c, N->w, ctx->d_t0);
+    k_axpy_d<<<gsize(nv), kBlock, 0, ctx->stream>>>(nv, -1.0, N->ee, ctx->d_t0);
+    *loss = dot_nodal(ctx, ctx->d_t0, ctx->d_t1, &rc) / er;
+    PF_TRY(rc);
+    k_scale_d<<<gsize(nv), kBlock, 0, ctx->stream>>>(nv, 2.0 * c / er, ctx->d_t1);
+    k_net_dy_from_nodal<<<gsize(ctx->nn), kBlock, 0, ctx->stream>>>(ctx->nn, ctx->d_t1, ctx->d_fix, N->gy);
+    PF_TRY(net_backward(ctx));
+    N->adam_t++;
+    const float b1 = 0.9f, b2 = 0.999f;
+    const float c1 = 1.f - std::pow(b1, (float)N->adam_t), c2 = 1.f - std::pow(b2, (float)N->adam_t);
+    for (NetLayer& l : N->L)
+        k_adam<<<nblocks(l.size()), kBlock, 0, ctx->stream>>>((int)l.size(), l.W, l.m, l.v, l.g, lr, b1, b2, 1e-8f,
+                                                             c1, c2, 1.0f);
+    PF_TRY(net_refresh_bf16(ctx, N));
+    return check_launch(ctx);
+}
+
+// Synthetic pairs (K v, v) for random smooth fields v at the current moduli.
+int net_collect_random(pf_ctx* ctx, int count, uint64_t seed, int* total) {
+    LearnedNet* N = ctx->net;
+    const long nv = ctx->nv, ne = ctx->ne;
+    int levels = 1;
+    while ((1 << levels) < std::max(ctx->nx, ctx->ny) / 2) levels++;
+    for (int q = 0; q < count && N->nsamp < N->cap; q++) {
+        const int slot = N->nsamp++;
+        k_random_field<<<gsize(ctx->nn), kBlock, 0, ctx->stream>>>(ctx->nx, ctx->ny, seed + 7919ull * q, levels,
+                                                                  ctx->d_fix, ctx->d_t0);
+        PF_TRY(apply_nodal(ctx, false, ctx->d_t0, ctx->d_t1, true));
+        k_pair_store<<<gsize(nv), kBlock, 0, ctx->stream>>>(nv, ctx->d_t1, ctx->d_t0, N->ds_r + (size_t)slot * nv,
+                                                           N->ds_e + (size_t)slot * nv);
+        k_d2f<<<gsize(ne), kBlock, 0, ctx->stream>>>(ne, ctx->d_rho, N->ds_rho + (size_t)slot * ne);
+    }
+    PF_TRY(check_launch(ctx));
+    PF_TRY(sync(ctx));
+    if (total) *total = N->nsamp;
+    return PF_OK;
+}
+
+// Stored directions of the flexible CG.  With m = 8 the 96 x 48 cantilever
+// needs about 1.4x the iterations of m = 32; CoefPack/PtrPack hold 32.
+constexpr int kFcgDirections = 32;
+
+// Flexible CG (Notay's FCG(m)) with M = the network correction: each new
+// direction is made K-conjugate to the last m stored ones, which keeps the
+// Krylov-like acceleration that a varying preconditioner would otherwise
+// lose.  FP64 vectors; stops on the FP64 true residual; when the network
+// breaks down, diverges (residual 10x above its best) or stagnates (no
+// halving in `stall` steps) Jacobi-PCG takes over from the current iterate.
+int pcg_learned(pf_ctx* ctx, const double* b, double* x, double eps, int itrmax, pf_solve_stats& st) {
+    LearnedNet* N = ctx->net;
+    PF_CHECK(N && N->trained, PF_ERR_STATE, "learned preconditioner has not been trained");
+    const long nv = ctx->nv;
+    const int m = kFcgDirections;
+    if (N->fcg_m != m) {
+        N->fP.assign(m, nullptr);
+        N->fQ.assign(m, nullptr);
+        for (int j = 0; j < m; j++) {
+            PF_TRY(net_alloc(ctx, N, &N->fP[j], nv));
+            PF_TRY(net_alloc(ctx, N, &N->fQ[j], nv));
+        }
+        N->fcg_m = m;
+    }
+    std::vector<double> pq(m, 0.0);
+    int rc;
+    PF_TRY(diag_nodal(ctx, false));
+    const double bn = norm_nodal(ctx, b, &rc);
+    PF_TRY(rc);
+    double *r = ctx->d_r, *z = ctx->d_z;
+    PF_HIP(hipMemsetAsync(x, 0, sizeof(double) * nv, ctx->stream));
+    PF_HIP(hipMemcpyAsync(r, b, sizeof(double) * nv, hipMemcpyDeviceToDevice, ctx->stream));
+    double rn = bn, best = bn;
+    int since_best = 0, k = 0, status = 2, fallbacks = 0;
+    const int stall = 2000;
+    bool fallback = false;
+    for (; k < itrmax; k++) {
+        PF_TRY(net_apply(ctx, ctx->d_rho, r, z, nullptr, rn));
+        const int nst = std::min(k, m);
+        const int slot = k % m;
+        double* p = N->fP[slot];
+        double* q = N->fQ[slot];
+        // p = z - sum_j (z.q_j / p_j.q_j) p_j over the stored directions
+        CoefPack cf{};
+        PtrPack Pp{};
+        if (nst > 0) {
+            std::vector<const double*> qs(nst);
+            std::vector<double> zq(nst);
+            for (int j = 0; j < nst; j++) qs[j] = N->fQ[(k - 1 - j + m) % m];
+            PF_TRY(multidot(ctx, z, qs.data(), nst, zq.data()));
+            for (int j = 0; j < nst; j++) {
+                const int sj = (k - 1 - j + m) % m;
+                cf.c[j] = zq[j] / pq[sj];
+                Pp.p[j] = N->fP[sj];
+            }
+        }
+        k_fcg_direction<<<gsize(nv), kBlock, 0, ctx->stream>>>(nv, nst, z, Pp, cf, p);
+        PF_TRY(apply_nodal(ctx, false, p, q, true));
+        double d2[2];
+        const double* bq[2] = {q, r};
+        PF_TRY(multidot(ctx, p, bq, 2, d2));
+        const double pqv = d2[0], pr = d2[1];
+        if (!(pqv > 0.0) || !std::isfinite(pr)) {  // breakdown: M(r) is not a descent direction
+            fallback = true;
+            break;
+        }
+        pq[slot] = pqv;
+        const double alpha = pr / pqv;
+        k_axpy_d<<<gsize(nv), kBlock, 0, ctx->stream>>>(nv, alpha, p, x);
+        k_axpy_d<<<gsize(nv), kBlock, 0, ctx->stream>>>(nv, -alpha, q, r);
+        rn = norm_nodal(ctx, r, &rc);
+        PF_TRY(rc);
+        if (!std::isfinite(rn)) {
+            fallback = true;
+            break;
+        }
+        if (rn <= eps * bn) {
+            // confirm on the true residual
+            double tn;
+            PF_TRY(true_residual(ctx, b, x, ctx->d_t0, &tn));
+            if (tn <= eps * bn) {
+                status = 1;
+                rn = tn;
+                k++;
+                break;
+            }
+            PF_HIP(hipMemcpyAsync(r, ctx->d_t0, sizeof(double) * nv, hipMemcpyDeviceToDevice, ctx->stream));
+            rn = tn;
+        }
+        if (rn < 0.5 * best) {
+            best = rn;
+            since_best = 0;
+        } else if (++since_best >= stall || rn > 10.0 * best) {
+            fallback = true;
+            break;
+        }
+    }
+    if (fallback) {
+        // Jacobi-PCG on the correction K dx = b - K x from the current iterate
+        // (from x = 0 when the iterate is not finite)
+        fallbacks++;
+        double tn;
+        PF_TRY(true_residual(ctx, b, x, ctx->d_t0, &tn));
+        if (!std::isfinite(tn)) {
+            PF_HIP(hipMemsetAsync(x, 0, sizeof(double) * nv, ctx->stream));
+            PF_HIP(hipMemcpyAsync(ctx->d_t0, b, sizeof(double) * nv, hipMemcpyDeviceToDevice, ctx->stream));
+            tn = bn;
+        }
+        PcgRun run;
+        PF_HIP(hipMemcpyAsync(ctx->d_t1, ctx->d_t0, sizeof(double) * nv, hipMemcpyDeviceToDevice, ctx->stream));
+        PF_TRY(pcg_device<double>(ctx, false, ctx->d_t1, N->dd, ctx->d_D, r, z, ctx->d_p0, ctx->d_p1, ctx->d_q,
+                                  eps * bn / std::max(tn, 1e-300), std::max(1, itrmax - k), run));
+        k_axpy_d<<<gsize(nv), kBlock, 0, ctx->stream>>>(nv, 1.0, N->dd, x);
+        k += run.iterations;
+        PF_TRY(true_residual(ctx, b, x, ctx->d_t0, &rn));
+        status = rn <= eps * bn ? 1 : 2;
+    }
+    st.iterations = k;
+    st.status = status;
+    st.fallbacks = fallbacks;
+    st.rel_residual = bn > 0 ? rn / bn : 0.0;
+    return check_launch(ctx);
+}
+
 // Solve K x = b for nodal device vectors b, x with the selected operator.
 int solve_nodal(pf_ctx* ctx, int op, const double* b, double* x, double eps, int itrmax, double inner_eps,
                 pf_solve_stats* st) {
@@ -451,6 +1072,8 @@ int solve_nodal(pf_ctx* ctx, int op, const double* b, double* x, double eps, int
         s.rel_residual = run.bnorm > 0 ? run.rnorm / run.bnorm : 0.0;
     } else if (op == PF_OP_EBE32) {
         PF_TRY(pcg_mixed(ctx, b, x, eps, itrmax, 0.1, s));
+    } else if (op == PF_OP_LEARNED) {
+        PF_TRY(pcg_learned(ctx, b, x, eps, itrmax, s));
     } else {
         ctx->err = "unknown operator";
         return PF_ERR_ARG;
@@ -800,6 +1423,7 @@ int pf_destroy(pf_ctx* ctx) {
     hipSetDevice(ctx->device);
     if (ctx->stream) hipStreamSynchronize(ctx->stream);
     free_grid(ctx);
+    release_net(ctx);
     mma_free(ctx->mma);
     ctx->mma = nullptr;
     dfree(ctx->d_K0);
@@ -1188,7 +1812,9 @@ int pf_simp_run(pf_ctx* ctx, const pf_simp_params* prm, int iters, double* s, do
     if (!ctx || !prm || !s) return PF_ERR_ARG;
     PF_TRY(require_ready(ctx, false));
     PF_CHECK(iters >= 0, PF_ERR_ARG, "iters must be non-negative");
-    PF_CHECK(!prm->learned, PF_ERR_UNSUPPORTED, "learned preconditioner is not available in this build");
+    PF_CHECK(!prm->learned || (ctx->net && ctx->net->trained), PF_ERR_STATE,
+             "learned solves need a trained preconditioner (pf_precond_init, collect, train)");
+    PF_CHECK(prm->collect <= 0 || ctx->net, PF_ERR_STATE, "collecting pairs needs pf_precond_init first");
     PF_TRY(set_device(ctx));
     ctx->E1 = prm->E1; ctx->E0 = prm->E0; ctx->p = prm->p; ctx->nu = prm->nu; ctx->t = prm->t;
     if (!ctx->have_filter || ctx->R != prm->R) PF_TRY(pf_filter_setup(ctx, prm->R));
@@ -1218,11 +1844,19 @@ int pf_simp_run(pf_ctx* ctx, const pf_simp_params* prm, int iters, double* s, do
         PF_TRY(moduli_from_rho(ctx));
         // solve K u = F
         pf_solve_stats st{};
-        int op = prm->mixed ? PF_OP_EBE32 : prm->solver;
+        int op = prm->learned ? PF_OP_LEARNED : (prm->mixed ? PF_OP_EBE32 : prm->solver);
         if (op == PF_OP_CSR64) PF_TRY(pf_assemble_csr(ctx, nullptr));
-        int src = solve_nodal(ctx, op, ctx->d_load, ctx->d_x, prm->mixed ? std::max(prm->cg_eps, 1e-14) : prm->cg_eps,
-                              prm->cg_itrmax, prm->inner_eps, &st);
-        if (src < 0) return src;
+        if (prm->collect > 0 && op == PF_OP_EBE64) {
+            PcgRun run;
+            int cnt;
+            PF_TRY(net_collect(ctx, ctx->d_load, prm->collect, prm->collect_max > 0 ? prm->collect_max : 16, &cnt,
+                               &run));
+            st.iterations = run.iterations;
+            st.status = run.status;
+        } else {
+            int src = solve_nodal(ctx, op, ctx->d_load, ctx->d_x, prm->cg_eps, prm->cg_itrmax, prm->inner_eps, &st);
+            if (src < 0) return src;
+        }
         // compliance, sensitivities, filtered sensitivities
         double f;
         PF_TRY(comp_sens_device(ctx, ctx->d_x, prm->scale0, d_dfdrho, &f));
@@ -1355,18 +1989,137 @@ int pf_profile_kernels(pf_ctx* ctx, int op, int reps, double* ms) {
 
 int pf_precond_init(pf_ctx* ctx, int channels, uint64_t seed) {
     if (!ctx) return PF_ERR_ARG;
-    ctx->err = "learned preconditioner is not available in this build";
-    return PF_ERR_UNSUPPORTED;
+    PF_CHECK(channels == 0 || channels == 16, PF_ERR_ARG, "pf_precond_init: base width 16 (pass 0 or 16)");
+    PF_TRY(require_ready(ctx, false));
+    PF_TRY(set_device(ctx));
+    return net_build(ctx, seed);
 }
+
+int pf_precond_num_params(pf_ctx* ctx, int64_t* count) {
+    if (!ctx || !count) return PF_ERR_ARG;
+    PF_CHECK(ctx->net, PF_ERR_STATE, "learned preconditioner not initialised");
+    int64_t n = 0;
+    for (NetLayer& l : ctx->net->L) n += l.size();
+    *count = n;
+    return PF_OK;
+}
+
 int pf_precond_save(pf_ctx* ctx, float* params, int64_t* count) {
     if (!ctx) return PF_ERR_ARG;
-    ctx->err = "learned preconditioner is not available in this build";
-    return PF_ERR_UNSUPPORTED;
+    PF_CHECK(ctx->net, PF_ERR_STATE, "learned preconditioner not initialised");
+    PF_TRY(set_device(ctx));
+    int64_t off = 0;
+    for (NetLayer& l : ctx->net->L) {
+        if (params) PF_HIP(hipMemcpyAsync(params + off, l.W, sizeof(float) * l.size(), hipMemcpyDeviceToHost, ctx->stream));
+        off += l.size();
+    }
+    if (count) *count = off;
+    return sync(ctx);
 }
+
 int pf_precond_load(pf_ctx* ctx, const float* params, int64_t count) {
+    if (!ctx || !params) return PF_ERR_ARG;
+    PF_CHECK(ctx->net, PF_ERR_STATE, "learned preconditioner not initialised");
+    int64_t n;
+    PF_TRY(pf_precond_num_params(ctx, &n));
+    PF_CHECK(n == count, PF_ERR_ARG, "pf_precond_load: parameter count mismatch");
+    for (int64_t q = 0; q < count; q++) PF_CHECK(std::isfinite(params[q]), PF_ERR_NUMERIC, "non-finite parameter");
+    PF_TRY(set_device(ctx));
+    int64_t off = 0;
+    for (NetLayer& l : ctx->net->L) {
+        PF_HIP(hipMemcpyAsync(l.W, params + off, sizeof(float) * l.size(), hipMemcpyHostToDevice, ctx->stream));
+        off += l.size();
+    }
+    PF_TRY(net_refresh_bf16(ctx, ctx->net));
+    ctx->net->trained = true;
+    return sync(ctx);
+}
+
+int pf_precond_collect(pf_ctx* ctx, int every, int max_pairs, int* count) {
     if (!ctx) return PF_ERR_ARG;
-    ctx->err = "learned preconditioner is not available in this build";
-    return PF_ERR_UNSUPPORTED;
+    PF_TRY(require_ready(ctx, true));
+    PF_CHECK(ctx->net, PF_ERR_STATE, "learned preconditioner not initialised");
+    PF_TRY(set_device(ctx));
+    return net_collect(ctx, ctx->d_load, every, max_pairs, count);
+}
+
+int pf_precond_collect_random(pf_ctx* ctx, int count, uint64_t seed, int* total) {
+    if (!ctx || count < 0) return PF_ERR_ARG;
+    PF_TRY(require_ready(ctx, true));
+    PF_CHECK(ctx->net, PF_ERR_STATE, "learned preconditioner not initialised");
+    PF_TRY(set_device(ctx));
+    PF_TRY(net_dataset_alloc(ctx));
+    return net_collect_random(ctx, count, seed, total);
+}
+
+int pf_precond_train(pf_ctx* ctx, int steps, double lr, uint64_t seed, double* loss_first, double* loss_last) {
+    if (!ctx) return PF_ERR_ARG;
+    PF_CHECK(ctx->net, PF_ERR_STATE, "learned preconditioner not initialised");
+    PF_CHECK(ctx->net->nsamp > 0, PF_ERR_STATE, "no training pairs: call pf_precond_collect first");
+    PF_CHECK(steps >= 0 && lr > 0, PF_ERR_ARG, "pf_precond_train: steps >= 0, lr > 0");
+    PF_TRY(set_device(ctx));
+    uint64_t rs = seed ? seed : 7ull;
+    double first = 0, last = 0;
+    int nf = 0, nl = 0;
+    const int win = std::max(1, std::min(20, steps / 4));
+    for (int it = 0; it < steps; it++) {
+        int smp = (int)(splitmix(rs) % (uint64_t)ctx->net->nsamp);
+        double loss;
+        PF_TRY(net_train_step(ctx, smp, (float)lr, &loss));
+        if (!std::isfinite(loss)) continue;
+        if (it < win) { first += loss; nf++; }
+        if (it >= steps - win) { last += loss; nl++; }
+    }
+    ctx->net->trained = true;
+    if (loss_first) *loss_first = nf ? first / nf : NAN;
+    if (loss_last) *loss_last = nl ? last / nl : NAN;
+    return sync(ctx);
+}
+
+int pf_precond_apply(pf_ctx* ctx, const double* r_free, double* d_free) {
+    if (!ctx || !r_free || !d_free) return PF_ERR_ARG;
+    PF_TRY(require_ready(ctx, true));
+    PF_CHECK(ctx->net, PF_ERR_STATE, "learned preconditioner not initialised");
+    PF_TRY(set_device(ctx));
+    PF_HIP(hipMemcpyAsync(ctx->d_t1, r_free, sizeof(double) * ctx->N, hipMemcpyHostToDevice, ctx->stream));
+    PF_TRY(to_nodal(ctx, ctx->d_t1, ctx->d_r));
+    PF_TRY(diag_nodal(ctx, false));
+    PF_TRY(net_apply(ctx, ctx->d_rho, ctx->d_r, ctx->d_z, nullptr));
+    PF_TRY(to_free(ctx, ctx->d_z, ctx->d_t0));
+    PF_HIP(hipMemcpyAsync(d_free, ctx->d_t0, sizeof(double) * ctx->N, hipMemcpyDeviceToHost, ctx->stream));
+    return sync(ctx);
+}
+
+int pf_precond_forward_raw(pf_ctx* ctx, const float* x0, float* y) {
+    if (!ctx || !x0 || !y) return PF_ERR_ARG;
+    PF_CHECK(ctx->net, PF_ERR_STATE, "learned preconditioner not initialised");
+    PF_TRY(set_device(ctx));
+    LearnedNet* N = ctx->net;
+    const long n = lvl_px(N, 0) * 16;
+    std::vector<bf16> h(n);
+    for (long q = 0; q < n; q++) h[q] = f2bf(x0[q]);
+    PF_HIP(hipMemcpyAsync(N->x0, h.data(), sizeof(bf16) * n, hipMemcpyHostToDevice, ctx->stream));
+    PF_TRY(net_forward(ctx));
+    PF_HIP(hipMemcpyAsync(y, N->y, sizeof(float) * n, hipMemcpyDeviceToHost, ctx->stream));
+    return sync(ctx);
+}
+
+int pf_precond_backward_raw(pf_ctx* ctx, const float* dy, float* grad) {
+    if (!ctx || !dy || !grad) return PF_ERR_ARG;
+    PF_CHECK(ctx->net, PF_ERR_STATE, "learned preconditioner not initialised");
+    PF_TRY(set_device(ctx));
+    LearnedNet* N = ctx->net;
+    const long n = lvl_px(N, 0) * 16;
+    std::vector<bf16> h(n);
+    for (long q = 0; q < n; q++) h[q] = f2bf(dy[q]);
+    PF_HIP(hipMemcpyAsync(N->gy, h.data(), sizeof(bf16) * n, hipMemcpyHostToDevice, ctx->stream));
+    PF_TRY(net_backward(ctx));
+    int64_t off = 0;
+    for (NetLayer& l : N->L) {
+        PF_HIP(hipMemcpyAsync(grad + off, l.g, sizeof(float) * l.size(), hipMemcpyDeviceToHost, ctx->stream));
+        off += l.size();
+    }
+    return sync(ctx);
 }
 
 }  // extern "C"

@@ -907,6 +907,40 @@ __global__ void __launch_bounds__(256) k_residual_q4(int nx, int ny, const doubl
     store_partials<1>(acc, lds, part);
 }
 
+__global__ void k_axpy_d(long n, double a, const double* __restrict__ x, double* __restrict__ y) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        y[i] = y[i] + a * x[i];
+}
+
+// y = a x + y written as y += a x (used for accumulating gradients)
+__global__ void k_axpby_d(long n, double a, const double* __restrict__ x, double* __restrict__ y) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        y[i] = y[i] + a * x[i];
+}
+
+__global__ void k_div_d(long n, const double* __restrict__ a, const double* __restrict__ b, double* __restrict__ y) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        y[i] = a[i] / b[i];
+}
+
+__global__ void k_scale_d(long n, double a, double* __restrict__ y) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        y[i] = a * y[i];
+}
+
+// d = a x + c y
+__global__ void k_lincomb_d(long n, double a, const double* __restrict__ x, double c, const double* __restrict__ y,
+                            double* __restrict__ d) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        d[i] = a * x[i] + c * y[i];
+}
+
+// p = z + beta p
+__global__ void k_axpby_d2(long n, const double* __restrict__ z, double beta, double* __restrict__ p) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        p[i] = z[i] + beta * p[i];
+}
+
 __global__ void k_axpy_f32(long n, const float* __restrict__ d, double* __restrict__ x) {
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
         x[i] = x[i] + (double)d[i];
