@@ -63,6 +63,23 @@ def cpu_baseline(nx, ny, n_cg, n_bis, cg_sample=100, oc_sample=30):
     }
 
 
+def kernels_only(ctx, pa, torch, nx, ny, reps=20, copy_mib=256):
+    """The PCG operator and vector kernels, the filter and the OC step on a
+    density field drawn i.i.d. uniform in [1e-3, 1] (seed 20261202), then one
+    device-to-device copy of `copy_mib` MiB: the copy's counter readings
+    calibrate the byte counters (it reads and writes exactly copy_mib MiB)."""
+    rng = np.random.default_rng(20261202)
+    ctx.set_density(rng.uniform(1e-3, 1.0, ctx.ne))
+    prof = ctx.profile_kernels(pa.PF_OP_EBE64, reps=reps)
+    src = torch.ones(copy_mib * 2**20 // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"kernels_only": True, "nx": nx, "ny": ny, "reps": reps, "copy_bytes": copy_mib * 2**20,
+                      "N": ctx.N, "ne": ctx.ne, "kernels_ms": prof}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -72,6 +89,9 @@ def main():
     ap.add_argument("--ny", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="only the PCG/filter/OC kernels (pf_profile_kernels) on a seeded density field, then a "
+                         "plain device copy of known size: the program the counter runs profile")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU decomposition is "
@@ -84,6 +104,9 @@ def main():
     nx, ny = args.nx, args.ny
     ctx = pa.cantilever2d(nx, ny)
     N, ne = ctx.N, ctx.ne
+    if args.kernels_only:
+        kernels_only(ctx, pa, torch, nx, ny)
+        return
     # warmup OC iterations (untimed), then K timed iterations continuing the same trajectory
     w = ctx.simp_run(args.warmup) if args.warmup > 0 else {"s": np.full(ne, 0.5)}
     torch.cuda.synchronize()

@@ -26,7 +26,38 @@ int grid_cap() {
     return cap;
 }
 
-constexpr int kTileJ = 64, kTileI = 8;  // nodes per LDS tile of the quad4 PCG operator kernel
+// nodes per LDS tile of the quad4 PCG operator kernel (TI rows x TJ along j)
+#ifndef PF_K1_TI
+#define PF_K1_TI 8
+#endif
+#ifndef PF_K1_TJ
+#define PF_K1_TJ 64
+#endif
+constexpr int kTileJ = PF_K1_TJ, kTileI = PF_K1_TI;
+
+// Launch shape of the quad4 operator kernel: tiles, and workgroups (a
+// multiple of 8, one group per XCD) so that each takes about kTilesPerBlock
+// tiles of its XCD's range.
+struct K1Shape {
+    int tilesJ, ntiles;
+    unsigned grid;
+};
+int k1_tiles_per_block() {
+    static int v = [] {
+        const char* s = std::getenv("PF_K1_TPB");
+        return s ? std::max(1, std::atoi(s)) : 1;
+    }();
+    return v;
+}
+K1Shape k1_shape(int nx, int ny) {
+    K1Shape sh;
+    sh.tilesJ = (ny + 1 + kTileJ - 1) / kTileJ;
+    sh.ntiles = sh.tilesJ * ((nx + 1 + kTileI - 1) / kTileI);
+    const int per = (sh.ntiles + 7) / 8;
+    const int tpb = k1_tiles_per_block();
+    sh.grid = 8u * (unsigned)((per + tpb - 1) / tpb);
+    return sh;
+}
 
 // workgroups of the streaming PCG vector kernel (few, so that the operator
 // kernel's redundant read of its partials stays small)
@@ -230,10 +261,9 @@ int pcg_device(pf_ctx* ctx, bool csr, const T* b, T* x, const T* D, T* r, T* z, 
     PcgDev* S = (PcgDev*)ctx->d_pcg;
     double* partA = ctx->red.partials;
     double* partB = ctx->red.partials + 2 * kMaxPartials;
-    const int tilesJ = (ctx->ny + 1 + kTileJ - 1) / kTileJ;
     const bool h8 = !csr && !is2d(ctx);
-    const unsigned g1 = csr ? gsize(ctx->N)
-                            : (h8 ? gsize(ctx->nn) : (unsigned)(tilesJ * ((ctx->nx + 1 + kTileI - 1) / kTileI)));
+    const K1Shape sh = k1_shape(ctx->nx, ctx->ny);
+    const unsigned g1 = csr ? gsize(ctx->N) : (h8 ? gsize(ctx->nn) : sh.grid);
     const unsigned g2 = std::min<unsigned>(gsize(n), (unsigned)k2_cap());
     PF_CHECK(g1 <= (unsigned)kMaxPartials, PF_ERR_ARG, "grid too large for the reduction scratch");
     k_pcg_init<T><<<g2, kBlock, 0, ctx->stream>>>(n, b, D, x, r, z, p0, partB, S);
@@ -268,8 +298,9 @@ int pcg_device(pf_ctx* ctx, bool csr, const T* b, T* x, const T* D, T* r, T* z, 
             } else if (is2d(ctx)) {
                 const T* Ev = std::is_same<T, float>::value ? (const T*)ctx->d_Ef : (const T*)ctx->d_E;
                 const T* Kv = std::is_same<T, float>::value ? (const T*)ctx->d_K0f : (const T*)ctx->d_K0;
-                k_pcg_k1_q4_tile<T, kTileJ, kTileI><<<g1, kBlock, 0, ctx->stream>>>(
-                    ctx->nx, ctx->ny, tilesJ, k, pold, z, pnew, q, Ev, Kv, ctx->d_fix, partB, (int)g2, partA, S, eps);
+                k_pcg_k1_q4_tile<T, kTileI, kTileJ><<<g1, kBlock, 0, ctx->stream>>>(
+                    ctx->nx, ctx->ny, sh.tilesJ, sh.ntiles, k, pold, z, pnew, q, Ev, Kv, ctx->d_fix, partB, (int)g2,
+                    partA, S, eps);
             }
             k_pcg_k2<T><<<g2, kBlock, 0, ctx->stream>>>(n, k, x, r, z, pnew, q, D, partA, (int)g1, partB, S);
         }
@@ -345,82 +376,45 @@ double norm_nodal(pf_ctx* ctx, const double* a, int* rc) {
     return std::sqrt(reduce_host(ctx, ctx->red.partials, (int)g, 1.0, 0.0, rc));
 }
 
-// Mixed precision: FP32 matrix-free Jacobi-PCG (FP32 vectors, FP32 operator,
-// FP64 dot accumulation) whose residual is replaced by the FP64 true residual
-// b - K x (x accumulated in FP64) each time the FP32 recursive residual has
-// dropped by `delta` since the last replacement: iterative refinement that
-// keeps the Krylov space (reliable updates) instead of restarting CG.
-int pcg_mixed(pf_ctx* ctx, const double* b, double* x, double eps, int itrmax, double delta, pf_solve_stats& st) {
-    const long n = ctx->nv;
-    PcgDev* S = (PcgDev*)ctx->d_pcg;
-    double* partA = ctx->red.partials;
-    double* partB = ctx->red.partials + 2 * kMaxPartials;
-    const int tilesJ = (ctx->ny + 1 + kTileJ - 1) / kTileJ;
-    const bool h8 = !is2d(ctx);
-    const unsigned g1 = h8 ? gsize(ctx->nn) : (unsigned)(tilesJ * ((ctx->nx + 1 + kTileI - 1) / kTileI));
-    const unsigned g2 = std::min<unsigned>(gsize(n), (unsigned)k2_cap());
-    PF_CHECK(g1 <= (unsigned)kMaxPartials, PF_ERR_ARG, "grid too large for the reduction scratch");
+// Mixed precision: FP32 matrix-free Jacobi-PCG inner solves (FP32 vectors
+// and operator, FP64 dot accumulation) inside FP64 iterative refinement:
+// r = b - K x in FP64, solve K d = r in FP32 to inner_eps (relative), x += d
+// in FP64, until ||b - K x|| <= eps ||b||.  Each outer step gains at most
+// about -log10(cond(K) * 6e-8) digits, whatever inner_eps is.
+int pcg_ir(pf_ctx* ctx, const double* b, double* x, double eps, int itrmax, double inner_eps, pf_solve_stats& st) {
     PF_TRY(diag_nodal(ctx, true));
     int rc;
-    double bn = norm_nodal(ctx, b, &rc);
+    const double bn = norm_nodal(ctx, b, &rc);
     PF_TRY(rc);
-    PF_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, ctx->stream));
-    k_convert<double, float><<<gsize(n), kBlock, 0, ctx->stream>>>(n, b, ctx->f_b);
-    k_pcg_init<float><<<g2, kBlock, 0, ctx->stream>>>(n, ctx->f_b, ctx->f_D, ctx->f_x, ctx->f_r, ctx->f_z, ctx->f_p0,
-                                                     partB, S);
-    PF_TRY(check_launch(ctx));
-    const int chunk = 64;
-    int k = 0, replacements = 0, status = 2;
-    double rlast = bn, tn = bn;
-    volatile PcgDev* H = (volatile PcgDev*)ctx->h_pcg;
-    while (k < itrmax) {
-        int kend = std::min(itrmax, k + chunk);
-        for (; k < kend; k++) {
-            if (h8) {
-                k_pcg_k3<float><<<g2, kBlock, 0, ctx->stream>>>(n, k, ctx->f_p0, ctx->f_z, partB, (int)g2, S, 0.0);
-                k_pcg_k1_h8<float><<<g1, kBlock, 0, ctx->stream>>>(g3(ctx), ctx->f_p0, ctx->f_q, ctx->d_Ef, ctx->d_K0f,
-                                                                 ctx->d_fix, partA, S);
-                k_pcg_k2<float><<<g2, kBlock, 0, ctx->stream>>>(n, k, ctx->f_x, ctx->f_r, ctx->f_z, ctx->f_p0,
-                                                              ctx->f_q, ctx->f_D, partA, (int)g1, partB, S);
-                continue;
-            }
-            float* pold = (k & 1) ? ctx->f_p1 : ctx->f_p0;
-            float* pnew = (k & 1) ? ctx->f_p0 : ctx->f_p1;
-            k_pcg_k1_q4_tile<float, kTileJ, kTileI><<<g1, kBlock, 0, ctx->stream>>>(
-                ctx->nx, ctx->ny, tilesJ, k, pold, ctx->f_z, pnew, ctx->f_q, ctx->d_Ef, ctx->d_K0f, ctx->d_fix, partB,
-                (int)g2, partA, S, 0.0);
-            k_pcg_k2<float><<<g2, kBlock, 0, ctx->stream>>>(n, k, ctx->f_x, ctx->f_r, ctx->f_z, pnew, ctx->f_q,
-                                                          ctx->f_D, partA, (int)g1, partB, S);
+    PF_HIP(hipMemsetAsync(x, 0, sizeof(double) * ctx->nv, ctx->stream));
+    double rn = bn;
+    int outer = 0, total = 0, status = 2;
+    const int max_outer = 60;
+    double* r64 = ctx->d_t0;
+    while (outer < max_outer) {
+        PF_TRY(true_residual(ctx, b, x, r64, &rn));
+        if (rn <= eps * bn) {
+            status = 1;
+            break;
         }
-        PF_TRY(check_launch(ctx));
-        k_pcg_final<<<1, kBlock, 0, ctx->stream>>>(k, partB, (int)g2, S, 0.0);  // recursive ||r|| of the last K2
-        PF_HIP(hipMemcpyAsync((void*)H, S, sizeof(PcgDev), hipMemcpyDeviceToHost, ctx->stream));
-        PF_TRY(sync(ctx));
-        const double rrec = H->rnorm;
-        // k_pcg_final marked the solve done; clear it for the next chunk
-        PF_HIP(hipMemsetAsync(&S->done, 0, sizeof(int), ctx->stream));
-        if (!std::isfinite(rrec)) {
+        if (total >= itrmax) break;
+        k_convert<double, float><<<gsize(ctx->nv), kBlock, 0, ctx->stream>>>(ctx->nv, r64, ctx->f_b);
+        PcgRun run;
+        PF_TRY(pcg_device<float>(ctx, false, ctx->f_b, ctx->f_x, ctx->f_D, ctx->f_r, ctx->f_z, ctx->f_p0, ctx->f_p1,
+                                 ctx->f_q, inner_eps, itrmax - total, run));
+        if (!std::isfinite(run.rnorm)) {
             status = 3;
             break;
         }
-        if (rrec <= delta * rlast || rrec <= eps * bn || k >= itrmax) {
-            k_axpy_f32<<<gsize(n), kBlock, 0, ctx->stream>>>(n, ctx->f_x, x);
-            PF_TRY(true_residual(ctx, b, x, ctx->d_t0, &tn));
-            replacements++;
-            if (tn <= eps * bn) {
-                status = 1;
-                break;
-            }
-            k_mixed_replace<<<g2, kBlock, 0, ctx->stream>>>(n, ctx->d_t0, ctx->f_r, ctx->f_D, ctx->f_z, ctx->f_x,
-                                                           partB);
-            PF_TRY(check_launch(ctx));
-            rlast = tn;
-        }
+        total += run.iterations;
+        k_axpy_f32<<<gsize(ctx->nv), kBlock, 0, ctx->stream>>>(ctx->nv, ctx->f_x, x);
+        PF_TRY(check_launch(ctx));
+        outer++;
     }
-    st.iterations = k;
+    st.iterations = total;
     st.status = status;
-    st.outer = replacements;
-    st.rel_residual = bn > 0 ? tn / bn : 0.0;
+    st.outer = outer;
+    st.rel_residual = bn > 0 ? rn / bn : 0.0;
     return PF_OK;
 }
 
@@ -1071,7 +1065,7 @@ int solve_nodal(pf_ctx* ctx, int op, const double* b, double* x, double eps, int
         s.status = run.status;
         s.rel_residual = run.bnorm > 0 ? run.rnorm / run.bnorm : 0.0;
     } else if (op == PF_OP_EBE32) {
-        PF_TRY(pcg_mixed(ctx, b, x, eps, itrmax, 0.1, s));
+        PF_TRY(pcg_ir(ctx, b, x, eps, itrmax, inner_eps, s));
     } else if (op == PF_OP_LEARNED) {
         PF_TRY(pcg_learned(ctx, b, x, eps, itrmax, s));
     } else {
@@ -1911,8 +1905,8 @@ int pf_profile_kernels(pf_ctx* ctx, int op, int reps, double* ms) {
     PcgDev* S = (PcgDev*)ctx->d_pcg;
     double* partA = ctx->red.partials;
     double* partB = ctx->red.partials + 2 * kMaxPartials;
-    const int tilesJ = (ctx->ny + 1 + kTileJ - 1) / kTileJ;
-    const unsigned g1 = (unsigned)(tilesJ * ((ctx->nx + 1 + kTileI - 1) / kTileI));
+    const K1Shape sh = k1_shape(ctx->nx, ctx->ny);
+    const unsigned g1 = sh.grid;
     const unsigned g2 = std::min<unsigned>(gsize(ctx->nv), (unsigned)k2_cap());
     PF_CHECK(g1 <= (unsigned)kMaxPartials, PF_ERR_ARG, "grid too large for the reduction scratch");
     // a defined state: b = load, x = 0, z = b/D, p = 0; launches use k = 0 (no
@@ -1942,8 +1936,8 @@ int pf_profile_kernels(pf_ctx* ctx, int op, int reps, double* ms) {
     };
     if (f32) {
         ms[0] = timed([&] {
-            k_pcg_k1_q4_tile<float, kTileJ, kTileI><<<g1, kBlock, 0, ctx->stream>>>(
-                ctx->nx, ctx->ny, tilesJ, 0, ctx->f_p0, ctx->f_z, ctx->f_p1, ctx->f_q, ctx->d_Ef, ctx->d_K0f, ctx->d_fix,
+            k_pcg_k1_q4_tile<float, kTileI, kTileJ><<<g1, kBlock, 0, ctx->stream>>>(
+                ctx->nx, ctx->ny, sh.tilesJ, sh.ntiles, 0, ctx->f_p0, ctx->f_z, ctx->f_p1, ctx->f_q, ctx->d_Ef, ctx->d_K0f, ctx->d_fix,
                 partB, (int)g2, partA, S, 0.0);
         });
         ms[1] = timed([&] {
@@ -1952,8 +1946,8 @@ int pf_profile_kernels(pf_ctx* ctx, int op, int reps, double* ms) {
         });
     } else {
         ms[0] = timed([&] {
-            k_pcg_k1_q4_tile<double, kTileJ, kTileI><<<g1, kBlock, 0, ctx->stream>>>(
-                ctx->nx, ctx->ny, tilesJ, 0, ctx->d_p0, ctx->d_z, ctx->d_p1, ctx->d_q, ctx->d_E, ctx->d_K0, ctx->d_fix,
+            k_pcg_k1_q4_tile<double, kTileI, kTileJ><<<g1, kBlock, 0, ctx->stream>>>(
+                ctx->nx, ctx->ny, sh.tilesJ, sh.ntiles, 0, ctx->d_p0, ctx->d_z, ctx->d_p1, ctx->d_q, ctx->d_E, ctx->d_K0, ctx->d_fix,
                 partB, (int)g2, partA, S, 0.0);
         });
         ms[1] = timed([&] {

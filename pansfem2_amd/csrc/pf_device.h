@@ -51,15 +51,46 @@ __device__ inline void store_partials(double (&v)[K], double* lds, double* parti
 
 // Every workgroup of the consuming launch reduces the producer's partials in
 // the same fixed order, so all of them obtain the same bits without atomics.
-template <int K>
-__device__ inline void reduce_partials(const double* partials, int nparts, double* lds, double (&out)[K]) {
+// The read is split in two so that a kernel can issue it together with its
+// other global loads and wait once: load_partials fetches entries
+// t, t+B, ..., t+(U-1)B (B = blockDim.x) of every column into registers,
+// sum_partials adds them in that order, then any entries beyond U*B.
+template <int K, int U = 8>
+struct PartRegs {
+    double v[U][K];
+};
+
+template <int K, int U>
+__device__ inline void load_partials(const double* partials, int nparts, PartRegs<K, U>& pr) {
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        const int i = threadIdx.x + u * blockDim.x;
+#pragma unroll
+        for (int k = 0; k < K; k++) pr.v[u][k] = i < nparts ? partials[i * K + k] : 0.0;
+    }
+}
+
+template <int K, int U>
+__device__ inline void sum_partials(const PartRegs<K, U>& pr, const double* partials, int nparts, double* lds,
+                                    double (&out)[K]) {
 #pragma unroll
     for (int k = 0; k < K; k++) out[k] = 0.0;
-    for (int i = threadIdx.x; i < nparts; i += blockDim.x) {
+#pragma unroll
+    for (int u = 0; u < U; u++)
+#pragma unroll
+        for (int k = 0; k < K; k++) out[k] += pr.v[u][k];
+    for (int i = threadIdx.x + U * blockDim.x; i < nparts; i += blockDim.x) {
 #pragma unroll
         for (int k = 0; k < K; k++) out[k] += partials[i * K + k];
     }
     block_sum<K>(out, lds);
+}
+
+template <int K>
+__device__ inline void reduce_partials(const double* partials, int nparts, double* lds, double (&out)[K]) {
+    PartRegs<K, 4> pr;
+    load_partials(partials, nparts, pr);
+    sum_partials(pr, partials, nparts, lds, out);
 }
 
 // ----------------------------------------------------- reference maths ----

@@ -179,10 +179,9 @@ struct PcgDev {
     int iterations, status, done, pad;
 };
 
-__device__ inline void pcg_head(int k, const double* __restrict__ partB, int nbB, PcgDev* __restrict__ S, double eps,
-                                double* lds, double& beta, bool& stop) {
-    double v[2];
-    reduce_partials<2>(partB, nbB, lds, v);  // K2(k-1): (z.r, r.r); init: (z.r, b.b)
+// beta for K1(k) from the sums (z.r, r.r) of K2(k-1) (init: (z.r, b.b)).
+__device__ inline void pcg_head_sums(int k, const double (&v)[2], PcgDev* __restrict__ S, double eps, double& beta,
+                                     bool& stop) {
     stop = false;
     if (k == 0) {
         beta = 0.0;
@@ -212,6 +211,13 @@ __device__ inline void pcg_head(int k, const double* __restrict__ partB, int nbB
         S->beta = beta;
         S->rnorm = rnorm;
     }
+}
+
+__device__ inline void pcg_head(int k, const double* __restrict__ partB, int nbB, PcgDev* __restrict__ S, double eps,
+                                double* lds, double& beta, bool& stop) {
+    double v[2];
+    reduce_partials<2>(partB, nbB, lds, v);
+    pcg_head_sums(k, v, S, eps, beta, stop);
 }
 
 template <class T>
@@ -262,100 +268,148 @@ __global__ void __launch_bounds__(256) k_pcg_k1_q4(int nx, int ny, int k, const 
     store_partials<1>(acc, lds, partA);
 }
 
-// LDS-tiled K1 for quad4 grids.  A workgroup owns a TI x TJ tile of nodes
-// (TJ along j, the contiguous direction).  It stages p_new = beta p_old + z
-// for the tile and a one-node halo in LDS (each halo value computed once),
-// plus the moduli of the tile's elements, then every thread gathers its 3x3
+// LDS-tiled K1 for quad4 grids.  A tile is TI x TJ nodes (TJ along j, the
+// contiguous direction).  For each tile a workgroup stages p_new = beta p_old
+// + z for the tile and a one-node halo in LDS (each halo value computed once)
+// plus the moduli of the tile's elements; every thread then gathers its 3x3
 // neighbourhood from LDS.  Global traffic per node: p_old, z, E read once
 // (plus the halo), p_new and q written once.
-template <class T, int TJ, int TI>
-__global__ void __launch_bounds__(256) k_pcg_k1_q4_tile(int nx, int ny, int tilesJ, int k, const T* __restrict__ pold,
-                                                        const T* __restrict__ z, T* __restrict__ pnew,
-                                                        T* __restrict__ q, const T* __restrict__ E,
-                                                        const T* __restrict__ K0, const uint8_t* __restrict__ fix,
+//
+// Launch shape: workgroups are dispatched round-robin over the 8 XCDs, so
+// XCD x = blockIdx.x % 8 is given the contiguous tile range [x per, (x+1) per)
+// (tile id = tile row * tilesJ + tile column): halo rows shared by
+// neighbouring tiles then come through the same L2.  Within its range a
+// workgroup takes every (gridDim.x / 8)-th tile and loads the next tile's
+// halo and moduli into registers while it computes the current one.
+// gridDim.x must be a multiple of 8.
+template <class T, int TI, int TJ>
+struct K1Tile {
+    static constexpr int HALO = (TI + 2) * (TJ + 2);
+    static constexpr int PER = (HALO + 255) / 256;
+    static constexpr int ES = (TI + 1) * (TJ + 1);
+    static constexpr int PE = (ES + 255) / 256;
+    Vec2<T> po[PER], zz[PER];
+    T ev[PE];
+
+    __device__ inline void load(int t, int tilesJ, int nx, int ny, const T* __restrict__ pold,
+                                const T* __restrict__ z, const T* __restrict__ E) {
+        const int j0 = (t % tilesJ) * TJ, i0 = (t / tilesJ) * TI;
+#pragma unroll
+        for (int u = 0; u < PER; u++) {
+            int h = threadIdx.x + u * 256;
+            int a = h / (TJ + 2), b = h - a * (TJ + 2);
+            int ii = i0 - 1 + a, jj = j0 - 1 + b;
+            po[u] = Vec2<T>{0, 0};
+            zz[u] = Vec2<T>{0, 0};
+            if (h < HALO && ii >= 0 && ii <= nx && jj >= 0 && jj <= ny) {
+                long m = (long)(ny + 1) * ii + jj;
+                po[u] = load2(pold, m);
+                zz[u] = load2(z, m);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < PE; u++) {
+            int h = threadIdx.x + u * 256;
+            int a = h / (TJ + 1), b = h - a * (TJ + 1);
+            int ei = i0 - 1 + a, ej = j0 - 1 + b;
+            ev[u] = (h < ES && ei >= 0 && ei < nx && ej >= 0 && ej < ny) ? E[(long)ny * ei + ej] : (T)0;
+        }
+    }
+};
+
+template <class T, int TI, int TJ>
+__global__ void __launch_bounds__(256) k_pcg_k1_q4_tile(int nx, int ny, int tilesJ, int ntiles, int k,
+                                                        const T* __restrict__ pold, const T* __restrict__ z,
+                                                        T* __restrict__ pnew, T* __restrict__ q,
+                                                        const T* __restrict__ E, const T* __restrict__ K0,
+                                                        const uint8_t* __restrict__ fix,
                                                         const double* __restrict__ partB, int nbB,
                                                         double* __restrict__ partA, PcgDev* __restrict__ S,
                                                         double eps) {
     __shared__ double lds[32];
     __shared__ Vec2<T> sp[TI + 2][TJ + 2];
     __shared__ T se[TI + 1][TJ + 1];
+    using Tile = K1Tile<T, TI, TJ>;
+    const int xcd = blockIdx.x & 7, nb = gridDim.x >> 3;
+    const int per = (ntiles + 7) >> 3;
+    const int tend = min(ntiles, (xcd + 1) * per);
+    int t = xcd * per + (blockIdx.x >> 3);
+    // The first tile's loads, the previous launch's partials and the done
+    // flag are all issued before the first wait.
+    Tile cur;
+    if (t < tend) cur.load(t, tilesJ, nx, ny, pold, z, E);
+    PartRegs<2, 8> pr;
+    load_partials(partB, nbB, pr);
     if (S->done) return;
-    const int j0 = (blockIdx.x % tilesJ) * TJ, i0 = (blockIdx.x / tilesJ) * TI;
-    // issue the halo and moduli loads first: they do not depend on beta, so
-    // their latency overlaps the reduction of the previous launch's partials
-    constexpr int HALO = (TI + 2) * (TJ + 2);
-    constexpr int PER = (HALO + 255) / 256;
-    Vec2<T> po[PER], zz[PER];
-#pragma unroll
-    for (int t = 0; t < PER; t++) {
-        int h = threadIdx.x + t * 256;
-        int a = h / (TJ + 2), b = h - a * (TJ + 2);
-        int ii = i0 - 1 + a, jj = j0 - 1 + b;
-        po[t] = Vec2<T>{0, 0};
-        zz[t] = Vec2<T>{0, 0};
-        if (h < HALO && ii >= 0 && ii <= nx && jj >= 0 && jj <= ny) {
-            long m = (long)(ny + 1) * ii + jj;
-            po[t] = load2(pold, m);
-            zz[t] = load2(z, m);
-        }
-    }
-    for (int h = threadIdx.x; h < (TI + 1) * (TJ + 1); h += blockDim.x) {
-        int a = h / (TJ + 1), b = h - a * (TJ + 1);
-        int ei = i0 - 1 + a, ej = j0 - 1 + b;
-        se[a][b] = (ei >= 0 && ei < nx && ej >= 0 && ej < ny) ? E[(long)ny * ei + ej] : (T)0;
-    }
     double beta;
     bool stop;
-    pcg_head(k, partB, nbB, S, eps, lds, beta, stop);
+    {
+        double v[2];
+        sum_partials(pr, partB, nbB, lds, v);
+        pcg_head_sums(k, v, S, eps, beta, stop);
+    }
     if (stop) return;
     const T tb = (T)beta;
-#pragma unroll
-    for (int t = 0; t < PER; t++) {
-        int h = threadIdx.x + t * 256;
-        if (h < HALO) {
-            int a = h / (TJ + 2), b = h - a * (TJ + 2);
-            Vec2<T> v;
-            v.x = tb * po[t].x + zz[t].x;  // xeaxpy: p = beta p + z (CG.h:42-49)
-            v.y = tb * po[t].y + zz[t].y;
-            sp[a][b] = v;
-        }
-    }
-    __syncthreads();
     double acc[1] = {0.0};
-    const int c = threadIdx.x % TJ;
-    for (int r = threadIdx.x / TJ; r < TI; r += blockDim.x / TJ) {
-        const int i = i0 + r, j = j0 + c;
-        if (i > nx || j > ny) continue;
-        T ax = 0, ay = 0, ex, ey;
-        if (i >= 1 && j >= 1) {  // element (i-1, j-1): this node is TR
-            elem_rows_q4(K0, 2, sp[r][c], sp[r + 1][c], sp[r + 1][c + 1], sp[r][c + 1], ex, ey);
-            T Ee = se[r][c];
-            ax += Ee * ex; ay += Ee * ey;
+    for (; t < tend; t += nb) {
+        const int j0 = (t % tilesJ) * TJ, i0 = (t / tilesJ) * TI;
+        __syncthreads();  // the previous tile's LDS reads are done
+#pragma unroll
+        for (int u = 0; u < Tile::PER; u++) {
+            int h = threadIdx.x + u * 256;
+            if (h < Tile::HALO) {
+                int a = h / (TJ + 2), b = h - a * (TJ + 2);
+                Vec2<T> v;
+                v.x = tb * cur.po[u].x + cur.zz[u].x;  // xeaxpy: p = beta p + z (CG.h:42-49)
+                v.y = tb * cur.po[u].y + cur.zz[u].y;
+                sp[a][b] = v;
+            }
         }
-        if (i >= 1 && j < ny) {  // element (i-1, j): BR
-            elem_rows_q4(K0, 1, sp[r][c + 1], sp[r + 1][c + 1], sp[r + 1][c + 2], sp[r][c + 2], ex, ey);
-            T Ee = se[r][c + 1];
-            ax += Ee * ex; ay += Ee * ey;
+#pragma unroll
+        for (int u = 0; u < Tile::PE; u++) {
+            int h = threadIdx.x + u * 256;
+            if (h < Tile::ES) {
+                int a = h / (TJ + 1), b = h - a * (TJ + 1);
+                se[a][b] = cur.ev[u];
+            }
         }
-        if (i < nx && j >= 1) {  // element (i, j-1): TL
-            elem_rows_q4(K0, 3, sp[r + 1][c], sp[r + 2][c], sp[r + 2][c + 1], sp[r + 1][c + 1], ex, ey);
-            T Ee = se[r + 1][c];
-            ax += Ee * ex; ay += Ee * ey;
+        __syncthreads();
+        if (t + nb < tend) cur.load(t + nb, tilesJ, nx, ny, pold, z, E);  // in flight during the compute
+        const int c = threadIdx.x % TJ;
+        for (int r = threadIdx.x / TJ; r < TI; r += 256 / TJ) {
+            const int i = i0 + r, j = j0 + c;
+            if (i > nx || j > ny) continue;
+            T ax = 0, ay = 0, ex, ey;
+            if (i >= 1 && j >= 1) {  // element (i-1, j-1): this node is TR
+                elem_rows_q4(K0, 2, sp[r][c], sp[r + 1][c], sp[r + 1][c + 1], sp[r][c + 1], ex, ey);
+                T Ee = se[r][c];
+                ax += Ee * ex; ay += Ee * ey;
+            }
+            if (i >= 1 && j < ny) {  // element (i-1, j): BR
+                elem_rows_q4(K0, 1, sp[r][c + 1], sp[r + 1][c + 1], sp[r + 1][c + 2], sp[r][c + 2], ex, ey);
+                T Ee = se[r][c + 1];
+                ax += Ee * ex; ay += Ee * ey;
+            }
+            if (i < nx && j >= 1) {  // element (i, j-1): TL
+                elem_rows_q4(K0, 3, sp[r + 1][c], sp[r + 2][c], sp[r + 2][c + 1], sp[r + 1][c + 1], ex, ey);
+                T Ee = se[r + 1][c];
+                ax += Ee * ex; ay += Ee * ey;
+            }
+            if (i < nx && j < ny) {  // element (i, j): BL
+                elem_rows_q4(K0, 0, sp[r + 1][c + 1], sp[r + 2][c + 1], sp[r + 2][c + 2], sp[r + 1][c + 2], ex, ey);
+                T Ee = se[r + 1][c + 1];
+                ax += Ee * ex; ay += Ee * ey;
+            }
+            const long n = (long)(ny + 1) * i + j;
+            uint8_t f = fix[n];
+            if (f & 1) ax = 0;
+            if (f & 2) ay = 0;
+            Vec2<T> pv = sp[r + 1][c + 1];
+            store2(pnew, n, pv);
+            store2(q, n, Vec2<T>{ax, ay});
+            acc[0] += (double)pv.x * (double)ax;
+            acc[0] += (double)pv.y * (double)ay;
         }
-        if (i < nx && j < ny) {  // element (i, j): BL
-            elem_rows_q4(K0, 0, sp[r + 1][c + 1], sp[r + 2][c + 1], sp[r + 2][c + 2], sp[r + 1][c + 2], ex, ey);
-            T Ee = se[r + 1][c + 1];
-            ax += Ee * ex; ay += Ee * ey;
-        }
-        const long n = (long)(ny + 1) * i + j;
-        uint8_t f = fix[n];
-        if (f & 1) ax = 0;
-        if (f & 2) ay = 0;
-        Vec2<T> pv = sp[r + 1][c + 1];
-        store2(pnew, n, pv);
-        store2(q, n, Vec2<T>{ax, ay});
-        acc[0] += (double)pv.x * (double)ax;
-        acc[0] += (double)pv.y * (double)ay;
     }
     store_partials<1>(acc, lds, partA);
 }
@@ -396,8 +450,8 @@ __global__ void __launch_bounds__(256) k_pcg_k2(long n, int k, T* __restrict__ x
                                                 const T* __restrict__ D, const double* __restrict__ partA, int nbA,
                                                 double* __restrict__ partB, PcgDev* __restrict__ S) {
     __shared__ double lds[32];
-    if (S->done) return;
-    // first pair of every thread loaded before the partial reduction
+    // first pair of every thread, the partials and the done flag are all
+    // loaded before the first wait
     const long n2 = n / 2;
     const long stride = (long)gridDim.x * blockDim.x;
     long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
@@ -405,8 +459,11 @@ __global__ void __launch_bounds__(256) k_pcg_k2(long n, int k, T* __restrict__ x
     if (i < n2) {
         xi = load2(x, i); ri = load2(r, i); pi = load2(p, i); qi = load2(q, i); di = load2(D, i);
     }
+    PartRegs<1, 8> pr;
+    load_partials(partA, nbA, pr);
+    if (S->done) return;
     double v[1];
-    reduce_partials<1>(partA, nbA, lds, v);
+    sum_partials(pr, partA, nbA, lds, v);
     const double zr = S->zr[k & 1];
     const double alpha = zr / v[0];  // CG.h:433
     if (threadIdx.x == 0 && blockIdx.x == 0) {
@@ -474,26 +531,6 @@ __global__ void __launch_bounds__(256) k_pcg_init(long n, const T* __restrict__ 
         S->status = 0;
         S->iterations = 0;
     }
-}
-
-// Residual replacement of the mixed-precision solve: r32 = (float) r64,
-// z = r32 ./ D32, the FP32 correction accumulator cleared, and the (z.r, r.r)
-// partials that the next K1 reads rewritten for the new residual.
-__global__ void __launch_bounds__(256) k_mixed_replace(long n, const double* __restrict__ r64, float* __restrict__ r32,
-                                                       const float* __restrict__ D, float* __restrict__ z,
-                                                       float* __restrict__ xacc, double* __restrict__ partB) {
-    __shared__ double lds[32];
-    double acc[2] = {0.0, 0.0};
-    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-        float ri = (float)r64[i];
-        float zi = ri / D[i];
-        r32[i] = ri;
-        z[i] = zi;
-        xacc[i] = 0.0f;
-        acc[0] += (double)zi * (double)ri;
-        acc[1] += (double)ri * (double)ri;
-    }
-    store_partials<2>(acc, lds, partB);
 }
 
 // After the host's last K2: test convergence of the final iterate the way
