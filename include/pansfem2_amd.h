@@ -150,6 +150,11 @@ int pf_set_moduli(pf_ctx* ctx, const double* E);
  * reference arithmetic from the current moduli.  Structure and values are
  * bit-identical to the reference's.  *nnz receives the entry count. */
 int pf_assemble_csr(pf_ctx* ctx, int64_t* nnz);
+/* Same structure, values from caller-given element matrices instead of the
+ * device's: ke_all is ne x nd x nd (row-major, element order of the grid),
+ * e.g. what the C++ facade's Assembling(LILCSR&, ...) recorded.  The values
+ * are the reference's sums (Assembling.h:46-66) for any element matrices. */
+int pf_assemble_csr_elements(pf_ctx* ctx, const double* ke_all, int64_t* nnz);
 /* Copy the assembled CSR to the host (indptr N+1, indices/data nnz). */
 int pf_get_csr(pf_ctx* ctx, int32_t* indptr, int32_t* indices, double* data);
 
@@ -180,12 +185,24 @@ int pf_filter_setup(pf_ctx* ctx, double R);
 /* mode 0: GetFilteredVariables (DensityFilter.h:45-56);
  * mode 1: GetFilteredSensitivitis (DensityFilter.h:60-71). ne in, ne out. */
 int pf_filter(pf_ctx* ctx, int mode, const double* in, double* out);
+/* DensityFilter<T>(n, neighbors, w) with caller-given lists
+ * (DensityFilter.h:34-43): element e's neighbours are idx[ptr[e]..ptr[e+1])
+ * with weights w (same order); pf_filter then uses the lists (sums in list
+ * order, bit-identical to the reference) until pf_filter_setup is called.
+ * The device OC closure (pf_oc_update, pf_simp_run) needs the grid rule. */
+int pf_filter_set_lists(pf_ctx* ctx, const int32_t* ptr, const int32_t* idx, const double* w);
 
 /* ---- optimisers -------------------------------------------------------- */
 /* OC<T>::UpdateVariables (src/Optimize/Solver/OC.h:76-107) with the
  * sample's filtered-volume closure; x (ne) updated in place. */
 int pf_oc_update(pf_ctx* ctx, double* x, const double* dfdx, const double* dgdx, const pf_oc_params* prm,
                  double* lambda_out, int* steps_out);
+/* One bisection candidate of OC<T>::UpdateVariables for a given lambda
+ * (OC.h:85-92): xplus = clamp(x (-dfdx/(dgdx lambda))^iota) within the move
+ * limit.  For callers that evaluate their own constraint closure (the C++
+ * facade's OC<T> with a host callable). ne in, ne out. */
+int pf_oc_candidate(pf_ctx* ctx, const double* x, const double* dfdx, const double* dgdx, double lambda,
+                    double iota, double movelimit, double* xplus);
 /* MMA<T> (src/Optimize/Solver/MMA.h:15-509), m = 1, state kept in the
  * context: pf_mma_init resets it (ctor + SetParameters), pf_mma_update runs
  * UpdateVariables(x, f, dfdx, {g}, {dgdx}). */

@@ -93,6 +93,7 @@ def lib():
             "pf_set_density": ([c_void_p, _f64p], c_int),
             "pf_set_moduli": ([c_void_p, _f64p], c_int),
             "pf_assemble_csr": ([c_void_p, P(c_int64)], c_int),
+            "pf_assemble_csr_elements": ([c_void_p, _f64p, P(c_int64)], c_int),
             "pf_get_csr": ([c_void_p, _i32p, _i32p, _f64p], c_int),
             "pf_apply_K": ([c_void_p, c_int, _f64p, _f64p], c_int),
             "pf_pcg": ([c_void_p, c_int, _f64p, _f64p, c_double, c_int, P(SolveStats)], c_int),
@@ -100,6 +101,8 @@ def lib():
             "pf_compliance_sens": ([c_void_p, _f64p, c_double, P(c_double), _f64p], c_int),
             "pf_filter_setup": ([c_void_p, c_double], c_int),
             "pf_filter": ([c_void_p, c_int, _f64p, _f64p], c_int),
+            "pf_filter_set_lists": ([c_void_p, _i32p, _i32p, _f64p], c_int),
+            "pf_oc_candidate": ([c_void_p, _f64p, _f64p, _f64p, c_double, c_double, c_double, _f64p], c_int),
             "pf_oc_update": ([c_void_p, _f64p, _f64p, _f64p, P(OcParams), P(c_double), P(c_int)], c_int),
             "pf_mma_init": ([c_void_p, P(MmaParams)], c_int),
             "pf_mma_update": ([c_void_p, _f64p, c_double, _f64p, c_double, _f64p, P(c_int)], c_int),
@@ -185,6 +188,7 @@ class Context:
         self._ck(lib().pf_grid2d(self._h, nx, ny, float(nx if lx is None else lx), float(ny if ly is None else ly)))
         self.nx, self.ny, self.dim, self.ndpn = nx, ny, 2, 2
         self.nn, self.ne = (nx + 1) * (ny + 1), nx * ny
+        self.nd = 8
         return self
 
     def grid3d(self, nx, ny, nz, lx=None, ly=None, lz=None):
@@ -192,6 +196,7 @@ class Context:
                                  float(ny if ly is None else ly), float(nz if lz is None else lz)))
         self.nx, self.ny, self.nz, self.dim, self.ndpn = nx, ny, nz, 3, 3
         self.nn, self.ne = (nx + 1) * (ny + 1) * (nz + 1), nx * ny * nz
+        self.nd = 24
         return self
 
     def set_dirichlet(self, nodes, dofs):
@@ -244,6 +249,15 @@ class Context:
         self.nnz = nnz.value
         return self.nnz
 
+    def assemble_csr_elements(self, ke_all):
+        """CSR from caller-given element matrices (ne x nd x nd)."""
+        ke = np.ascontiguousarray(ke_all, dtype=np.float64).reshape(-1)
+        assert ke.size == self.ne * self.nd * self.nd, "ke_all must hold ne x nd x nd values"
+        nnz = ctypes.c_int64(0)
+        self._ck(lib().pf_assemble_csr_elements(self._h, ke, ctypes.byref(nnz)))
+        self.nnz = nnz.value
+        return self.nnz
+
     def csr(self):
         ip = np.zeros(self.N + 1, np.int32)
         ix = np.zeros(max(self.nnz, 1), np.int32)
@@ -278,6 +292,16 @@ class Context:
 
     def filter_setup(self, R=1.5):
         self._ck(lib().pf_filter_setup(self._h, R))
+
+    def filter_set_lists(self, ptr, idx, w):
+        """DensityFilter(n, neighbors, w) with explicit lists (CSR-like)."""
+        self._ck(lib().pf_filter_set_lists(self._h, np.ascontiguousarray(ptr, np.int32),
+                                           np.ascontiguousarray(idx, np.int32), _f64(w)))
+
+    def oc_candidate(self, x, dfdx, dgdx, lam, iota=0.5, movelimit=0.2):
+        out = np.zeros(self.ne)
+        self._ck(lib().pf_oc_candidate(self._h, _f64(x), _f64(dfdx), _f64(dgdx), lam, iota, movelimit, out))
+        return out
 
     def filter(self, x, mode=PF_FILTER_VARIABLES):
         out = np.zeros(self.ne)

@@ -123,10 +123,12 @@ void free_grid(pf_ctx* ctx) {
     dfree(ctx->d_e0); dfree(ctx->d_e1); dfree(ctx->d_e2); dfree(ctx->d_e3); dfree(ctx->d_e4); dfree(ctx->d_e5);
     dfree(ctx->d_cg); dfree(ctx->d_load);
     dfree(ctx->d_indptr); dfree(ctx->d_indices); dfree(ctx->d_data); dfree(ctx->d_ke_all);
+    dfree(ctx->d_fptr); dfree(ctx->d_fidx); dfree(ctx->d_fw);
     ctx->nnz = -1;
     ctx->N = 0;
     ctx->have_E = false;
     ctx->have_filter = false;
+    ctx->have_lists = false;
     release_net(ctx);
 }
 
@@ -178,11 +180,15 @@ int upload_dofmap(pf_ctx* ctx, const std::vector<uint8_t>& fix) {
     return sync(ctx);
 }
 
+// need_E: the matrix-free operators (element matrix and moduli); the
+// assembled-CSR operator needs only the grid and the DOF map.
 int require_ready(pf_ctx* ctx, bool need_E) {
     PF_CHECK(ctx->dim == 2 || ctx->dim == 3, PF_ERR_STATE, "no grid: call pf_grid2d/pf_grid3d first");
     PF_CHECK(ctx->N > 0, PF_ERR_STATE, "no DOF map: call pf_set_dirichlet first");
-    PF_CHECK(!ctx->h_K0.empty(), PF_ERR_STATE, "no element matrix: call pf_set_element_matrix first");
-    if (need_E) PF_CHECK(ctx->have_E, PF_ERR_STATE, "no element moduli: call pf_set_density or pf_set_moduli first");
+    if (need_E) {
+        PF_CHECK(!ctx->h_K0.empty(), PF_ERR_STATE, "no element matrix: call pf_set_element_matrix first");
+        PF_CHECK(ctx->have_E, PF_ERR_STATE, "no element moduli: call pf_set_density or pf_set_moduli first");
+    }
     return PF_OK;
 }
 
@@ -1060,10 +1066,19 @@ int solve_nodal(pf_ctx* ctx, int op, const double* b, double* x, double eps, int
         PcgRun run;
         PF_TRY(pcg_device<double>(ctx, true, bf, xf, ctx->d_D, ctx->d_r, ctx->d_z, ctx->d_p0, ctx->d_p1, ctx->d_q, eps,
                                   itrmax, run));
+        // true residual with the assembled matrix itself
+        const unsigned g = gsize(ctx->N);
+        k_csr_residual<<<g, kBlock, 0, ctx->stream>>>(ctx->N, ctx->d_indptr, ctx->d_indices, ctx->d_data, bf, xf,
+                                                      ctx->red.partials);
+        PF_TRY(check_launch(ctx));
+        int rc;
+        const double rr = reduce_host(ctx, ctx->red.partials, (int)g, 1.0, 0.0, &rc);
+        PF_TRY(rc);
         PF_TRY(to_nodal(ctx, xf, x));
         s.iterations = run.iterations;
         s.status = run.status;
         s.rel_residual = run.bnorm > 0 ? run.rnorm / run.bnorm : 0.0;
+        s.true_residual = run.bnorm > 0 ? std::sqrt(rr) / run.bnorm : 0.0;
     } else if (op == PF_OP_EBE32) {
         PF_TRY(pcg_ir(ctx, b, x, eps, itrmax, inner_eps, s));
     } else if (op == PF_OP_LEARNED) {
@@ -1072,7 +1087,7 @@ int solve_nodal(pf_ctx* ctx, int op, const double* b, double* x, double eps, int
         ctx->err = "unknown operator";
         return PF_ERR_ARG;
     }
-    {
+    if (op != PF_OP_CSR64) {
         double trn;
         int rc;
         PF_TRY(true_residual(ctx, b, x, ctx->d_t0, &trn));
@@ -1089,7 +1104,8 @@ int solve_nodal(pf_ctx* ctx, int op, const double* b, double* x, double eps, int
 // ------------------------------------------------------------------ OC ----
 int oc_device(pf_ctx* ctx, const pf_oc_params* prm, const double* x, const double* dfdx, const double* dgdx,
               double* xplus, double* lambda_out, int* steps_out) {
-    PF_CHECK(ctx->have_filter, PF_ERR_STATE, "no filter: call pf_filter_setup first");
+    PF_CHECK(ctx->have_filter && !ctx->have_lists, PF_ERR_STATE,
+             "the device OC closure filters with the grid rule: call pf_filter_setup first");
     OcDev* S = (OcDev*)ctx->d_oc;
     OcDev init{};
     init.l0i = prm->lambdamin;
@@ -1136,7 +1152,12 @@ int oc_device(pf_ctx* ctx, const pf_oc_params* prm, const double* x, const doubl
 }
 
 int filter_device(pf_ctx* ctx, const double* in, double* out) {
-    PF_CHECK(ctx->have_filter, PF_ERR_STATE, "no filter: call pf_filter_setup first");
+    if (ctx->have_lists) {
+        k_filter_lists<<<gsize(ctx->ne), kBlock, 0, ctx->stream>>>(ctx->ne, ctx->d_fptr, ctx->d_fidx, ctx->d_fw, in,
+                                                                  out);
+        return check_launch(ctx);
+    }
+    PF_CHECK(ctx->have_filter, PF_ERR_STATE, "no filter: call pf_filter_setup or pf_filter_set_lists first");
     if (is2d(ctx))
         k_filter_q4<<<gsize(ctx->ne), kBlock, 0, ctx->stream>>>(ctx->nx, ctx->ny, ctx->win, ctx->R, ctx->d_cg, in,
                                                                out);
@@ -1595,11 +1616,15 @@ int pf_set_moduli(pf_ctx* ctx, const double* E) {
     return sync(ctx);
 }
 
-int pf_assemble_csr(pf_ctx* ctx, int64_t* nnz) {
-    if (!ctx) return PF_ERR_ARG;
-    PF_TRY(require_ready(ctx, true));
-    PF_TRY(set_device(ctx));
+// CSR structure in closed form on the grid, values gathered per row from
+// the element matrices in ascending element id (Assembling.h:46-66 summed
+// from 0.0, then CSR(LILCSR&) sorted by column, CSR.h:92-105).  ke_host:
+// caller-given element matrices (ne x nd x nd); NULL: computed on the device
+// in the reference's arithmetic from the current moduli.
+int assemble_csr(pf_ctx* ctx, const double* ke_host, int64_t* nnz) {
     const int N = ctx->N;
+    dfree(ctx->d_indptr); dfree(ctx->d_indices); dfree(ctx->d_data);
+    ctx->nnz = -1;
     PF_TRY(dalloc(ctx, &ctx->d_indptr, N + 1));
     int32_t* rowlen = nullptr;
     PF_TRY(dalloc(ctx, &rowlen, N));
@@ -1616,26 +1641,46 @@ int pf_assemble_csr(pf_ctx* ctx, int64_t* nnz) {
     PF_HIP(e1);
     PF_HIP(e2);
     PF_HIP(e3);
-    ctx->nnz = total;
     PF_TRY(dalloc(ctx, &ctx->d_indices, std::max<long>(total, 1)));
     PF_TRY(dalloc(ctx, &ctx->d_data, std::max<long>(total, 1)));
-    PF_TRY(dalloc(ctx, &ctx->d_ke_all, ctx->ne * (long)ctx->nd * ctx->nd));
-    if (is2d(ctx)) {
+    const long kesz = ctx->ne * (long)ctx->nd * ctx->nd;
+    PF_TRY(dalloc(ctx, &ctx->d_ke_all, kesz));
+    if (ke_host) {
+        PF_HIP(hipMemcpyAsync(ctx->d_ke_all, ke_host, sizeof(double) * kesz, hipMemcpyHostToDevice, ctx->stream));
+    } else if (is2d(ctx)) {
         k_ke_all_q4<<<gsize(ctx->ne), kBlock, 0, ctx->stream>>>(ctx->nx, ctx->ny, ctx->lx, ctx->ly, ctx->d_E, ctx->nu,
                                                                ctx->t, ctx->d_ke_all);
-        k_csr_fill_q4<<<gsize(ctx->nn), kBlock, 0, ctx->stream>>>(ctx->nx, ctx->ny, ctx->d_n2g, ctx->d_indptr,
-                                                                 ctx->d_ke_all, ctx->d_indices, ctx->d_data);
     } else {
         k_ke_all_h8<<<gsize(ctx->ne), kBlock, 0, ctx->stream>>>(g3(ctx), ctx->lx, ctx->ly, ctx->lz, ctx->d_E, ctx->nu,
                                                                ctx->d_ke_all);
+    }
+    if (is2d(ctx))
+        k_csr_fill_q4<<<gsize(ctx->nn), kBlock, 0, ctx->stream>>>(ctx->nx, ctx->ny, ctx->d_n2g, ctx->d_indptr,
+                                                                 ctx->d_ke_all, ctx->d_indices, ctx->d_data);
+    else
         k_csr_fill_h8<<<gsize(ctx->nn), kBlock, 0, ctx->stream>>>(g3(ctx), ctx->d_n2g, ctx->d_indptr, ctx->d_ke_all,
                                                                  ctx->d_indices, ctx->d_data);
-    }
     PF_TRY(check_launch(ctx));
     PF_TRY(sync(ctx));
     dfree(ctx->d_ke_all);
+    ctx->nnz = total;
     if (nnz) *nnz = total;
     return PF_OK;
+}
+
+int pf_assemble_csr(pf_ctx* ctx, int64_t* nnz) {
+    if (!ctx) return PF_ERR_ARG;
+    PF_TRY(require_ready(ctx, true));
+    PF_TRY(set_device(ctx));
+    return assemble_csr(ctx, nullptr, nnz);
+}
+
+int pf_assemble_csr_elements(pf_ctx* ctx, const double* ke_all, int64_t* nnz) {
+    if (!ctx || !ke_all) return PF_ERR_ARG;
+    PF_CHECK(ctx->dim == 2 || ctx->dim == 3, PF_ERR_STATE, "no grid: call pf_grid2d/pf_grid3d first");
+    PF_CHECK(ctx->N > 0, PF_ERR_STATE, "no DOF map: call pf_set_dirichlet first");
+    PF_TRY(set_device(ctx));
+    return assemble_csr(ctx, ke_all, nnz);
 }
 
 int pf_get_csr(pf_ctx* ctx, int32_t* indptr, int32_t* indices, double* data) {
@@ -1652,7 +1697,7 @@ int pf_get_csr(pf_ctx* ctx, int32_t* indptr, int32_t* indices, double* data) {
 
 int pf_apply_K(pf_ctx* ctx, int op, const double* x_free, double* y_free) {
     if (!ctx || !x_free || !y_free) return PF_ERR_ARG;
-    PF_TRY(require_ready(ctx, true));
+    PF_TRY(require_ready(ctx, op != PF_OP_CSR64));
     PF_TRY(set_device(ctx));
     const int N = ctx->N;
     PF_HIP(hipMemcpyAsync(ctx->d_t1, x_free, sizeof(double) * N, hipMemcpyHostToDevice, ctx->stream));
@@ -1682,7 +1727,7 @@ int pf_apply_K(pf_ctx* ctx, int op, const double* x_free, double* y_free) {
 int pf_pcg_dev(pf_ctx* ctx, int op, const double* b_free_dev, double* x_free_dev, double eps, int itrmax,
                pf_solve_stats* st) {
     if (!ctx || !b_free_dev || !x_free_dev) return PF_ERR_ARG;
-    PF_TRY(require_ready(ctx, true));
+    PF_TRY(require_ready(ctx, op != PF_OP_CSR64));
     PF_CHECK(itrmax >= 0 && eps >= 0, PF_ERR_ARG, "pf_pcg: itrmax and eps must be non-negative");
     PF_TRY(set_device(ctx));
     PF_TRY(to_nodal(ctx, b_free_dev, ctx->d_b));
@@ -1695,7 +1740,7 @@ int pf_pcg_dev(pf_ctx* ctx, int op, const double* b_free_dev, double* x_free_dev
 
 int pf_pcg(pf_ctx* ctx, int op, const double* b_free, double* x_free, double eps, int itrmax, pf_solve_stats* st) {
     if (!ctx || !b_free || !x_free) return PF_ERR_ARG;
-    PF_TRY(require_ready(ctx, true));
+    PF_TRY(require_ready(ctx, op != PF_OP_CSR64));
     for (int i = 0; i < ctx->N; i++) PF_CHECK(std::isfinite(b_free[i]), PF_ERR_NUMERIC, "pf_pcg: non-finite b");
     PF_TRY(set_device(ctx));
     double *bd = nullptr, *xd = nullptr;
@@ -1735,6 +1780,34 @@ int pf_filter_setup(pf_ctx* ctx, double R) {
     // is h up to rounding of the node coordinates)
     ctx->win = (int)std::floor(R / h * (1.0 + 1e-9));
     ctx->have_filter = true;
+    ctx->have_lists = false;
+    return PF_OK;
+}
+
+int pf_filter_set_lists(pf_ctx* ctx, const int32_t* ptr, const int32_t* idx, const double* w) {
+    if (!ctx || !ptr || !idx || !w) return PF_ERR_ARG;
+    PF_CHECK(ctx->dim > 0, PF_ERR_STATE, "no grid");
+    const long ne = ctx->ne;
+    PF_CHECK(ptr[0] == 0, PF_ERR_ARG, "pf_filter_set_lists: ptr[0] must be 0");
+    for (long e = 0; e < ne; e++)
+        PF_CHECK(ptr[e + 1] > ptr[e], PF_ERR_ARG, "pf_filter_set_lists: every element needs at least one neighbour");
+    const long total = ptr[ne];
+    double wmin = 0.0;
+    for (long k = 0; k < total; k++) {
+        PF_CHECK(idx[k] >= 0 && idx[k] < ne, PF_ERR_ARG, "pf_filter_set_lists: neighbour index out of range");
+        PF_CHECK(std::isfinite(w[k]), PF_ERR_ARG, "pf_filter_set_lists: non-finite weight");
+        wmin = std::min(wmin, w[k]);
+    }
+    PF_TRY(set_device(ctx));
+    dfree(ctx->d_fptr); dfree(ctx->d_fidx); dfree(ctx->d_fw);
+    PF_TRY(dalloc(ctx, &ctx->d_fptr, ne + 1));
+    PF_TRY(dalloc(ctx, &ctx->d_fidx, total));
+    PF_TRY(dalloc(ctx, &ctx->d_fw, total));
+    PF_HIP(hipMemcpyAsync(ctx->d_fptr, ptr, sizeof(int32_t) * (ne + 1), hipMemcpyHostToDevice, ctx->stream));
+    PF_HIP(hipMemcpyAsync(ctx->d_fidx, idx, sizeof(int32_t) * total, hipMemcpyHostToDevice, ctx->stream));
+    PF_HIP(hipMemcpyAsync(ctx->d_fw, w, sizeof(double) * total, hipMemcpyHostToDevice, ctx->stream));
+    PF_TRY(sync(ctx));
+    ctx->have_lists = true;
     return PF_OK;
 }
 
@@ -1763,6 +1836,22 @@ int pf_oc_update(pf_ctx* ctx, double* x, const double* dfdx, const double* dgdx,
     PF_HIP(hipMemcpyAsync(x, ctx->d_e3, bytes, hipMemcpyDeviceToHost, ctx->stream));
     PF_TRY(sync(ctx));
     return rc;
+}
+
+int pf_oc_candidate(pf_ctx* ctx, const double* x, const double* dfdx, const double* dgdx, double lambda,
+                    double iota, double movelimit, double* xplus) {
+    if (!ctx || !x || !dfdx || !dgdx || !xplus) return PF_ERR_ARG;
+    PF_CHECK(ctx->dim > 0, PF_ERR_STATE, "no grid");
+    PF_TRY(set_device(ctx));
+    const long ne = ctx->ne;
+    PF_HIP(hipMemcpyAsync(ctx->d_e0, x, sizeof(double) * ne, hipMemcpyHostToDevice, ctx->stream));
+    PF_HIP(hipMemcpyAsync(ctx->d_e1, dfdx, sizeof(double) * ne, hipMemcpyHostToDevice, ctx->stream));
+    PF_HIP(hipMemcpyAsync(ctx->d_e2, dgdx, sizeof(double) * ne, hipMemcpyHostToDevice, ctx->stream));
+    k_oc_candidate<<<gsize(ne), kBlock, 0, ctx->stream>>>(ne, ctx->d_e0, ctx->d_e1, ctx->d_e2, lambda, iota, movelimit,
+                                                         ctx->d_e4);
+    PF_TRY(check_launch(ctx));
+    PF_HIP(hipMemcpyAsync(xplus, ctx->d_e4, sizeof(double) * ne, hipMemcpyDeviceToHost, ctx->stream));
+    return sync(ctx);
 }
 
 int pf_mma_init(pf_ctx* ctx, const pf_mma_params* prm) {

@@ -104,10 +104,14 @@ struct pf_ctx {
     double* d_data = nullptr;
     double* d_ke_all = nullptr;  // ne*nd*nd element matrices in reference arithmetic
 
-    // filter
+    // filter: the grid rule (pf_filter_setup) or caller-given neighbour
+    // lists (pf_filter_set_lists, CSR-like ptr/idx/w over elements)
     double R = 0.0;
     int win = 0;
     bool have_filter = false;
+    bool have_lists = false;
+    int32_t *d_fptr = nullptr, *d_fidx = nullptr;
+    double* d_fw = nullptr;
 
     // device scalars
     pf::PcgScalars* d_pcg = nullptr;

@@ -628,6 +628,22 @@ __global__ void k_csr_spmv(int N, const int32_t* __restrict__ indptr, const int3
     }
 }
 
+// Partials of ||b - A x||^2 for free-numbered vectors (A assembled).
+__global__ void __launch_bounds__(256) k_csr_residual(int N, const int32_t* __restrict__ indptr,
+                                                      const int32_t* __restrict__ indices,
+                                                      const double* __restrict__ data, const double* __restrict__ b,
+                                                      const double* __restrict__ x, double* __restrict__ part) {
+    __shared__ double lds[32];
+    double acc[1] = {0.0};
+    for (long r = blockIdx.x * (long)blockDim.x + threadIdx.x; r < N; r += (long)gridDim.x * blockDim.x) {
+        double s = 0.0;
+        for (int jj = indptr[r]; jj < indptr[r + 1]; jj++) s += data[jj] * x[indices[jj]];
+        const double d = b[r] - s;
+        acc[0] += d * d;
+    }
+    store_partials<1>(acc, lds, part);
+}
+
 // GetDiagonal via CSR::get (CG.h:397-404, CSR.h:154-167).
 __global__ void k_csr_diag(int N, const int32_t* __restrict__ indptr, const int32_t* __restrict__ indices,
                            const double* __restrict__ data, double* __restrict__ D) {
@@ -784,6 +800,23 @@ __global__ void __launch_bounds__(256) k_filter_q4(int nx, int ny, int win, doub
         out[e] = filter_one(nx, ny, win, R, cg, in, e);
 }
 
+// DensityFilter<T> over caller-given neighbour lists (DensityFilter.h:45-71):
+// out_i = (sum_j w_ij in_{idx_ij}) / (sum_j w_ij), both sums from zero in
+// list order, as GetFilteredVariables / GetFilteredSensitivitis add them.
+__global__ void __launch_bounds__(256) k_filter_lists(long ne, const int32_t* __restrict__ ptr,
+                                                      const int32_t* __restrict__ idx, const double* __restrict__ w,
+                                                      const double* __restrict__ in, double* __restrict__ out) {
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < ne; e += (long)gridDim.x * blockDim.x) {
+        double acc = 0.0, wsum = 0.0;
+        for (int k = ptr[e]; k < ptr[e + 1]; k++) {
+            double wk = w[k];
+            acc = acc + wk * in[idx[k]];
+            wsum = wsum + wk;
+        }
+        out[e] = acc / wsum;
+    }
+}
+
 // g = sum_i scale1 rho_i / (vf ne) - scale1 (sample...oc.cpp:99-105); the
 // per-element terms are summed in a fixed tree order.
 __global__ void __launch_bounds__(256) k_volume_partials(long ne, const double* __restrict__ rho, double vf,
@@ -842,6 +875,19 @@ __global__ void __launch_bounds__(256) k_comp_sens_q4(int nx, int ny, const doub
 //     lambda = (l1+l0)/2 and x+ = clamp(x (-dfdx/(dgdx lambda))^iota);
 //   k_oc_volume(k): filter x+ and sum the closure's volume terms.
 // The bracket of step k lives in slot k of the history arrays.
+// x+ = (-dfdx / (dgdx lambda))^iota x clamped to [max(0, (1-m) x),
+// min(1, (1+m) x)] (OC.h:86-91); iota = 0.5 takes the correctly rounded
+// sqrt, which glibc's pow(., 0.5) also returns.
+__device__ inline double oc_candidate(double xi, double dfdxi, double dgdxi, double lambda, double iota,
+                                      double move) {
+    double base = -dfdxi / (dgdxi * lambda);
+    double v = (iota == 0.5 ? sqrt(base) : pow(base, iota)) * xi;
+    double lo = std_max0((1.0 - move) * xi), hi = std_min1((1.0 + move) * xi);
+    if (v < lo) v = lo;
+    else if (v > hi) v = hi;
+    return v;
+}
+
 struct OcDev {
     double l0i, l1i;  // initial bracket (lambdamin, lambdamax), set by the host
     double l0[2], l1[2], lambda[2];
@@ -885,16 +931,18 @@ __global__ void __launch_bounds__(256) k_oc_xplus(long ne, int k, const double* 
         S->lambda_last = lambda;
         S->steps = k + 1;
     }
-    const bool half = (iota == 0.5);
-    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < ne; i += (long)gridDim.x * blockDim.x) {
-        double xi = x[i];
-        double base = -dfdx[i] / (dgdx[i] * lambda);
-        double v = (half ? sqrt(base) : pow(base, iota)) * xi;
-        double lo = std_max0((1.0 - move) * xi), hi = std_min1((1.0 + move) * xi);
-        if (v < lo) v = lo;
-        else if (v > hi) v = hi;
-        xplus[i] = v;
-    }
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < ne; i += (long)gridDim.x * blockDim.x)
+        xplus[i] = oc_candidate(x[i], dfdx[i], dgdx[i], lambda, iota, move);
+}
+
+// One bisection candidate for a given lambda (OC.h:85-92), for callers
+// that evaluate the constraint themselves.
+__global__ void __launch_bounds__(256) k_oc_candidate(long ne, const double* __restrict__ x,
+                                                      const double* __restrict__ dfdx,
+                                                      const double* __restrict__ dgdx, double lambda, double iota,
+                                                      double move, double* __restrict__ xplus) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < ne; i += (long)gridDim.x * blockDim.x)
+        xplus[i] = oc_candidate(x[i], dfdx[i], dgdx[i], lambda, iota, move);
 }
 
 __global__ void __launch_bounds__(256) k_oc_volume(int nx, int ny, int win, double R, const double* __restrict__ cg,
