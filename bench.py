@@ -80,6 +80,46 @@ def kernels_only(ctx, pa, torch, nx, ny, reps=20, copy_mib=256):
                       "N": ctx.N, "ne": ctx.ne, "kernels_ms": prof}))
 
 
+def learned_vs_jacobi(pa, nx, ny, train_iters=30, collect_every=20, collect_max=16, steps=500, lr=2e-3):
+    """Learned-preconditioner workload: the network is trained on (residual,
+    error) pairs that the plain FP64 Jacobi-PCG produces during the first
+    `train_iters` OC iterations (every `collect_every` chunks of 32 CG
+    iterations, at most `collect_max` per solve), plus 120 synthetic (K v, v)
+    pairs; then both solvers take the system of OC iteration train_iters + 1
+    to a 1e-8 FP64 true relative residual."""
+    ctx = pa.cantilever2d(nx, ny)
+    ctx.precond_init(seed=11)
+    t0 = time.perf_counter()
+    r = ctx.simp_run(train_iters, pa.simp_default_params(collect=collect_every, collect_max=collect_max))
+    t_collect = time.perf_counter() - t0
+    s = r["s"]
+    for k in range(3):
+        rr = ctx.simp_run(1, pa.simp_default_params(), s0=s)
+        s = rr["s"]
+        ctx.set_density(rr["rho"])
+        pairs = ctx.precond_collect_random(40, seed=100 + k)
+    t0 = time.perf_counter()
+    loss0, loss1 = ctx.precond_train(steps, lr=lr, seed=17)
+    t_train = time.perf_counter() - t0
+    rr = ctx.simp_run(1, pa.simp_default_params(), s0=s)
+    ctx.set_density(rr["rho"])
+    b = ctx.load()
+    out = {"design": f"OC iteration {train_iters + 4} of the {nx}x{ny} loop", "pairs": pairs,
+           "train_steps": steps, "loss_first": loss0, "loss_last": loss1, "train_seconds": t_train,
+           "collect_oc_seconds": t_collect}
+    comp = {}
+    for label, op in (("jacobi_fp64", pa.PF_OP_EBE64), ("learned_fcg", pa.PF_OP_LEARNED)):
+        x, st = ctx.pcg(b, op=op, eps=1e-8)
+        comp[label] = float(x @ b)  # u^T K u = u^T f
+        out[label] = {"seconds": st["seconds"], "iterations": st["iterations"], "true_residual": st["true_residual"],
+                      "status": st["status"], "fallbacks": st["fallbacks"]}
+    out["speedup_learned_vs_jacobi"] = out["jacobi_fp64"]["seconds"] / out["learned_fcg"]["seconds"]
+    out["iteration_ratio"] = out["learned_fcg"]["iterations"] / out["jacobi_fp64"]["iterations"]
+    out["compliance_rel_diff"] = abs(comp["learned_fcg"] - comp["jacobi_fp64"]) / abs(comp["jacobi_fp64"])
+    ctx.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -89,6 +129,7 @@ def main():
     ap.add_argument("--ny", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-preconditioner comparison")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only the PCG/filter/OC kernels (pf_profile_kernels) on a seeded density field, then a "
                          "plain device copy of known size: the program the counter runs profile")
@@ -146,6 +187,8 @@ def main():
             extras[label] = {"seconds": st["seconds"], "iterations": st["iterations"],
                              "true_residual": st["true_residual"], "status": st["status"], "outer": st["outer"]}
         extras["speedup_mixed_vs_jacobi"] = extras["jacobi_fp64"]["seconds"] / extras["mixed_fp32_ir"]["seconds"]
+        if not args.no_learned:
+            extras["learned"] = learned_vs_jacobi(pa, nx, ny)
 
     cpu = None if args.no_cpu else cpu_baseline(nx, ny, n_cg, n_bis)
     out = {
