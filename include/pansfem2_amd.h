@@ -236,12 +236,14 @@ int pf_profile_kernels(pf_ctx* ctx, int op, int reps, double* ms);
 
 /* ---- learned preconditioner -------------------------------------------- */
 /* 4-level 3x3 convolutional network on the nodal grid of a quad4 problem
- * (BF16 weights and activations, FP32 accumulation on MFMA) mapping the
- * density and the residual to a correction: M(r) = (r + s y(rho, r/s)) ./ D,
- * s the RMS of r, D the Jacobi diagonal.  Used inside flexible CG
- * (PF_OP_LEARNED) with a Jacobi-PCG fallback when it stalls.  No counterpart
- * in the reference.  pf_precond_init builds it for the current grid with a
- * seeded initialisation (the correction starts near zero, i.e. at Jacobi). */
+ * (BF16 weights and activations, FP32 accumulation on MFMA), input
+ * (2 rho_node - 1, r/s) with s the RMS of r, output a field y.  Inside
+ * flexible CG (PF_OP_LEARNED, Notay's FCG(32)) the preconditioner is
+ * M(r) = a (r ./ D) + c y, D the Jacobi diagonal, with (a, c) minimising the
+ * energy norm of the error over span{r ./ D, y}; Jacobi-PCG takes over from
+ * the current iterate on breakdown, divergence or stagnation.  No
+ * counterpart in the reference.  pf_precond_init builds the network for the
+ * current grid with seeded weights. */
 int pf_precond_init(pf_ctx* ctx, int channels, uint64_t seed);
 /* Number of network parameters (FP32 master copy). */
 int pf_precond_num_params(pf_ctx* ctx, int64_t* count);
@@ -255,7 +257,8 @@ int pf_precond_collect(pf_ctx* ctx, int every, int max_pairs, int* count);
 /* Synthetic pairs (K v, v) for `count` random smooth displacement fields v
  * (seeded multi-scale lattice noise) at the current moduli. */
 int pf_precond_collect_random(pf_ctx* ctx, int count, uint64_t seed, int* total);
-/* Adam on the collected pairs, loss ||M(r) - e||_K^2 / ||e||_K^2. */
+/* Adam on the collected pairs, loss min_{a,c} ||a (r./D) + c y - e||_K^2 / (e.r)
+ * for each pair (K e = r); loss_first/loss_last are the first and last steps'. */
 int pf_precond_train(pf_ctx* ctx, int steps, double lr, uint64_t seed, double* loss_first, double* loss_last);
 /* d = M(r) for a free-DOF residual with the current moduli. */
 int pf_precond_apply(pf_ctx* ctx, const double* r_free, double* d_free);
