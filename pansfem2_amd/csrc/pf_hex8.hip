@@ -26,6 +26,7 @@ __device__ inline int loc_h8(int di, int dj, int dk) { return 4 * dk + (dj == 0 
 template <class T>
 __device__ inline void apply_h8_node(const G3& g, int i, int j, int k, const T (&v)[3][3][3][3],
                                      const T* __restrict__ E, const T* __restrict__ K0, T (&y)[3]) {
+#pragma clang fp contract(fast)  // not a check-path kernel: FMA allowed
     y[0] = y[1] = y[2] = 0;
     for (int ea = 0; ea < 2; ea++) {
         int ei = i - 1 + ea;
@@ -102,6 +103,7 @@ __global__ void __launch_bounds__(256) k_apply_h8(G3 g, const T* __restrict__ x,
                                                   const T* __restrict__ E, const T* __restrict__ K0,
                                                   const uint8_t* __restrict__ fix, int mask_fixed,
                                                   const T* __restrict__ b, double* __restrict__ part) {
+#pragma clang fp contract(fast)  // not a check-path kernel: FMA allowed
     __shared__ double lds[32];
     double acc[1] = {0.0};
     const long nn = g.nnode();
@@ -153,6 +155,7 @@ template <class T>
 __global__ void __launch_bounds__(256) k_pcg_k3(long n, int k, T* __restrict__ p, const T* __restrict__ z,
                                                 const double* __restrict__ partB, int nbB, PcgDev* __restrict__ S,
                                                 double eps) {
+#pragma clang fp contract(fast)  // not a check-path kernel: FMA allowed
     __shared__ double lds[32];
     if (S->done) return;
     double beta;
@@ -169,6 +172,7 @@ __global__ void __launch_bounds__(256) k_pcg_k1_h8(G3 g, const T* __restrict__ p
                                                    const T* __restrict__ E, const T* __restrict__ K0,
                                                    const uint8_t* __restrict__ fix, double* __restrict__ partA,
                                                    const PcgDev* __restrict__ S) {
+#pragma clang fp contract(fast)  // not a check-path kernel: FMA allowed
     __shared__ double lds[32];
     if (S->done) return;
     double acc[1] = {0.0};

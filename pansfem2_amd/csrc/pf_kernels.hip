@@ -1,10 +1,11 @@
 // HIP kernels (gfx950) of the SIMP loop on structured quad4 / hex8 grids.
 //
-// The whole file is compiled without floating-point contraction: every
-// a*b + c rounds twice, as the reference's x86-64 g++ build (no FMA) does,
-// so the check-path kernels (element matrices, CSR values, CSR product,
-// filter, OC step) reproduce the reference's bits.  These kernels are HBM /
-// latency bound; FMA would not make them faster.
+// The file is compiled without floating-point contraction: every a*b + c
+// rounds twice, as the reference's x86-64 g++ build (no FMA) does, so the
+// check-path kernels (element matrices, CSR values, CSR product, diagonal,
+// filter, OC step) reproduce the reference's bits.  The matrix-free operator
+// and the PCG vector updates, which no result is compared with bit for bit,
+// re-enable contraction in their own bodies (half the FP64 instructions).
 #pragma clang fp contract(off)
 
 #include "pf_device.h"
@@ -64,6 +65,7 @@ struct alignas(2 * sizeof(T)) Vec2 { T x, y; };
 template <class T>
 __device__ inline void elem_rows_q4(const T* __restrict__ K0, int loc, const Vec2<T> n0, const Vec2<T> n1,
                                     const Vec2<T> n2, const Vec2<T> n3, T& ox, T& oy) {
+#pragma clang fp contract(fast)  // not a check-path kernel: FMA allowed
     // rows 2*loc, 2*loc+1 of K0 times the element vector (BL, BR, TR, TL)
     const T* r0 = K0 + (2 * loc) * 8;
     const T* r1 = r0 + 8;
@@ -79,6 +81,7 @@ __device__ inline void elem_rows_q4(const T* __restrict__ K0, int loc, const Vec
 template <class T>
 __device__ inline void apply_q4_node(int nx, int ny, int i, int j, const Vec2<T> (&v)[3][3], const T* __restrict__ E,
                                      const T* __restrict__ K0, T& yx, T& yy) {
+#pragma clang fp contract(fast)  // not a check-path kernel: FMA allowed
     T ax = 0, ay = 0, ex, ey;
     if (i >= 1 && j >= 1) {  // element (i-1, j-1): this node is TR
         T Ee = E[(long)ny * (i - 1) + (j - 1)];
@@ -120,6 +123,7 @@ template <class T>
 __global__ void __launch_bounds__(256) k_apply_q4(int nx, int ny, const T* __restrict__ x, T* __restrict__ y,
                                                   const T* __restrict__ E, const T* __restrict__ K0,
                                                   const uint8_t* __restrict__ fix, int mask_fixed) {
+#pragma clang fp contract(fast)  // not a check-path kernel: FMA allowed
     const long nn = (long)(nx + 1) * (ny + 1);
     for (long n = blockIdx.x * (long)blockDim.x + threadIdx.x; n < nn; n += (long)gridDim.x * blockDim.x) {
         int i = (int)(n / (ny + 1)), j = (int)(n - (long)i * (ny + 1));
@@ -227,6 +231,7 @@ __global__ void __launch_bounds__(256) k_pcg_k1_q4(int nx, int ny, int k, const 
                                                    const uint8_t* __restrict__ fix, const double* __restrict__ partB,
                                                    int nbB, double* __restrict__ partA, PcgDev* __restrict__ S,
                                                    double eps) {
+#pragma clang fp contract(fast)  // not a check-path kernel: FMA allowed
     __shared__ double lds[32];
     if (S->done) return;
     double beta;
@@ -326,6 +331,7 @@ __global__ void __launch_bounds__(256) k_pcg_k1_q4_tile(int nx, int ny, int tile
                                                         const double* __restrict__ partB, int nbB,
                                                         double* __restrict__ partA, PcgDev* __restrict__ S,
                                                         double eps) {
+#pragma clang fp contract(fast)  // not a check-path kernel: FMA allowed
     __shared__ double lds[32];
     __shared__ Vec2<T> sp[TI + 2][TJ + 2];
     __shared__ T se[TI + 1][TJ + 1];
@@ -449,6 +455,7 @@ __global__ void __launch_bounds__(256) k_pcg_k2(long n, int k, T* __restrict__ x
                                                 const T* __restrict__ p, const T* __restrict__ q,
                                                 const T* __restrict__ D, const double* __restrict__ partA, int nbA,
                                                 double* __restrict__ partB, PcgDev* __restrict__ S) {
+#pragma clang fp contract(fast)  // not a check-path kernel: FMA allowed
     __shared__ double lds[32];
     // first pair of every thread, the partials and the done flag are all
     // loaded before the first wait
@@ -965,6 +972,7 @@ __global__ void __launch_bounds__(256) k_residual_q4(int nx, int ny, const doubl
                                                      const double* __restrict__ x, double* __restrict__ r,
                                                      const double* __restrict__ E, const double* __restrict__ K0,
                                                      const uint8_t* __restrict__ fix, double* __restrict__ part) {
+#pragma clang fp contract(fast)  // not a check-path kernel: FMA allowed
     __shared__ double lds[32];
     const long nn = (long)(nx + 1) * (ny + 1);
     double acc[1] = {0.0};
