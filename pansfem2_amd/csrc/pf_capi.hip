@@ -384,42 +384,72 @@ double norm_nodal(pf_ctx* ctx, const double* a, int* rc) {
 
 // Mixed precision: FP32 matrix-free Jacobi-PCG inner solves (FP32 vectors
 // and operator, FP64 dot accumulation) inside FP64 iterative refinement:
-// r = b - K x in FP64, solve K d = r in FP32 to inner_eps (relative), x += d
-// in FP64, until ||b - K x|| <= eps ||b||.  Each outer step gains at most
-// about -log10(cond(K) * 6e-8) digits, whatever inner_eps is.
+// r = b - K x in FP64, solve K d = r in FP32 to inner_eps (relative, at most
+// kInnerCap iterations), x += d in FP64, until ||b - K x|| <= eps ||b||.
+// Each outer step gains at most about -log10(cond(K) * 6e-8) digits, whatever
+// inner_eps is.  A step that does not halve the true residual (FP32 cannot
+// resolve the system: moduli contrasts of 1e9 between neighbours) is undone
+// and the refinement continues with FP64 Jacobi-PCG corrections from the
+// current iterate (counted in st.fallbacks).
 int pcg_ir(pf_ctx* ctx, const double* b, double* x, double eps, int itrmax, double inner_eps, pf_solve_stats& st) {
+    constexpr int kInnerCap = 20000;
     PF_TRY(diag_nodal(ctx, true));
     int rc;
     const double bn = norm_nodal(ctx, b, &rc);
     PF_TRY(rc);
     PF_HIP(hipMemsetAsync(x, 0, sizeof(double) * ctx->nv, ctx->stream));
     double rn = bn;
-    int outer = 0, total = 0, status = 2;
+    int outer = 0, total = 0, status = 2, fallbacks = 0;
+    bool fp64 = false;
     const int max_outer = 60;
     double* r64 = ctx->d_t0;
+    double* d64 = ctx->d_t1;
+    PF_TRY(true_residual(ctx, b, x, r64, &rn));
     while (outer < max_outer) {
-        PF_TRY(true_residual(ctx, b, x, r64, &rn));
         if (rn <= eps * bn) {
             status = 1;
             break;
         }
         if (total >= itrmax) break;
-        k_convert<double, float><<<gsize(ctx->nv), kBlock, 0, ctx->stream>>>(ctx->nv, r64, ctx->f_b);
         PcgRun run;
-        PF_TRY(pcg_device<float>(ctx, false, ctx->f_b, ctx->f_x, ctx->f_D, ctx->f_r, ctx->f_z, ctx->f_p0, ctx->f_p1,
-                                 ctx->f_q, inner_eps, itrmax - total, run));
-        if (!std::isfinite(run.rnorm)) {
-            status = 3;
-            break;
+        if (!fp64) {
+            k_convert<double, float><<<gsize(ctx->nv), kBlock, 0, ctx->stream>>>(ctx->nv, r64, ctx->f_b);
+            PF_TRY(pcg_device<float>(ctx, false, ctx->f_b, ctx->f_x, ctx->f_D, ctx->f_r, ctx->f_z, ctx->f_p0,
+                                     ctx->f_p1, ctx->f_q, inner_eps, std::min(kInnerCap, itrmax - total), run));
+            total += run.iterations;
+            outer++;
+            PF_HIP(hipMemcpyAsync(d64, x, sizeof(double) * ctx->nv, hipMemcpyDeviceToDevice, ctx->stream));
+            k_axpy_f32<<<gsize(ctx->nv), kBlock, 0, ctx->stream>>>(ctx->nv, ctx->f_x, x);
+            PF_TRY(check_launch(ctx));
+            double rnew;
+            PF_TRY(true_residual(ctx, b, x, r64, &rnew));
+            if (std::isfinite(run.rnorm) && std::isfinite(rnew) && rnew <= 0.5 * rn) {
+                rn = rnew;
+                continue;
+            }
+            // undo the step: x and its true residual as they were
+            PF_HIP(hipMemcpyAsync(x, d64, sizeof(double) * ctx->nv, hipMemcpyDeviceToDevice, ctx->stream));
+            PF_TRY(true_residual(ctx, b, x, r64, &rn));
+            fp64 = true;
+            fallbacks++;
+            continue;
         }
+        // FP64 correction: K d = r to the relative tolerance that brings the
+        // true residual to eps ||b||
+        PF_TRY(diag_nodal(ctx, false));
+        const double rel = std::min(0.5, eps * bn / rn);
+        PF_TRY(pcg_device<double>(ctx, false, r64, d64, ctx->d_D, ctx->d_r, ctx->d_z, ctx->d_p0, ctx->d_p1,
+                                  ctx->d_q, rel, itrmax - total, run));
         total += run.iterations;
-        k_axpy_f32<<<gsize(ctx->nv), kBlock, 0, ctx->stream>>>(ctx->nv, ctx->f_x, x);
-        PF_TRY(check_launch(ctx));
         outer++;
+        k_axpy_d<<<gsize(ctx->nv), kBlock, 0, ctx->stream>>>(ctx->nv, 1.0, d64, x);
+        PF_TRY(check_launch(ctx));
+        PF_TRY(true_residual(ctx, b, x, r64, &rn));
     }
     st.iterations = total;
     st.status = status;
     st.outer = outer;
+    st.fallbacks = fallbacks;
     st.rel_residual = bn > 0 ? rn / bn : 0.0;
     return PF_OK;
 }

@@ -1,0 +1,111 @@
+"""GPU checks at the workloads' full sizes, through size-independent
+properties (the oracle does not finish in seconds here).
+
+2048 x 1024 quad4 (4.2M DOFs): the mixed-precision solve (FP32 matrix-free
+inner PCG to 1e-5 inside FP64 iterative refinement) on a density field
+drawn i.i.d. uniform in [1e-3, 1] and on a SIMP-like field (density 1
+inside seeded random ellipses, 1e-3 elsewhere, smoothed once by the R = 1.5
+filter): the FP64 solve reaches a 1e-8 recursive residual (ScalingCG's
+criterion), the mixed one a 1e-8 FP64 true residual, and the two agree to
+what those residuals allow.  (With SIMP moduli E = rho^3 the i.i.d.
+field puts contrasts up to 1e9 between neighbouring elements, beyond what
+FP32 CG resolves: there the refinement falls back to FP64 corrections.)
+
+128 x 64 x 64 hex8 (1.6M DOFs): the matrix-free operator against the
+assembled one, its symmetry, and the FP64 solve's true residual.
+"""
+import numpy as np
+import pytest
+
+pytestmark = [pytest.mark.gpu, pytest.mark.slow]
+
+
+@pytest.fixture(scope="module")
+def pa():
+    import pansfem2_amd
+    return pansfem2_amd
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))
+
+
+def ellipse_field(nx, ny, count=60, seed=20261203, low=1e-3):
+    """Density 1 inside `count` ellipses (centres uniform over the domain,
+    semi-axes uniform in [0.02, 0.08] of the length, angles uniform; element
+    centroids decide membership), `low` elsewhere.  Solid pieces not
+    connected to the clamped edge are left at `low`, as a SIMP design holds
+    no floating material (floating stiff islands in a 1e-9 void have nearly
+    rigid modes that Jacobi-PCG does not resolve in 1e5 iterations)."""
+    from scipy import ndimage
+    rng = np.random.default_rng(seed)
+    ci = (np.arange(nx) + 0.5)[:, None] * np.ones((1, ny))
+    cj = np.ones((nx, 1)) * (np.arange(ny) + 0.5)[None, :]
+    solid = np.zeros((nx, ny), bool)
+    for _ in range(count):
+        x0, y0 = rng.uniform(0, nx), rng.uniform(0, ny)
+        a, b = rng.uniform(0.02, 0.08, 2) * nx
+        th = rng.uniform(0, np.pi)
+        dx, dy = ci - x0, cj - y0
+        u = (dx * np.cos(th) + dy * np.sin(th)) / a
+        v = (-dx * np.sin(th) + dy * np.cos(th)) / b
+        solid |= u * u + v * v <= 1.0
+    lab, _ = ndimage.label(solid)
+    keep = np.isin(lab, np.unique(lab[0, :][lab[0, :] > 0]))
+    return np.where(keep, 1.0, low).reshape(-1)
+
+
+@pytest.mark.parametrize("field", ["iid", "ellipses"])
+def test_c3_mixed_precision_2048x1024(pa, field):
+    nx, ny = 2048, 1024
+    print(f"c3 mixed precision, field {field}", flush=True)
+    ctx = pa.cantilever2d(nx, ny)
+    if field == "iid":
+        rho = np.random.default_rng(20261204).uniform(1e-3, 1.0, nx * ny)
+    else:
+        solid = ellipse_field(nx, ny)
+        print(f"  solid fraction {np.mean(solid == 1.0):.2f}", flush=True)
+        rho = ctx.filter(solid)
+    ctx.set_density(rho)
+    b = ctx.load()
+    x64, s64 = ctx.pcg(b, op=pa.PF_OP_EBE64, eps=1e-8)
+    x32, s32 = ctx.pcg(b, op=pa.PF_OP_EBE32, eps=1e-8)
+    print(f"  fp64 {s64['iterations']} it {s64['seconds']:.2f} s; mixed {s32['iterations']} it "
+          f"{s32['outer']} outer {s32['fallbacks']} fallbacks {s32['seconds']:.2f} s; "
+          f"|u32-u64|/|u64| {rel(x32, x64):.2e}", flush=True)
+    print(f"  true residuals: fp64 {s64['true_residual']:.2e}, mixed {s32['true_residual']:.2e}", flush=True)
+    # Jacobi-PCG stops on its recursive residual, as ScalingCG does; after
+    # 10^4 iterations on these fields its true residual can sit above it
+    # (1.6e-7 on the ellipse field)
+    assert s64["status"] == 1 and s64["rel_residual"] < 1e-8
+    # the refinement stops on the FP64 true residual itself
+    assert s32["status"] == 1 and s32["true_residual"] <= 1.0000001e-8
+    # the two solutions agree to what those residuals allow
+    assert rel(x32, x64) < 1e-4
+    # compliance f = u.b agrees far more tightly than the displacement
+    assert abs(x32 @ b - x64 @ b) / abs(x64 @ b) < 1e-7
+    ctx.close()
+
+
+def test_c5_hex8_128x64x64_operator_and_solve(pa):
+    nx, ny, nz = 128, 64, 64
+    print("c5 hex8 operator and solve", flush=True)
+    ctx = pa.cantilever3d(nx, ny, nz)
+    rng = np.random.default_rng(20261205)
+    ctx.set_density(rng.uniform(1e-3, 1.0, nx * ny * nz))
+    ctx.assemble_csr()
+    x = rng.uniform(-1, 1, ctx.N)
+    y = rng.uniform(-1, 1, ctx.N)
+    Kx_ebe = ctx.apply_K(x, pa.PF_OP_EBE64)
+    Kx_csr = ctx.apply_K(x, pa.PF_OP_CSR64)
+    assert rel(Kx_ebe, Kx_csr) < 1e-13
+    Ky = ctx.apply_K(y, pa.PF_OP_EBE64)
+    assert abs(x @ Ky - y @ Kx_ebe) / abs(x @ Ky) < 1e-12
+    ctx.set_density(np.full(nx * ny * nz, 0.3))
+    b = ctx.load()
+    u, st = ctx.pcg(b, op=pa.PF_OP_EBE64, eps=1e-8)
+    print(f"  fp64 {st['iterations']} it {st['seconds']:.2f} s, true residual {st['true_residual']:.2e}", flush=True)
+    assert st["status"] == 1 and st["rel_residual"] < 1e-8 and st["true_residual"] < 1e-7
+    # downward line load: the tip moves down, compliance positive
+    assert u @ b > 0
+    ctx.close()
