@@ -15,6 +15,7 @@
 //   SquareMesh<T>                   src/PrePost/Mesher/SquareMesh.h:19-207
 //   LILCSR<T>, CSR<T>, ScalingCG    src/LinearAlgebra/Models/{LILCSR,CSR}.h, Solvers/CG.h:419-453
 //   DensityFilter<T>                src/Optimize/Filter/DensityFilter.h:15-71
+//   HeavisideFilter<T>              src/Optimize/Filter/HeavisideFilter.h:15-99
 //   OC<T>                           src/Optimize/Solver/OC.h:15-107
 //   MMA<T>                          src/Optimize/Solver/MMA.h:15-509
 //
@@ -88,6 +89,7 @@ public:
         dim_ = 2, nx_ = nx, ny_ = ny, nz_ = 0, lx_ = lx, ly_ = ly, lz_ = 0.0;
         ++generation_;
         fixed_.clear();
+        have_fixed_ = false;  // a new grid has no DOF map on the device yet
         filter_owner_ = nullptr;
     }
     void grid3d(int nx, int ny, int nz, double lx, double ly, double lz) {
@@ -96,6 +98,7 @@ public:
         dim_ = 3, nx_ = nx, ny_ = ny, nz_ = nz, lx_ = lx, ly_ = ly, lz_ = lz;
         ++generation_;
         fixed_.clear();
+        have_fixed_ = false;
         filter_owner_ = nullptr;
     }
     int dim() const { return dim_; }
@@ -150,7 +153,7 @@ public:
         return nfree_;
     }
 
-    // which DensityFilter's lists are on the device
+    // which filter's neighbour lists are on the device
     const void* filter_owner_ = nullptr;
     long filter_generation_ = -1;
 
@@ -938,22 +941,48 @@ void Disassembling(std::vector<Vector<T> >& u, const std::vector<T>& result, con
 }
 
 // =========================================================== filter ======
-// rho_i = sum_j w_ij s_j / sum_j w_ij over the given neighbour lists, on the
-// device; GetFilteredSensitivitis is the same row-normalised sum.
+namespace detail {
+// Neighbour lists of a filter, uploaded to the device when a filter call
+// finds another filter's lists (or another grid) there.
 template <class T>
-class DensityFilter {
+class FilterLists {
 public:
-    DensityFilter(int n, std::vector<std::vector<int> > neighbors, std::vector<std::vector<T> > w) : n_(n) {
+    FilterLists() = default;
+    FilterLists(int n, const std::vector<std::vector<int> >& neighbors, const std::vector<std::vector<T> >& w) : n_(n) {
         static_assert(std::is_same<T, double>::value, "the device path computes in double");
-        if ((int)neighbors.size() != n || (int)w.size() != n) throw DeviceError("DensityFilter: list sizes differ from n");
+        if ((int)neighbors.size() != n || (int)w.size() != n) throw DeviceError("filter: list sizes differ from n");
         ptr_.assign(1, 0);
         for (int i = 0; i < n; i++) {
-            if (neighbors[i].size() != w[i].size()) throw DeviceError("DensityFilter: neighbour and weight lists differ");
+            if (neighbors[i].size() != w[i].size()) throw DeviceError("filter: neighbour and weight lists differ");
             idx_.insert(idx_.end(), neighbors[i].begin(), neighbors[i].end());
             w_.insert(w_.end(), w[i].begin(), w[i].end());
             ptr_.push_back((int32_t)idx_.size());
         }
     }
+    void ensure() {
+        auto& dev = Device::Get();
+        if (n_ != dev.elements()) throw DeviceError("filter: n differs from the registered grid's elements");
+        if (dev.filter_owner_ != this || dev.filter_generation_ != dev.generation()) {
+            dev.check(pf_filter_set_lists(dev.ctx(), ptr_.data(), idx_.data(), w_.data()), "pf_filter_set_lists");
+            dev.filter_owner_ = this;
+            dev.filter_generation_ = dev.generation();
+        }
+    }
+    int n_ = 0;
+
+private:
+    std::vector<int32_t> ptr_, idx_;
+    std::vector<T> w_;
+};
+}  // namespace detail
+
+// rho_i = sum_j w_ij s_j / sum_j w_ij over the given neighbour lists, on the
+// device; GetFilteredSensitivitis is the same row-normalised sum.
+template <class T>
+class DensityFilter {
+public:
+    DensityFilter(int n, std::vector<std::vector<int> > neighbors, std::vector<std::vector<T> > w)
+        : lists_(n, neighbors, w) {}
     std::vector<T> GetFilteredVariables(std::vector<T> s) { return run(PF_FILTER_VARIABLES, s); }
     std::vector<T> GetFilteredSensitivitis(std::vector<T> /*s*/, std::vector<T> dfdrho) {
         return run(PF_FILTER_SENSITIVITIES, dfdrho);
@@ -961,20 +990,39 @@ public:
 
 private:
     std::vector<T> run(int mode, const std::vector<T>& in) {
+        lists_.ensure();
         auto& dev = Device::Get();
-        if (n_ != dev.elements()) throw DeviceError("DensityFilter: n differs from the registered grid's elements");
-        if (dev.filter_owner_ != this || dev.filter_generation_ != dev.generation()) {
-            dev.check(pf_filter_set_lists(dev.ctx(), ptr_.data(), idx_.data(), w_.data()), "pf_filter_set_lists");
-            dev.filter_owner_ = this;
-            dev.filter_generation_ = dev.generation();
-        }
-        std::vector<T> out(n_);
+        std::vector<T> out(lists_.n_);
         dev.check(pf_filter(dev.ctx(), mode, in.data(), out.data()), "pf_filter");
         return out;
     }
-    int n_;
-    std::vector<int32_t> ptr_, idx_;
-    std::vector<T> w_;
+    detail::FilterLists<T> lists_;
+};
+
+// The density filter followed by the tanh projection
+// rho = (tanh(beta/2) + tanh(beta (s~ - 1/2))) / (2 tanh(beta/2)), with the
+// chain-rule sensitivities; beta is set by UpdateBeta (1 initially).
+template <class T>
+class HeavisideFilter {
+public:
+    HeavisideFilter() = default;
+    HeavisideFilter(int n, std::vector<std::vector<int> > neighbors, std::vector<std::vector<T> > w)
+        : lists_(n, neighbors, w) {}
+    void UpdateBeta(T beta) { beta_ = beta; }
+    std::vector<T> GetFilteredVariables(std::vector<T> s) { return run(0, s, nullptr); }
+    std::vector<T> GetFilteredSensitivitis(std::vector<T> s, std::vector<T> dfdrho) { return run(1, s, &dfdrho); }
+
+private:
+    std::vector<T> run(int mode, const std::vector<T>& s, const std::vector<T>* dfdrho) {
+        lists_.ensure();
+        auto& dev = Device::Get();
+        std::vector<T> out(lists_.n_);
+        dev.check(pf_heaviside(dev.ctx(), beta_, mode, s.data(), dfdrho ? dfdrho->data() : nullptr, out.data()),
+                  "pf_heaviside");
+        return out;
+    }
+    detail::FilterLists<T> lists_;
+    T beta_ = 1.0;
 };
 
 // ============================================================== OC =======

@@ -61,6 +61,8 @@ typedef struct pf_oc_params {
     double iota, lambdamin, lambdamax, lambdaeps, movelimit; /* OC<T> ctor (OC.h:45-60) */
     double vf, scale1;     /* volume closure of sample_optimize_density_oc.cpp:199-206 */
     int max_steps;         /* safety cap on bisection steps (the reference has none) */
+    double beta;           /* > 0: the closure filters with HeavisideFilter at this beta
+                              (src/Optimize/Filter/HeavisideFilter.h:62-75) */
 } pf_oc_params;
 
 typedef struct pf_mma_params {
@@ -86,6 +88,11 @@ typedef struct pf_simp_params {
     int collect;                   /* > 0: keep a training pair every `collect` CG chunks (32 iterations)
                                       of each solve (needs pf_precond_init) */
     int collect_max;               /* cap on pairs taken from one solve */
+    int heaviside;                 /* 1: HeavisideFilter (HeavisideFilter.h:15-99) instead of DensityFilter */
+    double beta0, beta_factor;     /* beta continuation of the density samples: beta0 multiplied by */
+    int beta_every;                /* beta_factor at every beta_every-th iteration from 0
+                                      (sample_optimize_density_oc.cpp:78,89-93: 0.5, 2, 40) */
+    int iter_offset;               /* index of this call's first iteration in the continuation */
 } pf_simp_params;
 
 typedef struct pf_simp_history {
@@ -185,6 +192,11 @@ int pf_filter_setup(pf_ctx* ctx, double R);
 /* mode 0: GetFilteredVariables (DensityFilter.h:45-56);
  * mode 1: GetFilteredSensitivitis (DensityFilter.h:60-71). ne in, ne out. */
 int pf_filter(pf_ctx* ctx, int mode, const double* in, double* out);
+/* HeavisideFilter<T> at `beta` over the current filter (grid rule or lists):
+ * mode 0: GetFilteredVariables (HeavisideFilter.h:62-75), out = H(filter(s));
+ * mode 1: GetFilteredSensitivitis (HeavisideFilter.h:78-99), out_i =
+ * sum_j dfdrho_j H'(s~_j) w_ij / sum_j w_ij with s~ = filter(s). ne in, ne out. */
+int pf_heaviside(pf_ctx* ctx, double beta, int mode, const double* s, const double* dfdrho, double* out);
 /* DensityFilter<T>(n, neighbors, w) with caller-given lists
  * (DensityFilter.h:34-43): element e's neighbours are idx[ptr[e]..ptr[e+1])
  * with weights w (same order); pf_filter then uses the lists (sums in list
@@ -221,7 +233,7 @@ int pf_simp_run(pf_ctx* ctx, const pf_simp_params* prm, int iters, double* s, do
 /* Fill prm with the defaults of the SIMP workloads (E1 1, E0 1e-9, nu 0.3,
  * t 1, p 3, vf 0.5, R 1.5, scales 1, ScalingCG(100000, 1e-10), OC(0.5, 0,
  * 1e9, 1e-3, 0.2), MMA(a0 1, a 0, c 1e4, d 0, [0.01,1], 1e-5, 0.1, 0.2, 0.5,
- * 0.7, 1.2, 1e-6)). */
+ * 0.7, 1.2, 1e-6), density filter; Heaviside continuation 0.5, 2, 40). */
 void pf_simp_default_params(pf_simp_params* prm);
 
 /* ---- measurement -------------------------------------------------------- */

@@ -433,10 +433,60 @@ void orc_filter_sens(int ne, const int* ptr, const int* idx, const double* w, co
 static double std_max(double a, double b) { return (a < b) ? b : a; } /* std::max */
 static double std_min(double a, double b) { return (b < a) ? b : a; } /* std::min */
 
+/* HeavisideFilter<T>::GetFilteredVariables (src/Optimize/Filter/HeavisideFilter.h:62-75):
+ * rho_i = 0.5 (tanh(beta/2) + tanh(beta (s~_i - 1/2))) / tanh(beta/2) with
+ * s~_i = (sum_j w_ij s_j) / (sum_j w_ij). */
+void orc_heaviside_vars(int ne, const int* ptr, const int* idx, const double* w, double beta, const double* s,
+                        double* rho) {
+    for (int i = 0; i < ne; i++) {
+        double wssum = 0.0, wsum = 0.0;
+        for (int k = ptr[i]; k < ptr[i + 1]; k++) {
+            wssum += w[k] * s[idx[k]];
+            wsum += w[k];
+        }
+        rho[i] = 0.5 * (tanh(0.5 * beta) + tanh(beta * (wssum / wsum - 0.5))) / tanh(0.5 * beta);
+    }
+}
+
+/* HeavisideFilter<T>::GetFilteredSensitivitis (HeavisideFilter.h:78-99): the
+ * projection's derivative at every element, then the row-normalised sum of
+ * dfdrho_j drho_j/ds~_j w_ij; pow(t, 2.0) is t*t as the reference's build
+ * folds it. */
+void orc_heaviside_sens(int ne, const int* ptr, const int* idx, const double* w, double beta, const double* s,
+                        const double* dfdrho, double* dfds) {
+    double* dr = (double*)malloc(sizeof(double) * (size_t)(ne > 0 ? ne : 1));
+    for (int i = 0; i < ne; i++) {
+        double wssum = 0.0, wsum = 0.0;
+        for (int k = ptr[i]; k < ptr[i + 1]; k++) {
+            wssum += w[k] * s[idx[k]];
+            wsum += w[k];
+        }
+        double t = tanh(beta * (wssum / wsum - 0.5));
+        dr[i] = 0.5 * beta * (1.0 - t * t) / tanh(0.5 * beta);
+    }
+    for (int i = 0; i < ne; i++) {
+        double r = 0.0, wsum = 0.0;
+        for (int k = ptr[i]; k < ptr[i + 1]; k++) {
+            r += dfdrho[idx[k]] * dr[idx[k]] * w[k];
+            wsum += w[k];
+        }
+        dfds[i] = r / wsum;
+    }
+    free(dr);
+}
+
 int orc_oc_update(int n, double* x, const double* dfdx, const double* dgdx, double iota, double lmin, double lmax,
                   double leps, double move, const int* fptr, const int* fidx, const double* fw, double vf,
                   double scale1, double* lambda_out, int* decisions, int max_dec) {
-    /* OC.h:76-107; the closure is sample_optimize_density_oc.cpp:199-206 */
+    return orc_oc_update_beta(n, x, dfdx, dgdx, iota, lmin, lmax, leps, move, fptr, fidx, fw, vf, scale1, 0.0,
+                              lambda_out, decisions, max_dec);
+}
+
+int orc_oc_update_beta(int n, double* x, const double* dfdx, const double* dgdx, double iota, double lmin,
+                       double lmax, double leps, double move, const int* fptr, const int* fidx, const double* fw,
+                       double vf, double scale1, double beta, double* lambda_out, int* decisions, int max_dec) {
+    /* OC.h:76-107; the closure is sample_optimize_density_oc.cpp:199-206,
+     * filtering with HeavisideFilter at beta when beta > 0 */
     double l0 = lmin, l1 = lmax, lambda = 0.0;
     double* xk1 = (double*)calloc((size_t)n, sizeof(double));
     double* rr = (double*)malloc(sizeof(double) * (size_t)n);
@@ -448,7 +498,8 @@ int orc_oc_update(int n, double* x, const double* dfdx, const double* dgdx, doub
             if (xk1[i] < std_max(0.0, (1.0 - move) * x[i])) xk1[i] = std_max(0.0, (1.0 - move) * x[i]);
             else if (xk1[i] > std_min(1.0, (1.0 + move) * x[i])) xk1[i] = std_min(1.0, (1.0 + move) * x[i]);
         }
-        orc_filter_apply(n, fptr, fidx, fw, xk1, rr);
+        if (beta > 0.0) orc_heaviside_vars(n, fptr, fidx, fw, beta, xk1, rr);
+        else orc_filter_apply(n, fptr, fidx, fw, xk1, rr);
         double g = 0.0;
         for (int i = 0; i < n; i++) g += scale1 * rr[i] / (vf * n);
         g = g - 1.0 * scale1;
@@ -799,6 +850,7 @@ void orc_simp_default_params(orc_simp_params* p) {
     p->oc_iota = 0.5; p->oc_lmin = 0.0; p->oc_lmax = 1.0e9; p->oc_leps = 1.0e-3; p->oc_move = 0.2;
     p->mma_a0 = 1.0; p->mma_c = 1.0e4; p->mma_xmin = 0.01; p->mma_xmax = 1.0; p->mma_raa0 = 1.0e-5; p->mma_albefa = 0.1;
     p->mma_move = 0.2; p->mma_asyinit = 0.5; p->mma_asydecr = 0.7; p->mma_asyincr = 1.2; p->mma_eps = 1.0e-6;
+    p->heaviside = 0; p->beta0 = 0.5; p->beta_factor = 2.0; p->beta_every = 40;
 }
 
 int orc_simp2d(int nx, int ny, int iters, const orc_simp_params* P, double* s, double* rho_out, double* f_hist,
@@ -848,8 +900,12 @@ int orc_simp2d(int nx, int ny, int iters, const orc_simp_params* P, double* s, d
         orc_mma_set_parameters(mma, P->mma_raa0, P->mma_albefa, P->mma_move, P->mma_asyinit, P->mma_asydecr, P->mma_asyincr, P->mma_eps);
         free(xmin); free(xmax);
     }
+    double beta = P->beta0;
     for (int k = 0; k < iters; k++) {
-        orc_filter_apply(ne, fptr, fidx, fw, s, rho);
+        /* beta continuation of the density samples (sample_optimize_density_oc.cpp:89-93) */
+        if (P->heaviside && k % P->beta_every == 0) beta *= P->beta_factor;
+        if (P->heaviside) orc_heaviside_vars(ne, fptr, fidx, fw, beta, s, rho);
+        else orc_filter_apply(ne, fptr, fidx, fw, s, rho);
         double g = 0.0;
         for (int i = 0; i < ne; i++) {
             g += P->scale1 * rho[i] / (P->vf * ne);
@@ -898,20 +954,28 @@ int orc_simp2d(int nx, int ny, int iters, const orc_simp_params* P, double* s, d
             for (int a = 0; a < 8; a++) uku += ue[a] * ke_ue[a];
             dfdrho[e] = -P->scale0 * P->p * (-P->E0 + P->E1) * pow_folded(rho[e], P->p - 1.0) * uku;
         }
-        orc_filter_sens(ne, fptr, fidx, fw, dfdrho, dfds);
-        orc_filter_sens(ne, fptr, fidx, fw, dgdrho, dgds);
+        if (P->heaviside) {
+            orc_heaviside_sens(ne, fptr, fidx, fw, beta, s, dfdrho, dfds);
+            orc_heaviside_sens(ne, fptr, fidx, fw, beta, s, dgdrho, dgds);
+        } else {
+            orc_filter_sens(ne, fptr, fidx, fw, dfdrho, dfds);
+            orc_filter_sens(ne, fptr, fidx, fw, dgdrho, dgds);
+        }
         if (f_hist) f_hist[k] = f;
         if (g_hist) g_hist[k] = g;
         double lam = 0.0;
         if (!P->use_mma) {
-            orc_oc_update(ne, s, dfds, dgds, P->oc_iota, P->oc_lmin, P->oc_lmax, P->oc_leps, P->oc_move, fptr, fidx, fw,
-                          P->vf, P->scale1, &lam, NULL, 0);
+            orc_oc_update_beta(ne, s, dfds, dgds, P->oc_iota, P->oc_lmin, P->oc_lmax, P->oc_leps, P->oc_move, fptr, fidx,
+                               fw, P->vf, P->scale1, P->heaviside ? beta : 0.0, &lam, NULL, 0);
         } else {
             orc_mma_update(mma, s, f, dfds, &g, dgds);
         }
         if (lambda_hist) lambda_hist[k] = lam;
     }
-    if (rho_out) orc_filter_apply(ne, fptr, fidx, fw, s, rho_out);
+    if (rho_out) {
+        if (P->heaviside) orc_heaviside_vars(ne, fptr, fidx, fw, beta, s, rho_out);
+        else orc_filter_apply(ne, fptr, fidx, fw, s, rho_out);
+    }
     orc_mma_destroy(mma);
     free(xy); free(conn); free(fn); free(fd); free(n2g); free(cg); free(fptr); free(fidx); free(fw);
     free(rho); free(dgdrho); free(dfdrho); free(dfds); free(dgds); free(ke_all); free(F); free(ufix); free(uf); free(u);

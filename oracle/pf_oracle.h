@@ -82,6 +82,19 @@ int orc_oc_update(int n, double* x, const double* dfdx, const double* dgdx, doub
                   double lmax, double leps, double move, const int* fptr, const int* fidx, const double* fw,
                   double vf, double scale1, double* lambda_out, int* decisions, int max_dec);
 
+/* HeavisideFilter<T> (src/Optimize/Filter/HeavisideFilter.h:62-99) over the
+ * same neighbour lists: projected filtered variables and filtered
+ * sensitivities at a given beta. */
+void orc_heaviside_vars(int ne, const int* ptr, const int* idx, const double* w, double beta, const double* s,
+                        double* rho);
+void orc_heaviside_sens(int ne, const int* ptr, const int* idx, const double* w, double beta, const double* s,
+                        const double* dfdrho, double* dfds);
+/* orc_oc_update with the closure filtering through HeavisideFilter at beta
+ * (beta <= 0: the density filter). */
+int orc_oc_update_beta(int n, double* x, const double* dfdx, const double* dgdx, double iota, double lmin,
+                       double lmax, double leps, double move, const int* fptr, const int* fidx, const double* fw,
+                       double vf, double scale1, double beta, double* lambda_out, int* decisions, int max_dec);
+
 /* MMA<T> state (src/Optimize/Solver/MMA.h:15-100). */
 typedef struct orc_mma {
     int n, m, k;
@@ -113,6 +126,9 @@ typedef struct orc_simp_params {
     int use_mma;
     double oc_iota, oc_lmin, oc_lmax, oc_leps, oc_move;
     double mma_a0, mma_c, mma_xmin, mma_xmax, mma_raa0, mma_albefa, mma_move, mma_asyinit, mma_asydecr, mma_asyincr, mma_eps;
+    /* HeavisideFilter instead of DensityFilter, with beta multiplied by
+     * beta_factor at every beta_every-th iteration from k = 0 */
+    int heaviside; double beta0, beta_factor; int beta_every;
 } orc_simp_params;
 void orc_simp_default_params(orc_simp_params* p);
 int orc_simp2d(int nx, int ny, int iters, const orc_simp_params* prm, double* s, double* rho_out,

@@ -53,6 +53,8 @@ def lib():
         L.orc_filter_build_brute.restype = c_long
         L.orc_filter_apply.argtypes = [c_int, _i32p, _i32p, _f64p, _f64p, _f64p]
         L.orc_filter_sens.argtypes = [c_int, _i32p, _i32p, _f64p, _f64p, _f64p]
+        L.orc_heaviside_vars.argtypes = [c_int, _i32p, _i32p, _f64p, c_double, _f64p, _f64p]
+        L.orc_heaviside_sens.argtypes = [c_int, _i32p, _i32p, _f64p, c_double, _f64p, _f64p, _f64p]
         L.orc_oc_update.argtypes = [c_int, _f64p, _f64p, _f64p, c_double, c_double, c_double, c_double, c_double,
                                     _i32p, _i32p, _f64p, c_double, c_double, ctypes.POINTER(c_double), _i32p, c_int]
         L.orc_oc_update.restype = c_int
@@ -257,6 +259,21 @@ def filter_sens(ptr, idx, w, dfdrho):
     return out
 
 
+def heaviside_vars(ptr, idx, w, beta, s):
+    """HeavisideFilter::GetFilteredVariables (HeavisideFilter.h:62-75)."""
+    out = np.zeros(len(ptr) - 1)
+    lib().orc_heaviside_vars(len(ptr) - 1, ptr, idx, w, beta, np.ascontiguousarray(s, np.float64), out)
+    return out
+
+
+def heaviside_sens(ptr, idx, w, beta, s, dfdrho):
+    """HeavisideFilter::GetFilteredSensitivitis (HeavisideFilter.h:78-99)."""
+    out = np.zeros(len(ptr) - 1)
+    lib().orc_heaviside_sens(len(ptr) - 1, ptr, idx, w, beta, np.ascontiguousarray(s, np.float64),
+                             np.ascontiguousarray(dfdrho, np.float64), out)
+    return out
+
+
 def oc_update(x, dfdx, dgdx, fptr, fidx, fw, iota=0.5, lmin=0.0, lmax=1e9, leps=1e-3, move=0.2, vf=0.5, scale1=1.0):
     x = np.array(x, np.float64)
     lam = ctypes.c_double(0.0)
@@ -297,7 +314,9 @@ class SimpParams(ctypes.Structure):
                [("cg_itrmax", ctypes.c_int), ("use_mma", ctypes.c_int)] + \
                [(n, ctypes.c_double) for n in ("oc_iota", "oc_lmin", "oc_lmax", "oc_leps", "oc_move", "mma_a0", "mma_c",
                                                "mma_xmin", "mma_xmax", "mma_raa0", "mma_albefa", "mma_move",
-                                               "mma_asyinit", "mma_asydecr", "mma_asyincr", "mma_eps")]
+                                               "mma_asyinit", "mma_asydecr", "mma_asyincr", "mma_eps")] + \
+               [("heaviside", ctypes.c_int), ("beta0", ctypes.c_double), ("beta_factor", ctypes.c_double),
+                ("beta_every", ctypes.c_int)]
 
 
 def simp_params(**kw):

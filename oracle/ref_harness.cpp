@@ -10,10 +10,13 @@
 // tests/golden/make_golden.py packs into fixtures.
 //
 // Modes
-//   simp <nx> <ny> <iters> <oc|mma> <outdir> [nsave]
+//   simp <nx> <ny> <iters> <oc|mma|ocH|mmaH> <outdir> [nsave]
 //       plane-strain SIMP compliance loop (DensityFilter R=1.5, E1=1,
 //       E0=1e-9, nu=0.3, t=1, p=3, vf=0.5, s0=0.5, left edge clamped, unit
 //       downward load at the bottom-right node, ScalingCG(100000, 1e-10)).
+//       The H variants use HeavisideFilter with the density samples' beta
+//       continuation (beta = 0.5, doubled at every 40th iteration from k = 0,
+//       sample_optimize_density_oc.cpp:78,89-93).
 //   operator <nx> <ny> <seed> <outdir>
 //       assembled K(rho) for rho iid U[1e-3,1] (SplitMix64), y = K x for x iid
 //       U[-1,1], and the ScalingCG solve of K u = F to 1e-10.
@@ -55,6 +58,7 @@
 #include "Optimize/Solver/OC.h"
 #include "Optimize/Solver/MMA.h"
 #include "Optimize/Filter/DensityFilter.h"
+#include "Optimize/Filter/HeavisideFilter.h"
 #include "PrePost/Mesher/SquareMesh.h"
 #include "PrePost/Import/ImportFromCSV.h"
 #include "FEM/Equation/General.h"
@@ -182,7 +186,17 @@ std::vector<double> flat_nodes(const std::vector<Vector<double> >& v, int d) {
 }
 
 // ---------------------------------------------------------------- simp ----
-int run_simp(int nx, int ny, int iters, bool use_mma, const std::string& out, int nsave) {
+// The filter of the loop: DensityFilter, or HeavisideFilter with beta.
+struct LoopFilter {
+    DensityFilter<double>* d = nullptr;
+    HeavisideFilter<double>* h = nullptr;
+    std::vector<double> vars(const std::vector<double>& s) { return h ? h->GetFilteredVariables(s) : d->GetFilteredVariables(s); }
+    std::vector<double> sens(const std::vector<double>& s, const std::vector<double>& df) {
+        return h ? h->GetFilteredSensitivitis(s, df) : d->GetFilteredSensitivitis(s, df);
+    }
+};
+
+int run_simp(int nx, int ny, int iters, bool use_mma, bool heaviside, const std::string& out, int nsave) {
     Problem2D P = make_problem(nx, ny);
     const int ne = (int)P.elements.size();
     const double R = 1.5, E0 = 1.0e-9, E1 = 1.0, p = 3.0, weightlimit = 0.5, scale0 = 1.0, scale1 = 1.0;
@@ -196,7 +210,12 @@ int run_simp(int nx, int ny, int iters, bool use_mma, const std::string& out, in
         build_neighbors(P.cg, nx, ny, R, false, nb2, w2);
         if (nb2 != neighbors || w2 != w) { std::fprintf(stderr, "windowed neighbour search differs\n"); return 3; }
     }
-    DensityFilter<double> filter = DensityFilter<double>(ne, neighbors, w);
+    DensityFilter<double> dfilter = DensityFilter<double>(ne, neighbors, w);
+    HeavisideFilter<double> hfilter = HeavisideFilter<double>(ne, neighbors, w);
+    LoopFilter filter;
+    if (heaviside) filter.h = &hfilter; else filter.d = &dfilter;
+    double beta = 0.5;
+    std::vector<double> hist_beta;
     std::vector<double> s(ne, 0.5);
     if (const char* s0 = std::getenv("PF_HARNESS_S0")) {
         // optional starting design read from a raw float64 file (ne values)
@@ -215,7 +234,12 @@ int run_simp(int nx, int ny, int iters, bool use_mma, const std::string& out, in
     std::vector<int32_t> hist_decisions;  // bisection decisions (1: g>0) of every OC update, flattened
     std::vector<double> saved_s, saved_rho;
     for (int k = 0; k < iters; k++) {
-        std::vector<double> rho = filter.GetFilteredVariables(s);
+        if (heaviside && k % 40 == 0) {
+            beta *= 2.0;
+            hfilter.UpdateBeta(beta);
+        }
+        hist_beta.push_back(heaviside ? beta : 0.0);
+        std::vector<double> rho = filter.vars(s);
         double g = 0.0;
         std::vector<double> dgdrho(ne, 0.0);
         for (int i = 0; i < ne; i++) {
@@ -251,8 +275,8 @@ int run_simp(int nx, int ny, int iters, bool use_mma, const std::string& out, in
             Vector<double> ue = ElementVector(A.u, nodetoelement, P.elements[i]);
             dfdrho[i] = -scale0 * p * (-E0 + E1) * pow(rho[i], p - 1.0) * (ue * (Ke * ue));
         }
-        std::vector<double> dfds = filter.GetFilteredSensitivitis(s, dfdrho);
-        std::vector<double> dgds = filter.GetFilteredSensitivitis(s, dgdrho);
+        std::vector<double> dfds = filter.sens(s, dfdrho);
+        std::vector<double> dgds = filter.sens(s, dgdrho);
 
         if (k == 0) {
             std::vector<int32_t> dofmap;
@@ -308,7 +332,7 @@ int run_simp(int nx, int ny, int iters, bool use_mma, const std::string& out, in
             int nbis = 0;
             oc.UpdateVariables(s, f, dfds, g, dgds, [&](std::vector<double> _xkp1) {
                 double gg = 0.0;
-                std::vector<double> rr = filter.GetFilteredVariables(_xkp1);
+                std::vector<double> rr = filter.vars(_xkp1);
                 for (int i = 0; i < ne; i++) gg += scale1 * rr[i] / (weightlimit * ne);
                 gg = gg - 1.0 * scale1;
                 hist_decisions.push_back(gg > 0.0 ? 1 : 0);
@@ -325,7 +349,8 @@ int run_simp(int nx, int ny, int iters, bool use_mma, const std::string& out, in
         }
         if (k < nsave) { saved_s.insert(saved_s.end(), s.begin(), s.end()); saved_rho.insert(saved_rho.end(), rho.begin(), rho.end()); }
     }
-    std::vector<double> rho_final = filter.GetFilteredVariables(s);
+    std::vector<double> rho_final = filter.vars(s);
+    if (heaviside) dump(out, "hist_beta.f64", hist_beta);
     dump(out, "hist_f.f64", hist_f);
     dump(out, "hist_g.f64", hist_g);
     dump(out, "hist_lambda.f64", hist_lambda);
@@ -551,8 +576,9 @@ int main(int argc, char** argv) {
     if (argc < 2) { std::fprintf(stderr, "usage: ref_harness <mode> ...\n"); return 1; }
     std::string mode = argv[1];
     if (mode == "simp" && argc >= 7)
-        return run_simp(std::atoi(argv[2]), std::atoi(argv[3]), std::atoi(argv[4]), std::string(argv[5]) == "mma", argv[6],
-                        argc >= 8 ? std::atoi(argv[7]) : 0);
+        return run_simp(std::atoi(argv[2]), std::atoi(argv[3]), std::atoi(argv[4]),
+                        std::string(argv[5]) == "mma" || std::string(argv[5]) == "mmaH",
+                        std::string(argv[5]).back() == 'H', argv[6], argc >= 8 ? std::atoi(argv[7]) : 0);
     if (mode == "operator" && argc >= 6)
         return run_operator(std::atoi(argv[2]), std::atoi(argv[3]), std::strtoull(argv[4], nullptr, 10), argv[5]);
     if (mode == "solid" && argc >= 4) return run_solid(argv[2], argv[3]);

@@ -39,7 +39,7 @@ class SolveStats(ctypes.Structure):
 
 class OcParams(ctypes.Structure):
     _fields_ = [(n, ctypes.c_double) for n in ("iota", "lambdamin", "lambdamax", "lambdaeps", "movelimit", "vf",
-                                               "scale1")] + [("max_steps", ctypes.c_int)]
+                                               "scale1")] + [("max_steps", ctypes.c_int), ("beta", ctypes.c_double)]
 
 
 class MmaParams(ctypes.Structure):
@@ -54,7 +54,9 @@ class SimpParams(ctypes.Structure):
                [("cg_itrmax", ctypes.c_int), ("use_mma", ctypes.c_int), ("solver", ctypes.c_int),
                 ("mixed", ctypes.c_int), ("inner_eps", ctypes.c_double), ("learned", ctypes.c_int),
                 ("stop_on_convergence", ctypes.c_int), ("oc", OcParams), ("mma", MmaParams),
-                ("collect", ctypes.c_int), ("collect_max", ctypes.c_int)]
+                ("collect", ctypes.c_int), ("collect_max", ctypes.c_int), ("heaviside", ctypes.c_int),
+                ("beta0", ctypes.c_double), ("beta_factor", ctypes.c_double), ("beta_every", ctypes.c_int),
+                ("iter_offset", ctypes.c_int)]
 
 
 class SimpHistory(ctypes.Structure):
@@ -102,6 +104,7 @@ def lib():
             "pf_filter_setup": ([c_void_p, c_double], c_int),
             "pf_filter": ([c_void_p, c_int, _f64p, _f64p], c_int),
             "pf_filter_set_lists": ([c_void_p, _i32p, _i32p, _f64p], c_int),
+            "pf_heaviside": ([c_void_p, c_double, c_int, _f64p, c_void_p, _f64p], c_int),
             "pf_oc_candidate": ([c_void_p, _f64p, _f64p, _f64p, c_double, c_double, c_double, _f64p], c_int),
             "pf_oc_update": ([c_void_p, _f64p, _f64p, _f64p, P(OcParams), P(c_double), P(c_int)], c_int),
             "pf_mma_init": ([c_void_p, P(MmaParams)], c_int),
@@ -308,10 +311,20 @@ class Context:
         self._ck(lib().pf_filter(self._h, mode, _f64(x), out))
         return out
 
+    def heaviside(self, beta, s, dfdrho=None):
+        """HeavisideFilter at beta over the current filter: the projected
+        filtered variables, or (with dfdrho) the filtered sensitivities."""
+        out = np.zeros(self.ne)
+        d = None if dfdrho is None else _f64(dfdrho)
+        self._ck(lib().pf_heaviside(self._h, beta, 0 if d is None else 1, _f64(s),
+                                    None if d is None else d.ctypes.data, out))
+        return out
+
     def oc_update(self, x, dfdx, dgdx, iota=0.5, lambdamin=0.0, lambdamax=1e9, lambdaeps=1e-3, movelimit=0.2,
-                  vf=0.5, scale1=1.0, max_steps=200):
-        """OC<T>::UpdateVariables (OC.h:76-107) with the filtered-volume closure."""
-        prm = OcParams(iota, lambdamin, lambdamax, lambdaeps, movelimit, vf, scale1, max_steps)
+                  vf=0.5, scale1=1.0, max_steps=200, beta=0.0):
+        """OC<T>::UpdateVariables (OC.h:76-107) with the filtered-volume closure
+        (HeavisideFilter at beta when beta > 0)."""
+        prm = OcParams(iota, lambdamin, lambdamax, lambdaeps, movelimit, vf, scale1, max_steps, beta)
         x = np.array(x, dtype=np.float64)
         lam = ctypes.c_double(0.0)
         steps = ctypes.c_int(0)

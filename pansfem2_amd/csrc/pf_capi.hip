@@ -121,6 +121,7 @@ void free_grid(pf_ctx* ctx) {
     dfree(ctx->f_x); dfree(ctx->f_r); dfree(ctx->f_z); dfree(ctx->f_q); dfree(ctx->f_p0); dfree(ctx->f_p1);
     dfree(ctx->f_D); dfree(ctx->f_b);
     dfree(ctx->d_e0); dfree(ctx->d_e1); dfree(ctx->d_e2); dfree(ctx->d_e3); dfree(ctx->d_e4); dfree(ctx->d_e5);
+    dfree(ctx->d_e6);
     dfree(ctx->d_cg); dfree(ctx->d_load);
     dfree(ctx->d_indptr); dfree(ctx->d_indices); dfree(ctx->d_data); dfree(ctx->d_ke_all);
     dfree(ctx->d_fptr); dfree(ctx->d_fidx); dfree(ctx->d_fw);
@@ -144,7 +145,7 @@ int alloc_grid(pf_ctx* ctx) {
     for (double** p : dv) PF_TRY(dalloc(ctx, p, nv));
     float** fv[] = {&ctx->f_x, &ctx->f_r, &ctx->f_z, &ctx->f_q, &ctx->f_p0, &ctx->f_p1, &ctx->f_D, &ctx->f_b};
     for (float** p : fv) PF_TRY(dalloc(ctx, p, nv));
-    double** de[] = {&ctx->d_e0, &ctx->d_e1, &ctx->d_e2, &ctx->d_e3, &ctx->d_e4, &ctx->d_e5};
+    double** de[] = {&ctx->d_e0, &ctx->d_e1, &ctx->d_e2, &ctx->d_e3, &ctx->d_e4, &ctx->d_e5, &ctx->d_e6};
     for (double** p : de) PF_TRY(dalloc(ctx, p, ne));
     PF_TRY(dalloc(ctx, &ctx->d_cg, ne * ctx->dim));
     PF_HIP(hipMemsetAsync(ctx->d_load, 0, sizeof(double) * nv, ctx->stream));
@@ -1155,10 +1156,10 @@ int oc_device(pf_ctx* ctx, const pf_oc_params* prm, const double* x, const doubl
                                                      prm->lambdaeps, prm->scale1, ctx->red.partials, (int)g, xplus, S);
             if (is2d(ctx))
                 k_oc_volume<<<g, kBlock, 0, ctx->stream>>>(ctx->nx, ctx->ny, ctx->win, ctx->R, ctx->d_cg, xplus,
-                                                          prm->vf, prm->scale1, ctx->red.partials, S);
+                                                          prm->vf, prm->scale1, prm->beta, ctx->red.partials, S);
             else
                 k_oc_volume_h8<<<g, kBlock, 0, ctx->stream>>>(g3(ctx), ctx->win, ctx->R, ctx->d_cg, xplus, prm->vf,
-                                                             prm->scale1, ctx->red.partials, S);
+                                                             prm->scale1, prm->beta, ctx->red.partials, S);
         }
         PF_TRY(check_launch(ctx));
         PF_HIP(hipMemcpyAsync((void*)H, S, sizeof(OcDev), hipMemcpyDeviceToHost, ctx->stream));
@@ -1919,6 +1920,8 @@ void pf_simp_default_params(pf_simp_params* p) {
     p->mma.m = 1; p->mma.a0 = 1.0; p->mma.a = 0.0; p->mma.c = 1e4; p->mma.d = 0.0;
     p->mma.xmin = 0.01; p->mma.xmax = 1.0; p->mma.raa0 = 1e-5; p->mma.albefa = 0.1; p->mma.move = 0.2;
     p->mma.asyinit = 0.5; p->mma.asydecr = 0.7; p->mma.asyincr = 1.2; p->mma.epsvalue = 1e-6; p->mma.max_newton = 500;
+    p->oc.beta = 0.0;
+    p->heaviside = 0; p->beta0 = 0.5; p->beta_factor = 2.0; p->beta_every = 40; p->iter_offset = 0;
 }
 
 int pf_simp_run(pf_ctx* ctx, const pf_simp_params* prm, int iters, double* s, double* rho_out, pf_simp_history* hist) {
@@ -1935,21 +1938,44 @@ int pf_simp_run(pf_ctx* ctx, const pf_simp_params* prm, int iters, double* s, do
     const size_t eb = sizeof(double) * ne;
     double *d_s = ctx->d_e0, *d_dfdrho = ctx->d_e1, *d_dfds = ctx->d_e2, *d_dgds = ctx->d_e3, *d_xplus = ctx->d_e4;
     double* d_tmp = ctx->d_e5;
+    double* d_dr = ctx->d_e6;
+    const bool hv = prm->heaviside != 0;
+    PF_CHECK(!hv || (prm->beta0 > 0 && prm->beta_factor > 0 && prm->beta_every > 0), PF_ERR_ARG,
+             "Heaviside continuation needs beta0, beta_factor and beta_every > 0");
     PF_HIP(hipMemcpyAsync(d_s, s, eb, hipMemcpyHostToDevice, ctx->stream));
-    // dgdrho is the constant scale1/(vf ne) (sample...oc.cpp:103); its filtered form is fixed
-    k_fill<double><<<gsize(ne), kBlock, 0, ctx->stream>>>(ne, d_tmp, prm->scale1 / (prm->vf * (double)ne));
-    PF_TRY(filter_device(ctx, d_tmp, d_dgds));
+    // dgdrho is the constant scale1/(vf ne) (sample...oc.cpp:103); with the
+    // density filter its filtered form is fixed
+    const double dgdrho = prm->scale1 / (prm->vf * (double)ne);
+    if (!hv) {
+        k_fill<double><<<gsize(ne), kBlock, 0, ctx->stream>>>(ne, d_tmp, dgdrho);
+        PF_TRY(filter_device(ctx, d_tmp, d_dgds));
+    }
     pf_oc_params oc = prm->oc;
     oc.vf = prm->vf;
     oc.scale1 = prm->scale1;
+    oc.beta = 0.0;
+    double beta = 0.0;
     if (prm->use_mma) PF_TRY(mma_init_device(ctx, &prm->mma));
     double prev_f = 0.0;
     int it = 0;
     for (; it < iters; it++) {
         auto t0 = std::chrono::steady_clock::now();
-        // rho = filter(s); g = volume(rho)
-        PF_TRY(filter_device(ctx, d_s, ctx->d_rho));
         const unsigned ge = gsize(ne);
+        if (hv) {
+            // beta continuation (sample_optimize_density_oc.cpp:89-93): beta0
+            // multiplied by beta_factor at every beta_every-th iteration from 0
+            const int K = prm->iter_offset + it;
+            beta = prm->beta0;
+            for (int q = 0; q <= K / prm->beta_every; q++) beta *= prm->beta_factor;
+            oc.beta = beta;
+            // rho = H(filter(s)), dr = H'(filter(s)) (HeavisideFilter.h:62-99)
+            PF_TRY(filter_device(ctx, d_s, d_tmp));
+            k_heaviside<<<ge, kBlock, 0, ctx->stream>>>(ne, d_tmp, beta, ctx->d_rho, d_dr);
+        } else {
+            // rho = filter(s)
+            PF_TRY(filter_device(ctx, d_s, ctx->d_rho));
+        }
+        // g = volume(rho)
         k_volume_partials<<<ge, kBlock, 0, ctx->stream>>>(ne, ctx->d_rho, prm->vf, prm->scale1, ctx->red.partials);
         int rc;
         double g = reduce_host(ctx, ctx->red.partials, (int)ge, 1.0, 1.0 * prm->scale1, &rc);
@@ -1973,7 +1999,15 @@ int pf_simp_run(pf_ctx* ctx, const pf_simp_params* prm, int iters, double* s, do
         // compliance, sensitivities, filtered sensitivities
         double f;
         PF_TRY(comp_sens_device(ctx, ctx->d_x, prm->scale0, d_dfdrho, &f));
-        PF_TRY(filter_device(ctx, d_dfdrho, d_dfds));
+        if (hv) {
+            // sum_j dfdrho_j dr_j w_ij / sum_j w_ij, and the same for dgdrho
+            k_mul_d<<<ge, kBlock, 0, ctx->stream>>>(ne, d_dfdrho, d_dr, d_tmp);
+            PF_TRY(filter_device(ctx, d_tmp, d_dfds));
+            k_cmul_d<<<ge, kBlock, 0, ctx->stream>>>(ne, dgdrho, d_dr, d_tmp);
+            PF_TRY(filter_device(ctx, d_tmp, d_dgds));
+        } else {
+            PF_TRY(filter_device(ctx, d_dfdrho, d_dfds));
+        }
         // IsConvergence of OC (OC.h:67-73, eps 1e-5) / MMA (MMA.h:107-113, epsvalue)
         const double ceps = prm->use_mma ? prm->mma.epsvalue : 1.0e-5;
         bool conv = std::fabs(f - prev_f) / (f + prev_f) < ceps;
@@ -2008,9 +2042,34 @@ int pf_simp_run(pf_ctx* ctx, const pf_simp_params* prm, int iters, double* s, do
     if (hist) hist->iterations = it;
     PF_HIP(hipMemcpyAsync(s, d_s, eb, hipMemcpyDeviceToHost, ctx->stream));
     if (rho_out) {
-        PF_TRY(filter_device(ctx, d_s, ctx->d_rho));
+        PF_TRY(filter_device(ctx, d_s, hv ? d_tmp : ctx->d_rho));
+        if (hv) k_heaviside<<<gsize(ne), kBlock, 0, ctx->stream>>>(ne, d_tmp, beta, ctx->d_rho, nullptr);
         PF_HIP(hipMemcpyAsync(rho_out, ctx->d_rho, eb, hipMemcpyDeviceToHost, ctx->stream));
     }
+    return sync(ctx);
+}
+
+int pf_heaviside(pf_ctx* ctx, double beta, int mode, const double* s, const double* dfdrho, double* out) {
+    if (!ctx || !s || !out || (mode == 1 && !dfdrho)) return PF_ERR_ARG;
+    PF_CHECK(mode == 0 || mode == 1, PF_ERR_ARG, "pf_heaviside: mode 0 (variables) or 1 (sensitivities)");
+    PF_CHECK(beta > 0 && std::isfinite(beta), PF_ERR_ARG, "pf_heaviside: beta must be positive");
+    PF_CHECK(ctx->dim > 0, PF_ERR_STATE, "no grid");
+    PF_TRY(set_device(ctx));
+    const long ne = ctx->ne;
+    const size_t eb = sizeof(double) * ne;
+    const unsigned ge = gsize(ne);
+    PF_HIP(hipMemcpyAsync(ctx->d_e0, s, eb, hipMemcpyHostToDevice, ctx->stream));
+    PF_TRY(filter_device(ctx, ctx->d_e0, ctx->d_e1));  // s~
+    if (mode == 0) {
+        k_heaviside<<<ge, kBlock, 0, ctx->stream>>>(ne, ctx->d_e1, beta, ctx->d_e2, nullptr);
+    } else {
+        k_heaviside<<<ge, kBlock, 0, ctx->stream>>>(ne, ctx->d_e1, beta, ctx->d_e3, ctx->d_e6);
+        PF_HIP(hipMemcpyAsync(ctx->d_e4, dfdrho, eb, hipMemcpyHostToDevice, ctx->stream));
+        k_mul_d<<<ge, kBlock, 0, ctx->stream>>>(ne, ctx->d_e4, ctx->d_e6, ctx->d_e5);
+        PF_TRY(filter_device(ctx, ctx->d_e5, ctx->d_e2));
+    }
+    PF_TRY(check_launch(ctx));
+    PF_HIP(hipMemcpyAsync(out, ctx->d_e2, eb, hipMemcpyDeviceToHost, ctx->stream));
     return sync(ctx);
 }
 
@@ -2091,7 +2150,7 @@ int pf_profile_kernels(pf_ctx* ctx, int op, int reps, double* ms) {
             k_oc_xplus<<<ge, kBlock, 0, ctx->stream>>>(ctx->ne, 0, ctx->d_rho, ctx->d_e1, ctx->d_e2, 0.5, 0.2, 1e-3,
                                                       1.0, ctx->red.partials, (int)ge, ctx->d_e4, (OcDev*)ctx->d_oc);
             k_oc_volume<<<ge, kBlock, 0, ctx->stream>>>(ctx->nx, ctx->ny, ctx->win, ctx->R, ctx->d_cg, ctx->d_e4, 0.5,
-                                                       1.0, ctx->red.partials, (OcDev*)ctx->d_oc);
+                                                       1.0, 0.0, ctx->red.partials, (OcDev*)ctx->d_oc);
         });
     }
     hipEventDestroy(e0);

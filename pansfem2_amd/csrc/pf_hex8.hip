@@ -249,14 +249,18 @@ __global__ void __launch_bounds__(256) k_filter_h8(G3 g, int win, double R, cons
 
 __global__ void __launch_bounds__(256) k_oc_volume_h8(G3 g, int win, double R, const double* __restrict__ cg,
                                                       const double* __restrict__ xplus, double vf, double scale1,
-                                                      double* __restrict__ part, const OcDev* __restrict__ S) {
+                                                      double beta, double* __restrict__ part,
+                                                      const OcDev* __restrict__ S) {
     __shared__ double lds[32];
     if (S->done) return;
     const long ne = g.nelem();
     const double den = vf * (double)ne;
     double acc[1] = {0.0};
-    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < ne; e += (long)gridDim.x * blockDim.x)
-        acc[0] += scale1 * filter_one_h8(g, win, R, cg, xplus, e) / den;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < ne; e += (long)gridDim.x * blockDim.x) {
+        double v = filter_one_h8(g, win, R, cg, xplus, e);
+        if (beta > 0.0) v = heaviside_rho(v, beta);
+        acc[0] += scale1 * v / den;
+    }
     store_partials<1>(acc, lds, part);
 }
 

@@ -93,6 +93,7 @@ struct pf_ctx {
 
     // element work vectors (ne)
     double *d_e0 = nullptr, *d_e1 = nullptr, *d_e2 = nullptr, *d_e3 = nullptr, *d_e4 = nullptr, *d_e5 = nullptr;
+    double* d_e6 = nullptr;     // projection derivative of the Heaviside filter
     double* d_cg = nullptr;     // element centroids (ne*dim)
 
     // point loads (nodal)

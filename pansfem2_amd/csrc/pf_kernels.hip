@@ -895,6 +895,41 @@ __device__ inline double oc_candidate(double xi, double dfdxi, double dgdxi, dou
     return v;
 }
 
+// HeavisideFilter projection of a filtered value st (HeavisideFilter.h:73)
+// and its derivative d rho / d st (HeavisideFilter.h:87), in the reference's
+// operation order (pow(t, 2.0) is t*t in the reference's build).
+__device__ inline double heaviside_rho(double st, double beta) {
+    return 0.5 * (tanh(0.5 * beta) + tanh(beta * (st - 0.5))) / tanh(0.5 * beta);
+}
+__device__ inline double heaviside_drho(double st, double beta) {
+    const double t = tanh(beta * (st - 0.5));
+    return 0.5 * beta * (1.0 - t * t) / tanh(0.5 * beta);
+}
+
+// rho = H(st); dr = H'(st) when dr is non-null.
+__global__ void __launch_bounds__(256) k_heaviside(long ne, const double* __restrict__ st, double beta,
+                                                   double* __restrict__ rho, double* __restrict__ dr) {
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < ne; e += (long)gridDim.x * blockDim.x) {
+        const double v = st[e];
+        rho[e] = heaviside_rho(v, beta);
+        if (dr) dr[e] = heaviside_drho(v, beta);
+    }
+}
+
+// out = a .* b
+__global__ void __launch_bounds__(256) k_mul_d(long n, const double* __restrict__ a, const double* __restrict__ b,
+                                               double* __restrict__ out) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        out[i] = a[i] * b[i];
+}
+
+// out = c * b
+__global__ void __launch_bounds__(256) k_cmul_d(long n, double c, const double* __restrict__ b,
+                                                double* __restrict__ out) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        out[i] = c * b[i];
+}
+
 struct OcDev {
     double l0i, l1i;  // initial bracket (lambdamin, lambdamax), set by the host
     double l0[2], l1[2], lambda[2];
@@ -954,14 +989,18 @@ __global__ void __launch_bounds__(256) k_oc_candidate(long ne, const double* __r
 
 __global__ void __launch_bounds__(256) k_oc_volume(int nx, int ny, int win, double R, const double* __restrict__ cg,
                                                    const double* __restrict__ xplus, double vf, double scale1,
-                                                   double* __restrict__ part, const OcDev* __restrict__ S) {
+                                                   double beta, double* __restrict__ part,
+                                                   const OcDev* __restrict__ S) {
     __shared__ double lds[32];
     if (S->done) return;
     const long ne = (long)nx * ny;
     const double den = vf * (double)ne;
     double acc[1] = {0.0};
-    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < ne; e += (long)gridDim.x * blockDim.x)
-        acc[0] += scale1 * filter_one(nx, ny, win, R, cg, xplus, e) / den;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < ne; e += (long)gridDim.x * blockDim.x) {
+        double v = filter_one(nx, ny, win, R, cg, xplus, e);
+        if (beta > 0.0) v = heaviside_rho(v, beta);  // HeavisideFilter in the closure
+        acc[0] += scale1 * v / den;
+    }
     store_partials<1>(acc, lds, part);
 }
 

@@ -5,19 +5,23 @@
 // reactions, sensitivities, DensityFilter, OC with a volume closure), with
 // the assembly, solve, filter and OC candidates running on the GPU.
 //
-//   simp_oc nx ny iters [out_prefix]
+//   simp_oc nx ny iters [out_prefix] [H]
 //
 // Grid nx x ny of unit quad4 elements, left edge clamped, unit downward load
 // at the bottom-right node, E1 = 1, E0 = 1e-9, nu = 0.3, p = 3, volume
 // fraction 0.5, filter radius 1.5, ScalingCG(100000, 1e-10), OC(0.5, 0,
 // 1e9, 1e-3, 0.2).  Prints "k f g lambda" per iteration; with out_prefix,
 // writes the final design variables and their filtered densities as text.
+// With "H": HeavisideFilter with the density samples' beta continuation
+// (beta 0.5, doubled at every 40th iteration from k = 0) instead of
+// DensityFilter.
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
 #include <iomanip>
 #include <numeric>
+#include <string>
 #include <vector>
 
 #include "pansfem2/pansfem2.hpp"
@@ -62,7 +66,16 @@ int main(int argc, char** argv) {
                         }
                     }
             }
-        DensityFilter<double> filter(ne, neighbors, w);
+        const bool heaviside = argc > 5 && std::string(argv[5]) == "H";
+        DensityFilter<double> dfilter(ne, neighbors, w);
+        HeavisideFilter<double> hfilter(ne, neighbors, w);
+        double beta = 0.5;
+        auto vars = [&](const std::vector<double>& x) {
+            return heaviside ? hfilter.GetFilteredVariables(x) : dfilter.GetFilteredVariables(x);
+        };
+        auto sens = [&](const std::vector<double>& x, const std::vector<double>& d) {
+            return heaviside ? hfilter.GetFilteredSensitivitis(x, d) : dfilter.GetFilteredSensitivitis(x, d);
+        };
 
         std::vector<double> s(ne, 0.5);
         const double E0 = 1.0e-9, E1 = 1.0, nu = 0.3, p = 3.0, vf = 0.5, scale0 = 1.0, scale1 = 1.0;
@@ -70,7 +83,11 @@ int main(int argc, char** argv) {
                              std::vector<double>(ne, 1.0));
         std::vector<double> rho;
         for (int k = 0; k < iters; k++) {
-            rho = filter.GetFilteredVariables(s);
+            if (heaviside && k % 40 == 0) {
+                beta *= 2.0;
+                hfilter.UpdateBeta(beta);
+            }
+            rho = vars(s);
             double g = 0.0;
             std::vector<double> dgdrho(ne);
             for (int i = 0; i < ne; i++) {
@@ -126,20 +143,20 @@ int main(int argc, char** argv) {
                 Vector<double> ue = ElementVector(u, n2e, elements[e]);
                 dfdrho[e] = -scale0 * p * (-E0 + E1) * std::pow(rho[e], p - 1.0) * (ue * (Ke * ue));
             }
-            std::vector<double> dfds = filter.GetFilteredSensitivitis(s, dfdrho);
-            std::vector<double> dgds = filter.GetFilteredSensitivitis(s, dgdrho);
+            std::vector<double> dfds = sens(s, dfdrho);
+            std::vector<double> dgds = sens(s, dgdrho);
 
             if (optimizer.IsConvergence(f)) break;
             optimizer.UpdateVariables(s, f, dfds, g, dgds, [&](std::vector<double> xk1) {
                 double gk = 0.0;
-                std::vector<double> rk = filter.GetFilteredVariables(xk1);
+                std::vector<double> rk = vars(xk1);
                 for (int i = 0; i < ne; i++) gk += scale1 * rk[i] / (vf * ne);
                 return gk - 1.0 * scale1;
             });
             std::printf("%d %.17g %.17g %.17g\n", k, f, g, optimizer.LastLambda());
             std::fflush(stdout);
         }
-        rho = filter.GetFilteredVariables(s);  // densities of the final design
+        rho = vars(s);  // densities of the final design
         if (argc > 4) {
             std::ofstream fs(std::string(argv[4]) + "_s.txt"), fr(std::string(argv[4]) + "_rho.txt");
             fs << std::setprecision(17);

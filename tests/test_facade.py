@@ -77,3 +77,18 @@ def test_sample_driver_against_reference(tmp_path):
     rho = np.loadtxt(prefix + "_rho.txt")
     assert np.max(np.abs(s - g["s_final"])) < 1e-6
     assert np.max(np.abs(rho - g["rho_final"])) < 1e-6
+
+
+@pytest.mark.gpu
+def test_sample_driver_heaviside_against_reference(tmp_path):
+    """The same driver with HeavisideFilter and its beta continuation."""
+    case = "simp_120x40_ocH"
+    g = gi.load(case)
+    prefix = str(tmp_path / "c1h")
+    out = subprocess.run([SAMPLE, "120", "40", "50", prefix, "H"], capture_output=True, text=True, timeout=240)
+    assert out.returncode == 0, out.stderr
+    hist = np.array([[float(v) for v in line.split()] for line in out.stdout.splitlines() if line.strip()])
+    assert hist.shape == (50, 4)
+    assert np.max(np.abs(hist[:, 1] - g["hist_f"]) / g["hist_f"]) < 1e-9
+    assert np.max(np.abs(np.loadtxt(prefix + "_s.txt") - g["s_final"])) < 1e-6
+    assert np.max(np.abs(np.loadtxt(prefix + "_rho.txt") - g["rho_final"])) < 1e-6

@@ -237,3 +237,46 @@ def test_facade_entry_points_against_reference(pa):
     base = -g["dfds0"] / (g["dgds0"] * lam)
     want = np.clip(np.sqrt(base) * s0, np.maximum(0.0, 0.8 * s0), np.minimum(1.0, 1.2 * s0))
     assert np.array_equal(xp, want)
+
+
+def test_heaviside_entry_point_against_oracle(pa):
+    """pf_heaviside over the reference's filter lists: projected variables
+    and chain-rule sensitivities (HeavisideFilter.h:62-99).  The device tanh
+    and glibc's may differ in the last bit, so the bar is 1e-14 relative."""
+    from oracle import pyoracle as po
+    g = gi.load("simp_120x40_oc")
+    nx, ny = 120, 40
+    ctx = pa.cantilever2d(nx, ny)
+    ctx.filter_set_lists(g["filter_ptr"], g["filter_idx"], g["filter_w"])
+    rng = np.random.default_rng(8)
+    s = rng.uniform(0.0, 1.0, nx * ny)
+    d = rng.uniform(-1.0, 0.0, nx * ny)
+    fp, fi, fw = g["filter_ptr"], g["filter_idx"], g["filter_w"]
+    for beta in (1.0, 2.0, 16.0):
+        r = ctx.heaviside(beta, s)
+        assert np.max(np.abs(r - po.heaviside_vars(fp, fi, fw, beta, s))) < 1e-14
+        ds = ctx.heaviside(beta, s, d)
+        want = po.heaviside_sens(fp, fi, fw, beta, s, d)
+        assert np.max(np.abs(ds - want)) < 1e-14 * np.max(np.abs(want)) * 10
+    # the grid rule gives the same values as the reference's lists
+    ctx.filter_setup(1.5)
+    assert np.max(np.abs(ctx.heaviside(2.0, s) - po.heaviside_vars(fp, fi, fw, 2.0, s))) < 1e-14
+
+
+def test_simp_heaviside_loop_c1_against_reference(pa):
+    """50 OC iterations at 120x40 with HeavisideFilter and the beta
+    continuation (beta 1, then 2 from iteration 40) against the reference."""
+    case = "simp_120x40_ocH"
+    g = gi.load(case)
+    ctx = pa.cantilever2d(120, 40)
+    r = ctx.simp_run(50, pa.simp_default_params(heaviside=1))
+    from oracle import pyoracle as po
+    ref = po.simp2d(120, 40, 50, po.simp_params(heaviside=1))
+    assert gi.sha(ref["f"]) == gi.digest(case, "hist_f")
+    assert np.max(np.abs(r["f"] - ref["f"]) / ref["f"]) < 1e-9
+    assert np.max(np.abs(r["rho"] - g["rho_final"])) < 1e-6
+    assert np.max(np.abs(r["s"] - g["s_final"])) < 1e-6
+    # continuation across calls: 40 iterations, then 10 more from iteration 40
+    r1 = ctx.simp_run(40, pa.simp_default_params(heaviside=1))
+    r2 = ctx.simp_run(10, pa.simp_default_params(heaviside=1, iter_offset=40), s0=r1["s"])
+    assert np.max(np.abs(r2["s"] - g["s_final"])) < 1e-6

@@ -62,10 +62,10 @@ def test_iteration0_against_reference(case, nx, ny):
 
 
 @pytest.mark.parametrize("case,nx,ny,iters", [("simp_4x2_oc", 4, 2, 3), ("simp_8x4_oc", 8, 4, 5),
-                                              ("simp_16x8_mma", 16, 8, 5)])
+                                              ("simp_16x8_mma", 16, 8, 5), ("simp_16x8_ocH", 16, 8, 5)])
 def test_simp_loop_small_against_reference(case, nx, ny, iters):
     g = gi.load(case)
-    prm = po.simp_params(use_mma=1 if "mma" in case else 0)
+    prm = po.simp_params(use_mma=1 if "mma" in case else 0, heaviside=1 if case.endswith("H") else 0)
     r = po.simp2d(nx, ny, iters, prm)
     assert gi.sha(r["f"]) == gi.digest(case, "hist_f")
     assert gi.sha(r["g"]) == gi.digest(case, "hist_g")
@@ -77,11 +77,12 @@ def test_simp_loop_small_against_reference(case, nx, ny, iters):
 
 
 @pytest.mark.slow
-@pytest.mark.parametrize("case", ["simp_120x40_oc", "simp_120x40_mma"])
+@pytest.mark.parametrize("case", ["simp_120x40_oc", "simp_120x40_mma", "simp_120x40_ocH"])
 def test_simp_loop_c1_against_reference(case):
-    """All 50 OC (or MMA) iterations of the 120x40 workload, bit for bit."""
+    """All 50 OC (or MMA) iterations of the 120x40 workload, bit for bit
+    (ocH: HeavisideFilter with beta continuation, beta 1 then 2 at k = 40)."""
     g = gi.load(case)
-    prm = po.simp_params(use_mma=1 if "mma" in case else 0)
+    prm = po.simp_params(use_mma=1 if "mma" in case else 0, heaviside=1 if case.endswith("H") else 0)
     r = po.simp2d(120, 40, 50, prm)
     assert gi.sha(r["f"]) == gi.digest(case, "hist_f")
     assert gi.sha(r["g"]) == gi.digest(case, "hist_g")
@@ -310,3 +311,40 @@ def test_oc_uniform_case():
     assert abs(x[0] - 0.5) < 1e-3
     # first decisions: lambda halves from 5e8 while the design is too light
     assert list(dec[:5]) == [0, 0, 0, 0, 0]
+
+
+def test_heaviside_projection_closed_forms():
+    """Pins that do not need the reference: s~ = 1/2 maps to 1/2, 0 to 0 and 1
+    to 1 for any beta; large beta approaches a step; the sensitivity is the
+    chain rule of the projection through the row-normalised filter
+    (checked by central differences on a small grid)."""
+    nx, ny = 10, 8
+    xy, conn = po.grid2d(nx, ny)
+    fp, fi, fw = po.filter_build(nx, ny, po.centroids(conn, xy))
+    ne = nx * ny
+    for beta in (1.0, 2.0, 8.0):
+        for c, want in ((0.5, 0.5), (0.0, 0.0), (1.0, 1.0)):
+            r = po.heaviside_vars(fp, fi, fw, beta, np.full(ne, c))
+            assert np.allclose(r, want, atol=1e-15, rtol=0)
+    r = po.heaviside_vars(fp, fi, fw, 64.0, np.full(ne, 0.55))
+    assert np.all(r > 0.99)
+    # the reference's sensitivity is row-normalised by element i's weight sum,
+    # so it equals the exact gradient d(c . rho)/ds where every weight sum in
+    # the neighbourhood is the same: two elements away from the boundary
+    rng = np.random.default_rng(3)
+    s = rng.uniform(0.2, 0.8, ne)
+    c = rng.uniform(-1, 1, ne)
+    beta = 4.0
+    dfds = po.heaviside_sens(fp, fi, fw, beta, s, c)
+    h = 1e-6
+    grad = np.zeros(ne)
+    for k in range(ne):
+        sp, sm = s.copy(), s.copy()
+        sp[k] += h
+        sm[k] -= h
+        grad[k] = (c @ po.heaviside_vars(fp, fi, fw, beta, sp) - c @ po.heaviside_vars(fp, fi, fw, beta, sm)) / (2 * h)
+    interior = [ny * i + j for i in range(2, nx - 2) for j in range(2, ny - 2)]
+    assert len(interior) == 24
+    assert np.allclose(dfds[interior], grad[interior], rtol=1e-6, atol=1e-9)
+    # and a dropped chain-rule factor would not pass
+    assert not np.allclose(po.filter_sens(fp, fi, fw, c)[interior], grad[interior], rtol=1e-3)
