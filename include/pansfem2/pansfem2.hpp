@@ -16,8 +16,12 @@
 //   LILCSR<T>, CSR<T>, ScalingCG    src/LinearAlgebra/Models/{LILCSR,CSR}.h, Solvers/CG.h:419-453
 //   DensityFilter<T>                src/Optimize/Filter/DensityFilter.h:15-71
 //   HeavisideFilter<T>              src/Optimize/Filter/HeavisideFilter.h:15-99
+//   SensitivityFilter<T>, SensitivityFilter2<T>  src/Optimize/Filter/SensitivityFilter.h:14-101
 //   OC<T>                           src/Optimize/Solver/OC.h:15-107
 //   MMA<T>                          src/Optimize/Solver/MMA.h:15-509
+//   MakeHeadderToVTK, AddPointsToVTK, AddElementToVTK, AddElementTypes,
+//   AddPointScalers, AddPointVectors, AddElementScalers
+//                                   src/PrePost/Export/ExportToVTK.h:18-137
 //
 // Where the work goes:
 //   * element matrices, meshes, DOF maps, boundary conditions and the
@@ -43,6 +47,7 @@
 #include <cassert>
 #include <cmath>
 #include <cstdint>
+#include <fstream>
 #include <initializer_list>
 #include <iostream>
 #include <memory>
@@ -252,8 +257,8 @@ public:
         return Vector(std::vector<T>(v_.begin() + head, v_.begin() + tail));
     }
     Vector Normal() const { return *this / Norm(); }
-    friend std::ostream& operator<<(std::ostream& out, const Vector& x) {
-        for (int i = 0; i < x.SIZE(); i++) out << x.v_[i] << (i + 1 < x.SIZE() ? "\t" : "");
+    friend std::ostream& operator<<(std::ostream& out, const Vector& x) {  // one value per line (Vector.h:288-293)
+        for (int i = 0; i < x.SIZE(); i++) out << x.v_[i] << std::endl;
         return out;
     }
     const std::vector<T>& data() const { return v_; }
@@ -417,10 +422,10 @@ public:
             for (int j = 0; j < w; j++) m(i, j) = (*this)(row + i, col + j);
         return m;
     }
-    friend std::ostream& operator<<(std::ostream& out, const Matrix& m) {
+    friend std::ostream& operator<<(std::ostream& out, const Matrix& m) {  // rows, tab after each value
         for (int i = 0; i < m.r_; i++) {
-            for (int j = 0; j < m.c_; j++) out << m(i, j) << (j + 1 < m.c_ ? "\t" : "");
-            out << "\n";
+            for (int j = 0; j < m.c_; j++) out << m(i, j) << "\t";
+            out << std::endl;
         }
         return out;
     }
@@ -999,6 +1004,31 @@ private:
     detail::FilterLists<T> lists_;
 };
 
+// Sensitivity filters on the device over the given lists:
+// SensitivityFilter (Sigmund's scheme) divides the weighted sum of s_j df_j
+// by s_i times the weight sum, SensitivityFilter2 by the s-weighted sum.
+template <class T, int Scheme>
+class SensitivityFilterOf {
+public:
+    SensitivityFilterOf(int n, std::vector<std::vector<int> > neighbors, std::vector<std::vector<T> > w)
+        : lists_(n, neighbors, w) {}
+    std::vector<T> GetFilteredSensitivitis(std::vector<T> s, std::vector<T> dfdrho) {
+        lists_.ensure();
+        auto& dev = Device::Get();
+        std::vector<T> out(lists_.n_);
+        dev.check(pf_sensitivity_filter(dev.ctx(), Scheme, s.data(), dfdrho.data(), out.data()),
+                  "pf_sensitivity_filter");
+        return out;
+    }
+
+private:
+    detail::FilterLists<T> lists_;
+};
+template <class T>
+using SensitivityFilter = SensitivityFilterOf<T, 1>;
+template <class T>
+using SensitivityFilter2 = SensitivityFilterOf<T, 2>;
+
 // The density filter followed by the tanh projection
 // rho = (tanh(beta/2) + tanh(beta (s~ - 1/2))) / (2 tanh(beta/2)), with the
 // chain-rule sensitivities; beta is set by UpdateBeta (1 initially).
@@ -1118,5 +1148,73 @@ private:
     long generation_ = -1;
     T prev_ = T(), eps_ = 1.0e-5;
 };
+
+// ========================================================= VTK export ====
+// Legacy ASCII VTK, formatted as the reference's exporter writes it
+// (src/PrePost/Export/ExportToVTK.h:18-137): default stream formatting of
+// every value, tabs and newlines in the same places.
+inline void MakeHeadderToVTK(std::ofstream& out) {
+    out << "# vtk DataFile Version 4.1\n" << "vtk output\n" << "ASCII\n" << "DATASET UNSTRUCTURED_GRID\n";
+}
+
+template <class T>
+void AddPointsToVTK(const std::vector<Vector<T> >& nodes, std::ofstream& out) {
+    out << "\nPOINTS\t" << nodes.size() << "\tfloat\n";
+    for (const auto& node : nodes) {
+        int i = 0;
+        for (; i < node.SIZE(); i++) out << node(i) << "\t";
+        for (; i < 3; i++) out << T() << "\t";
+        out << std::endl;
+    }
+}
+
+inline void AddElementToVTK(const std::vector<std::vector<int> >& elements, std::ofstream& out) {
+    int datanum = 0;
+    for (const auto& e : elements) datanum += (int)e.size() + 1;
+    out << "\nCELLS " << elements.size() << "\t" << datanum << "\n";
+    for (const auto& e : elements) {
+        out << e.size() << "\t";
+        for (int nd : e) out << nd << "\t";
+        out << std::endl;
+    }
+}
+
+inline void AddElementTypes(const std::vector<int>& types, std::ofstream& out) {
+    out << "\nCELL_TYPES\t" << types.size() << "\n";
+    for (int t : types) out << t << "\n";
+}
+
+template <class T>
+void AddPointScalers(const std::vector<T>& values, std::string symbol, std::ofstream& out, bool isheader) {
+    if (isheader) out << "\nPOINT_DATA\t" << values.size() << "\n";
+    out << "SCALARS " << symbol << " float\n" << "LOOKUP_TABLE default\n";
+    for (const auto& v : values) out << v << std::endl;
+}
+
+template <class T>
+void AddPointScalers(const std::vector<Vector<T> >& values, std::string symbol, std::ofstream& out, bool isheader) {
+    if (isheader) out << "\nPOINT_DATA\t" << values.size() << "\n";
+    out << "SCALARS " << symbol << " float\n" << "LOOKUP_TABLE default\n";
+    for (const auto& v : values) out << v;
+}
+
+template <class T>
+void AddPointVectors(const std::vector<Vector<T> >& values, std::string symbol, std::ofstream& out, bool isheader) {
+    if (isheader) out << "\nPOINT_DATA\t" << values.size() << "\n";
+    out << "VECTORS " << symbol << " float\n";
+    for (const auto& v : values) {
+        int i = 0;
+        for (; i < v.SIZE(); i++) out << v(i) << "\t";
+        for (; i < 3; i++) out << T() << "\t";
+        out << std::endl;
+    }
+}
+
+template <class T>
+void AddElementScalers(const std::vector<T>& values, std::string symbol, std::ofstream& out, bool isheader) {
+    if (isheader) out << "\nCELL_DATA\t" << values.size() << "\n";
+    out << "SCALARS " << symbol << " float\n" << "LOOKUP_TABLE default\n";
+    for (const auto& v : values) out << v << std::endl;
+}
 
 }  // namespace PANSFEM2

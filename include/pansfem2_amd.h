@@ -197,6 +197,12 @@ int pf_filter(pf_ctx* ctx, int mode, const double* in, double* out);
  * mode 1: GetFilteredSensitivitis (HeavisideFilter.h:78-99), out_i =
  * sum_j dfdrho_j H'(s~_j) w_ij / sum_j w_ij with s~ = filter(s). ne in, ne out. */
 int pf_heaviside(pf_ctx* ctx, double beta, int mode, const double* s, const double* dfdrho, double* out);
+/* SensitivityFilter<T> (scheme 1, Sigmund's) and SensitivityFilter2<T>
+ * (scheme 2) (src/Optimize/Filter/SensitivityFilter.h:44-55,88-99) over the
+ * current neighbour lists or the quad4 grid rule: scheme 1 gives
+ * sum_j w_ij s_j dfds_j / (s_i sum_j w_ij), scheme 2 divides by
+ * sum_j w_ij s_j instead.  ne in, ne out. */
+int pf_sensitivity_filter(pf_ctx* ctx, int scheme, const double* s, const double* dfds, double* out);
 /* DensityFilter<T>(n, neighbors, w) with caller-given lists
  * (DensityFilter.h:34-43): element e's neighbours are idx[ptr[e]..ptr[e+1])
  * with weights w (same order); pf_filter then uses the lists (sums in list

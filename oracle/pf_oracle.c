@@ -433,6 +433,22 @@ void orc_filter_sens(int ne, const int* ptr, const int* idx, const double* w, co
 static double std_max(double a, double b) { return (a < b) ? b : a; } /* std::max */
 static double std_min(double a, double b) { return (b < a) ? b : a; } /* std::min */
 
+/* SensitivityFilter<T> (scheme 1, src/Optimize/Filter/SensitivityFilter.h:44-55):
+ * out_i = sum_j (w_ij s_j) dfds_j / (sum_j w_ij * s_i);
+ * SensitivityFilter2<T> (scheme 2, SensitivityFilter.h:88-99):
+ * out_i = sum_j (w_ij s_j) dfds_j / sum_j (w_ij s_j). */
+void orc_sensitivity_filter(int ne, const int* ptr, const int* idx, const double* w, const double* s,
+                            const double* dfds, int scheme, double* out) {
+    for (int i = 0; i < ne; i++) {
+        double acc = 0.0, den = 0.0;
+        for (int k = ptr[i]; k < ptr[i + 1]; k++) {
+            acc += w[k] * s[idx[k]] * dfds[idx[k]];
+            den += scheme == 2 ? w[k] * s[idx[k]] : w[k];
+        }
+        out[i] = scheme == 2 ? acc / den : acc / (den * s[i]);
+    }
+}
+
 /* HeavisideFilter<T>::GetFilteredVariables (src/Optimize/Filter/HeavisideFilter.h:62-75):
  * rho_i = 0.5 (tanh(beta/2) + tanh(beta (s~_i - 1/2))) / tanh(beta/2) with
  * s~_i = (sum_j w_ij s_j) / (sum_j w_ij). */

@@ -82,6 +82,10 @@ int orc_oc_update(int n, double* x, const double* dfdx, const double* dgdx, doub
                   double lmax, double leps, double move, const int* fptr, const int* fidx, const double* fw,
                   double vf, double scale1, double* lambda_out, int* decisions, int max_dec);
 
+/* SensitivityFilter<T> (scheme 1) and SensitivityFilter2<T> (scheme 2)
+ * (src/Optimize/Filter/SensitivityFilter.h:44-55,88-99) over the lists. */
+void orc_sensitivity_filter(int ne, const int* ptr, const int* idx, const double* w, const double* s,
+                            const double* dfds, int scheme, double* out);
 /* HeavisideFilter<T> (src/Optimize/Filter/HeavisideFilter.h:62-99) over the
  * same neighbour lists: projected filtered variables and filtered
  * sensitivities at a given beta. */

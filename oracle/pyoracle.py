@@ -53,6 +53,7 @@ def lib():
         L.orc_filter_build_brute.restype = c_long
         L.orc_filter_apply.argtypes = [c_int, _i32p, _i32p, _f64p, _f64p, _f64p]
         L.orc_filter_sens.argtypes = [c_int, _i32p, _i32p, _f64p, _f64p, _f64p]
+        L.orc_sensitivity_filter.argtypes = [c_int, _i32p, _i32p, _f64p, _f64p, _f64p, c_int, _f64p]
         L.orc_heaviside_vars.argtypes = [c_int, _i32p, _i32p, _f64p, c_double, _f64p, _f64p]
         L.orc_heaviside_sens.argtypes = [c_int, _i32p, _i32p, _f64p, c_double, _f64p, _f64p, _f64p]
         L.orc_oc_update.argtypes = [c_int, _f64p, _f64p, _f64p, c_double, c_double, c_double, c_double, c_double,
@@ -256,6 +257,14 @@ def filter_apply(ptr, idx, w, s):
 def filter_sens(ptr, idx, w, dfdrho):
     out = np.zeros(len(ptr) - 1)
     lib().orc_filter_sens(len(ptr) - 1, ptr, idx, w, np.ascontiguousarray(dfdrho, np.float64), out)
+    return out
+
+
+def sensitivity_filter(ptr, idx, w, s, dfds, scheme=1):
+    """SensitivityFilter (scheme 1) / SensitivityFilter2 (scheme 2) (SensitivityFilter.h:44-99)."""
+    out = np.zeros(len(ptr) - 1)
+    lib().orc_sensitivity_filter(len(ptr) - 1, ptr, idx, w, np.ascontiguousarray(s, np.float64),
+                                 np.ascontiguousarray(dfds, np.float64), scheme, out)
     return out
 
 

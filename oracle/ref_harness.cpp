@@ -20,6 +20,12 @@
 //   operator <nx> <ny> <seed> <outdir>
 //       assembled K(rho) for rho iid U[1e-3,1] (SplitMix64), y = K x for x iid
 //       U[-1,1], and the ScalingCG solve of K u = F to 1e-10.
+//   sensfilter <nx> <ny> <seed> <outdir>
+//       SensitivityFilter and SensitivityFilter2 on the R = 1.5 lists of an
+//       nx x ny grid for s iid U[0.01, 1] and dfds iid U[-1, 0] (SplitMix64).
+//   vtk <file>
+//       a 4 x 2 SquareMesh with nodal u, r and element rho fields (closed-form
+//       values) written through the reference's VTK exporter.
 //   solid <csvdir> <outdir>
 //       the hex8 linear-solid sample (Node/Element/Dirichlet/Neumann.csv).
 //   mmatest <outdir>
@@ -59,7 +65,9 @@
 #include "Optimize/Solver/MMA.h"
 #include "Optimize/Filter/DensityFilter.h"
 #include "Optimize/Filter/HeavisideFilter.h"
+#include "Optimize/Filter/SensitivityFilter.h"
 #include "PrePost/Mesher/SquareMesh.h"
+#include "PrePost/Export/ExportToVTK.h"
 #include "PrePost/Import/ImportFromCSV.h"
 #include "FEM/Equation/General.h"
 
@@ -390,6 +398,54 @@ int run_operator(int nx, int ny, uint64_t seed, const std::string& out) {
     return 0;
 }
 
+// ---------------------------------------------------------- sensfilter ----
+int run_sensfilter(int nx, int ny, uint64_t seed, const std::string& out) {
+    Problem2D P = make_problem(nx, ny);
+    const int ne = (int)P.elements.size();
+    std::vector<std::vector<int> > neighbors;
+    std::vector<std::vector<double> > w;
+    build_neighbors(P.cg, nx, ny, 1.5, false, neighbors, w);
+    SplitMix64 rng(seed);
+    std::vector<double> sv(ne), df(ne);
+    for (int i = 0; i < ne; i++) sv[i] = 0.01 + 0.99 * rng.uniform();
+    for (int i = 0; i < ne; i++) df[i] = -rng.uniform();
+    SensitivityFilter<double> f1(ne, neighbors, w);
+    SensitivityFilter2<double> f2(ne, neighbors, w);
+    dump(out, "s.f64", sv);
+    dump(out, "dfds.f64", df);
+    dump(out, "out1.f64", f1.GetFilteredSensitivitis(sv, df));
+    dump(out, "out2.f64", f2.GetFilteredSensitivitis(sv, df));
+    return 0;
+}
+
+// ----------------------------------------------------------------- vtk ----
+// The field values of the VTK fixture; tests/facade_vtk_check.cpp writes the
+// same ones through the C++ facade.
+int run_vtk(const std::string& file) {
+    SquareMesh<double> mesh(4.0, 2.0, 4, 2);
+    std::vector<Vector<double> > x = mesh.GenerateNodes();
+    std::vector<std::vector<int> > elements = mesh.GenerateElements();
+    std::vector<Vector<double> > u(x.size(), Vector<double>(2)), r(x.size(), Vector<double>(2));
+    for (size_t n = 0; n < x.size(); n++) {
+        u[n](0) = 1.0e-3 * n - 2.5e-7;
+        u[n](1) = -1.0 / (n + 3.0);
+        r[n](0) = n % 3 == 0 ? 0.0 : 12345.678 / (n + 1.0);
+        r[n](1) = -1.0e-12 * n;
+    }
+    std::vector<double> rho(elements.size());
+    for (size_t e = 0; e < elements.size(); e++) rho[e] = 1.0 / (e + 7.0);
+    std::ofstream fout(file);
+    MakeHeadderToVTK(fout);
+    AddPointsToVTK(x, fout);
+    AddElementToVTK(elements, fout);
+    AddElementTypes(std::vector<int>(elements.size(), 9), fout);
+    AddPointVectors(u, "u", fout, true);
+    AddPointVectors(r, "r", fout, false);
+    AddPointScalers(u, "ux_uy", fout, false);
+    AddElementScalers(rho, "s", fout, true);
+    return 0;
+}
+
 // --------------------------------------------------------------- solid ----
 int run_solid(const std::string& dir, const std::string& out) {
     std::vector<Vector<double> > x;
@@ -582,6 +638,9 @@ int main(int argc, char** argv) {
     if (mode == "operator" && argc >= 6)
         return run_operator(std::atoi(argv[2]), std::atoi(argv[3]), std::strtoull(argv[4], nullptr, 10), argv[5]);
     if (mode == "solid" && argc >= 4) return run_solid(argv[2], argv[3]);
+    if (mode == "vtk" && argc >= 3) return run_vtk(argv[2]);
+    if (mode == "sensfilter" && argc >= 6)
+        return run_sensfilter(std::atoi(argv[2]), std::atoi(argv[3]), std::strtoull(argv[4], nullptr, 10), argv[5]);
     if (mode == "mmatest" && argc >= 3) return run_mmatest(argv[2]);
     if (mode == "time" && argc >= 5)
         return run_time(std::atoi(argv[2]), std::atoi(argv[3]), std::atoi(argv[4]), argc >= 6 ? std::atoi(argv[5]) : 30);

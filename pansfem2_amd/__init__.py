@@ -105,6 +105,7 @@ def lib():
             "pf_filter": ([c_void_p, c_int, _f64p, _f64p], c_int),
             "pf_filter_set_lists": ([c_void_p, _i32p, _i32p, _f64p], c_int),
             "pf_heaviside": ([c_void_p, c_double, c_int, _f64p, c_void_p, _f64p], c_int),
+            "pf_sensitivity_filter": ([c_void_p, c_int, _f64p, _f64p, _f64p], c_int),
             "pf_oc_candidate": ([c_void_p, _f64p, _f64p, _f64p, c_double, c_double, c_double, _f64p], c_int),
             "pf_oc_update": ([c_void_p, _f64p, _f64p, _f64p, P(OcParams), P(c_double), P(c_int)], c_int),
             "pf_mma_init": ([c_void_p, P(MmaParams)], c_int),
@@ -309,6 +310,12 @@ class Context:
     def filter(self, x, mode=PF_FILTER_VARIABLES):
         out = np.zeros(self.ne)
         self._ck(lib().pf_filter(self._h, mode, _f64(x), out))
+        return out
+
+    def sensitivity_filter(self, s, dfds, scheme=1):
+        """SensitivityFilter (scheme 1) / SensitivityFilter2 (scheme 2)."""
+        out = np.zeros(self.ne)
+        self._ck(lib().pf_sensitivity_filter(self._h, scheme, _f64(s), _f64(dfds), out))
         return out
 
     def heaviside(self, beta, s, dfdrho=None):

@@ -1815,6 +1815,28 @@ int pf_filter_setup(pf_ctx* ctx, double R) {
     return PF_OK;
 }
 
+int pf_sensitivity_filter(pf_ctx* ctx, int scheme, const double* s, const double* dfds, double* out) {
+    if (!ctx || !s || !dfds || !out) return PF_ERR_ARG;
+    PF_CHECK(scheme == 1 || scheme == 2, PF_ERR_ARG, "pf_sensitivity_filter: scheme 1 or 2");
+    PF_CHECK(ctx->dim > 0, PF_ERR_STATE, "no grid");
+    PF_CHECK(ctx->have_lists || (ctx->have_filter && is2d(ctx)), PF_ERR_STATE,
+             "pf_sensitivity_filter: needs neighbour lists (pf_filter_set_lists) or the quad4 grid rule");
+    PF_TRY(set_device(ctx));
+    const long ne = ctx->ne;
+    const size_t eb = sizeof(double) * ne;
+    PF_HIP(hipMemcpyAsync(ctx->d_e0, s, eb, hipMemcpyHostToDevice, ctx->stream));
+    PF_HIP(hipMemcpyAsync(ctx->d_e1, dfds, eb, hipMemcpyHostToDevice, ctx->stream));
+    if (ctx->have_lists)
+        k_sensfilter_lists<<<gsize(ne), kBlock, 0, ctx->stream>>>(ne, ctx->d_fptr, ctx->d_fidx, ctx->d_fw, ctx->d_e0,
+                                                                 ctx->d_e1, scheme, ctx->d_e2);
+    else
+        k_sensfilter_q4<<<gsize(ne), kBlock, 0, ctx->stream>>>(ctx->nx, ctx->ny, ctx->win, ctx->R, ctx->d_cg,
+                                                              ctx->d_e0, ctx->d_e1, scheme, ctx->d_e2);
+    PF_TRY(check_launch(ctx));
+    PF_HIP(hipMemcpyAsync(out, ctx->d_e2, eb, hipMemcpyDeviceToHost, ctx->stream));
+    return sync(ctx);
+}
+
 int pf_filter_set_lists(pf_ctx* ctx, const int32_t* ptr, const int32_t* idx, const double* w) {
     if (!ctx || !ptr || !idx || !w) return PF_ERR_ARG;
     PF_CHECK(ctx->dim > 0, PF_ERR_STATE, "no grid");

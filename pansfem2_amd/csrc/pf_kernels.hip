@@ -807,6 +807,51 @@ __global__ void __launch_bounds__(256) k_filter_q4(int nx, int ny, int win, doub
         out[e] = filter_one(nx, ny, win, R, cg, in, e);
 }
 
+// SensitivityFilter<T> (scheme 1, SensitivityFilter.h:44-55):
+//   out_i = sum_j (w_ij s_j) df_j / (sum_j w_ij * s_i);
+// SensitivityFilter2<T> (scheme 2, SensitivityFilter.h:88-99):
+//   out_i = sum_j (w_ij s_j) df_j / sum_j (w_ij s_j);
+// sums from zero in neighbour order, as the reference adds them.
+__device__ inline void sens_filter_add(int scheme, double w, double sj, double dfj, double& acc, double& den) {
+    const double ws = w * sj;
+    acc += ws * dfj;
+    den += scheme == 2 ? ws : w;
+}
+__device__ inline double sens_filter_end(int scheme, double acc, double den, double si) {
+    return scheme == 2 ? acc / den : acc / (den * si);
+}
+
+__global__ void __launch_bounds__(256) k_sensfilter_q4(int nx, int ny, int win, double R,
+                                                       const double* __restrict__ cg, const double* __restrict__ s,
+                                                       const double* __restrict__ df, int scheme,
+                                                       double* __restrict__ out) {
+    const long ne = (long)nx * ny;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < ne; e += (long)gridDim.x * blockDim.x) {
+        int i = (int)(e / ny), j = (int)(e - (long)i * ny);
+        double acc = 0.0, den = 0.0;
+        int i0 = max(i - win, 0), i1 = min(i + win, nx - 1), j0 = max(j - win, 0), j1 = min(j + win, ny - 1);
+        for (int ii = i0; ii <= i1; ii++)
+            for (int jj = j0; jj <= j1; jj++) {
+                long m = (long)ny * ii + jj;
+                double w;
+                if (filter_w(cg, e, m, R, w)) sens_filter_add(scheme, w, s[m], df[m], acc, den);
+            }
+        out[e] = sens_filter_end(scheme, acc, den, s[e]);
+    }
+}
+
+__global__ void __launch_bounds__(256) k_sensfilter_lists(long ne, const int32_t* __restrict__ ptr,
+                                                          const int32_t* __restrict__ idx,
+                                                          const double* __restrict__ w, const double* __restrict__ s,
+                                                          const double* __restrict__ df, int scheme,
+                                                          double* __restrict__ out) {
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < ne; e += (long)gridDim.x * blockDim.x) {
+        double acc = 0.0, den = 0.0;
+        for (int k = ptr[e]; k < ptr[e + 1]; k++) sens_filter_add(scheme, w[k], s[idx[k]], df[idx[k]], acc, den);
+        out[e] = sens_filter_end(scheme, acc, den, s[e]);
+    }
+}
+
 // DensityFilter<T> over caller-given neighbour lists (DensityFilter.h:45-71):
 // out_i = (sum_j w_ij in_{idx_ij}) / (sum_j w_ij), both sums from zero in
 // list order, as GetFilteredVariables / GetFilteredSensitivitis add them.

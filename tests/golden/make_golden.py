@@ -89,6 +89,18 @@ def main():
                   E1=1.0, E0=1e-9, p=3.0, nu=0.3), manifest)
         shutil.rmtree(tmp)
         print("wrote operator", nx, ny, file=sys.stderr)
+    subprocess.run([HARNESS, "vtk", os.path.join(OUT, "vtk_4x2.vtk")], check=True)
+    manifest["vtk_4x2"] = {"meta": dict(source="src/PrePost/Export/ExportToVTK.h", file="vtk_4x2.vtk",
+                                        fields="closed-form u, r (nodal vectors), rho (element scalars)"),
+                           "arrays": {}}
+    print("wrote vtk", file=sys.stderr)
+    tmp = tempfile.mkdtemp()
+    arrays = run(["sensfilter", 24, 12, 20261206, tmp], tmp)
+    pack("sensfilter_24x12", arrays, dict(nx=24, ny=12, R=1.5, seed=20261206, s="iid U[0.01,1] SplitMix64",
+                                          dfds="iid U[-1,0] SplitMix64 (after s)",
+                                          source="src/Optimize/Filter/SensitivityFilter.h"), manifest)
+    shutil.rmtree(tmp)
+    print("wrote sensfilter", file=sys.stderr)
     tmp = tempfile.mkdtemp()
     arrays = run(["solid", os.path.join(REF, "sample", "solid"), tmp], tmp)
     pack("solid_sample", arrays, dict(source="sample/solid/{Node,Element,Dirichlet,Neumann}.csv", E=210000.0, nu=0.3,

@@ -92,3 +92,15 @@ def test_sample_driver_heaviside_against_reference(tmp_path):
     assert np.max(np.abs(hist[:, 1] - g["hist_f"]) / g["hist_f"]) < 1e-9
     assert np.max(np.abs(np.loadtxt(prefix + "_s.txt") - g["s_final"])) < 1e-6
     assert np.max(np.abs(np.loadtxt(prefix + "_rho.txt") - g["rho_final"])) < 1e-6
+
+
+def test_vtk_export_byte_identical_to_reference(tmp_path):
+    """The facade's VTK functions write the same bytes as the reference's
+    exporter (src/PrePost/Export/ExportToVTK.h) for the fixture's fields."""
+    exe = str(tmp_path / "facade_vtk_check")
+    subprocess.run(["g++", "-std=c++17", "-O2", "-I", os.path.join(ROOT, "include"),
+                    os.path.join(ROOT, "tests", "facade_vtk_check.cpp"), "-o", exe], check=True)
+    out = str(tmp_path / "out.vtk")
+    subprocess.run([exe, out], check=True)
+    with open(out, "rb") as a, open(os.path.join(ROOT, "tests", "golden", "vtk_4x2.vtk"), "rb") as b:
+        assert a.read() == b.read()

@@ -280,3 +280,19 @@ def test_simp_heaviside_loop_c1_against_reference(pa):
     r1 = ctx.simp_run(40, pa.simp_default_params(heaviside=1))
     r2 = ctx.simp_run(10, pa.simp_default_params(heaviside=1, iter_offset=40), s0=r1["s"])
     assert np.max(np.abs(r2["s"] - g["s_final"])) < 1e-6
+
+
+def test_sensitivity_filters_against_reference(pa):
+    """SensitivityFilter and SensitivityFilter2 on the device (reference lists
+    and the grid rule) against the reference's own classes, bit for bit."""
+    from oracle import pyoracle as po
+    g = gi.load("sensfilter_24x12")
+    nx, ny = 24, 12
+    ctx = pa.cantilever2d(nx, ny)
+    xy, conn = po.grid2d(nx, ny)
+    fp, fi, fw = po.filter_build(nx, ny, po.centroids(conn, xy))
+    for lists in (False, True):
+        if lists:
+            ctx.filter_set_lists(fp, fi, fw)
+        assert np.array_equal(ctx.sensitivity_filter(g["s"], g["dfds"], 1), g["out1"])
+        assert np.array_equal(ctx.sensitivity_filter(g["s"], g["dfds"], 2), g["out2"])

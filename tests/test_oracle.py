@@ -348,3 +348,19 @@ def test_heaviside_projection_closed_forms():
     assert np.allclose(dfds[interior], grad[interior], rtol=1e-6, atol=1e-9)
     # and a dropped chain-rule factor would not pass
     assert not np.allclose(po.filter_sens(fp, fi, fw, c)[interior], grad[interior], rtol=1e-3)
+
+
+def test_sensitivity_filters_against_reference():
+    """Both sensitivity filters against the reference's own classes on a
+    24x12 grid (golden sensfilter_24x12), and the closed form: with uniform
+    s they reduce to the row-normalised density-filter sensitivity."""
+    g = gi.load("sensfilter_24x12")
+    nx, ny = 24, 12
+    xy, conn = po.grid2d(nx, ny)
+    fp, fi, fw = po.filter_build(nx, ny, po.centroids(conn, xy))
+    assert np.array_equal(po.sensitivity_filter(fp, fi, fw, g["s"], g["dfds"], 1), g["out1"])
+    assert np.array_equal(po.sensitivity_filter(fp, fi, fw, g["s"], g["dfds"], 2), g["out2"])
+    u = np.full(nx * ny, 0.7)
+    want = po.filter_sens(fp, fi, fw, g["dfds"])
+    assert np.allclose(po.sensitivity_filter(fp, fi, fw, u, g["dfds"], 1), want, rtol=1e-14)
+    assert np.allclose(po.sensitivity_filter(fp, fi, fw, u, g["dfds"], 2), want, rtol=1e-14)
