@@ -19,6 +19,7 @@
 //   SensitivityFilter<T>, SensitivityFilter2<T>  src/Optimize/Filter/SensitivityFilter.h:14-101
 //   OC<T>                           src/Optimize/Solver/OC.h:15-107
 //   MMA<T>                          src/Optimize/Solver/MMA.h:15-509
+//   CONLIN<T>                       src/Optimize/Solver/CONLIN.h:18-455
 //   MakeHeadderToVTK, AddPointsToVTK, AddElementToVTK, AddElementTypes,
 //   AddPointScalers, AddPointVectors, AddElementScalers
 //                                   src/PrePost/Export/ExportToVTK.h:18-137
@@ -1143,6 +1144,62 @@ public:
 
 private:
     int n_, m_;
+    pf_mma_params prm_{};
+    bool initialised_ = false;
+    long generation_ = -1;
+    T prev_ = T(), eps_ = 1.0e-5;
+};
+
+// =========================================================== CONLIN ======
+// Fleury's convex linearisation with one constraint on the device (the
+// interior-point subproblem solver is MMA's).
+template <class T>
+class CONLIN {
+public:
+    CONLIN(int n, int m, T a0, std::vector<T> a, std::vector<T> c, std::vector<T> d, const std::vector<T>& xmin,
+           const std::vector<T>& xmax)
+        : n_(n) {
+        if (m != 1) throw DeviceError("CONLIN: the device solver handles one constraint (m = 1)");
+        if ((int)xmin.size() != n || (int)xmax.size() != n) throw DeviceError("CONLIN: bound sizes differ from n");
+        for (int i = 1; i < n; i++)
+            if (xmin[i] != xmin[0] || xmax[i] != xmax[0])
+                throw DeviceError("CONLIN: the device solver takes uniform bounds");
+        prm_.m = 1;
+        prm_.a0 = a0;
+        prm_.a = a.at(0);
+        prm_.c = c.at(0);
+        prm_.d = d.at(0);
+        prm_.xmin = xmin.at(0);
+        prm_.xmax = xmax.at(0);
+        prm_.move = 0.5;  // CONLIN.h:66
+        prm_.epsvalue = 1.0e-5;
+        prm_.max_newton = 1000;
+    }
+    void SetParameters(T move, T epsvalue) {
+        prm_.move = move;
+        prm_.epsvalue = epsvalue;
+        eps_ = epsvalue;
+        initialised_ = false;
+    }
+    bool IsConvergence(T f) { return std::fabs(f - prev_) / (f + prev_) < eps_; }
+    void UpdateVariables(std::vector<T>& xk, T f, std::vector<T> dfdx, std::vector<T> g,
+                         std::vector<std::vector<T> > dgdx) {
+        auto& dev = Device::Get();
+        if (n_ != dev.elements()) throw DeviceError("CONLIN: n differs from the registered grid's elements");
+        if (g.size() != 1 || dgdx.size() != 1) throw DeviceError("CONLIN: one constraint expected");
+        if (!initialised_ || dev.generation() != generation_) {
+            dev.check(pf_conlin_init(dev.ctx(), &prm_), "pf_conlin_init");
+            initialised_ = true;
+            generation_ = dev.generation();
+        }
+        int steps = 0;
+        dev.check(pf_conlin_update(dev.ctx(), xk.data(), f, dfdx.data(), g[0], dgdx[0].data(), &steps),
+                  "pf_conlin_update");
+        prev_ = f;
+    }
+
+private:
+    int n_;
     pf_mma_params prm_{};
     bool initialised_ = false;
     long generation_ = -1;

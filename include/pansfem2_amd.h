@@ -77,7 +77,8 @@ typedef struct pf_simp_params {
     double E1, E0, nu, t, p;       /* SIMP: E = E1 rho^p + E0 (1 - rho^p) (sample...oc.cpp:123) */
     double vf, R, scale0, scale1;  /* volume fraction, filter radius, objective/constraint scales */
     double cg_eps; int cg_itrmax;  /* ScalingCG(K, F, itrmax, eps) (sample...oc.cpp:132) */
-    int use_mma;                   /* 0: OC update, 1: MMA update */
+    int use_mma;                   /* 0: OC update, 1: MMA update, 2: CONLIN update (uses mma's a0, a, c, d,
+                                      bounds, move, epsvalue) */
     int solver;                    /* PF_OP_EBE64 (plain FP64) or PF_OP_CSR64 */
     int mixed;                     /* 1: FP32 EbE inner PCG inside FP64 refinement */
     double inner_eps;              /* inner tolerance of the mixed solve */
@@ -227,6 +228,13 @@ int pf_oc_candidate(pf_ctx* ctx, const double* x, const double* dfdx, const doub
 int pf_mma_init(pf_ctx* ctx, const pf_mma_params* prm);
 int pf_mma_update(pf_ctx* ctx, double* x, double f, const double* dfdx, double g, const double* dgdx,
                   int* newton_steps);
+/* CONLIN<T> (src/Optimize/Solver/CONLIN.h:18-455), m = 1, state in the
+ * context: pf_conlin_init takes the ctor's a0, a, c, d, xmin, xmax and
+ * SetParameters' move and epsvalue from prm (the MMA-only fields are
+ * unused); pf_conlin_update runs UpdateVariables(x, f, dfdx, {g}, {dgdx}). */
+int pf_conlin_init(pf_ctx* ctx, const pf_mma_params* prm);
+int pf_conlin_update(pf_ctx* ctx, double* x, double f, const double* dfdx, double g, const double* dgdx,
+                     int* newton_steps);
 
 /* ---- the whole loop ---------------------------------------------------- */
 /* The SIMP loop of sample_optimize_density_oc.cpp:83-208 (or the MMA

@@ -860,6 +860,262 @@ void orc_mma_update(orc_mma* s, double* xk, double f, const double* dfdx, const 
 }
 
 /* ----------------------------------------------------------- SIMP loop -- */
+/* -------------------------------------------------------------- CONLIN -- */
+/* CONLIN<T> (src/Optimize/Solver/CONLIN.h:18-455): the objective and each
+ * constraint are linearised in x where their derivative is positive and in
+ * 1/x where it is negative; the separable subproblem is solved by the same
+ * primal-dual interior-point iteration as MMA's. */
+orc_conlin* orc_conlin_create(int n, int m, double a0, const double* a, const double* c, const double* d,
+                              const double* xmin, const double* xmax) {
+    orc_conlin* s = (orc_conlin*)calloc(1, sizeof(orc_conlin));
+    s->n = n; s->m = m; s->k = 0;
+    s->previousvalue = 0.0; s->epsvalue = 1.0e-5; s->move = 0.5;
+    s->a0 = a0;
+    s->a = dalloc(m); s->c = dalloc(m); s->d = dalloc(m);
+    memcpy(s->a, a, sizeof(double) * m); memcpy(s->c, c, sizeof(double) * m); memcpy(s->d, d, sizeof(double) * m);
+    s->xmin = dalloc(n); s->xmax = dalloc(n);
+    memcpy(s->xmin, xmin, sizeof(double) * n); memcpy(s->xmax, xmax, sizeof(double) * n);
+    return s;
+}
+
+void orc_conlin_set_parameters(orc_conlin* s, double move, double epsvalue) { s->move = move; s->epsvalue = epsvalue; }
+
+int orc_conlin_is_convergence(orc_conlin* s, double f) {
+    return fabs(f - s->previousvalue) / (f + s->previousvalue) < s->epsvalue;
+}
+
+void orc_conlin_destroy(orc_conlin* s) {
+    if (!s) return;
+    free(s->a); free(s->c); free(s->d); free(s->xmin); free(s->xmax);
+    free(s);
+}
+
+/* CONLIN<T>::KKTNorm (CONLIN.h:371-413) */
+static double conlin_kktnorm(const orc_conlin* s, const double* p, const double* q, const double* p0,
+                             const double* q0, const double* alpha, const double* beta, const double* b,
+                             const double* x, const double* y, double z, const double* lambda, const double* gsi,
+                             const double* ita, const double* mu, double zeta, const double* sv, double eps) {
+    const int n = s->n, m = s->m;
+    double norm = 0.0;
+    double* g = dalloc(m);
+    double* pl = dalloc(n);
+    double* ql = dalloc(n);
+    for (int j = 0; j < n; j++) {
+        pl[j] = p0[j];
+        ql[j] = q0[j];
+        for (int i = 0; i < m; i++) {
+            pl[j] += lambda[i] * p[(long)i * n + j];
+            ql[j] += lambda[i] * q[(long)i * n + j];
+            g[i] += p[(long)i * n + j] * x[j] + q[(long)i * n + j] / x[j];
+        }
+    }
+    for (int j = 0; j < n; j++) {
+        norm += pow2(pl[j] - ql[j] / pow2(x[j]) - gsi[j] + ita[j]);
+        norm += pow2(gsi[j] * (x[j] - alpha[j]) - eps);
+        norm += pow2(ita[j] * (beta[j] - x[j]) - eps);
+    }
+    for (int i = 0; i < m; i++) {
+        norm += pow2(s->c[i] + s->d[i] * y[i] - lambda[i] - mu[i]);
+        norm += pow2(g[i] - s->a[i] * z - y[i] + sv[i] - b[i]);
+        norm += pow2(mu[i] * y[i] - eps);
+        norm += pow2(lambda[i] * sv[i] - eps);
+    }
+    double la = 0.0;
+    for (int i = 0; i < m; i++) la = la + lambda[i] * s->a[i];
+    norm += pow2(s->a0 - zeta - la);
+    norm += pow2(zeta * z - eps);
+    free(g); free(pl); free(ql);
+    return sqrt(norm);
+}
+
+/* CONLIN<T>::UpdateVariables (CONLIN.h:90-365) */
+void orc_conlin_update(orc_conlin* s, double* xk, double f, const double* dfdx, const double* gv, const double* dgdx) {
+    const int n = s->n, m = s->m;
+    const double *xmin = s->xmin, *xmax = s->xmax;
+    double* alpha = dalloc(n);
+    double* beta = dalloc(n);
+    for (int j = 0; j < n; j++) {
+        alpha[j] = std_max(xmin[j], xk[j] - s->move * (xmax[j] - xmin[j]));
+        beta[j] = std_min(xmax[j], xk[j] + s->move * (xmax[j] - xmin[j]));
+    }
+    double* p0 = dalloc(n);
+    double* q0 = dalloc(n);
+    for (int j = 0; j < n; j++) {
+        p0[j] = std_max(dfdx[j], 0.0);
+        q0[j] = std_max(-dfdx[j], 0.0) * pow2(xk[j]);
+    }
+    double* p = dalloc((long)m * n);
+    double* q = dalloc((long)m * n);
+    double* b = dalloc(m);
+    for (int i = 0; i < m; i++) {
+        b[i] = -gv[i];
+        for (int j = 0; j < n; j++) {
+            p[(long)i * n + j] = std_max(dgdx[(long)i * n + j], 0.0);
+            q[(long)i * n + j] = std_max(-dgdx[(long)i * n + j], 0.0) * pow2(xk[j]);
+            b[i] += p[(long)i * n + j] * xk[j] + q[(long)i * n + j] / xk[j];
+        }
+    }
+    double eps = 1.0;
+    double* x = dalloc(n);
+    double* y = dalloc(m);
+    double z = 1.0, zeta = 1.0;
+    double* lambda = dalloc(m);
+    double* sv = dalloc(m);
+    double* gsi = dalloc(n);
+    double* ita = dalloc(n);
+    double* mu = dalloc(m);
+    for (int i = 0; i < m; i++) { y[i] = 1.0; lambda[i] = 1.0; sv[i] = 1.0; mu[i] = std_max(1.0, 0.5 * s->c[i]); }
+    for (int j = 0; j < n; j++) {
+        x[j] = 0.5 * (alpha[j] + beta[j]);
+        gsi[j] = std_max(1.0, 1.0 / (x[j] - alpha[j]));
+        ita[j] = std_max(1.0, 1.0 / (beta[j] - x[j]));
+    }
+    double *pl = dalloc(n), *ql = dalloc(n), *G = dalloc((long)m * n), *Dx = dalloc(n), *dlx = dalloc(n);
+    double *Dy = dalloc(m), *Dl = dalloc(m), *dly = dalloc(m), *dll = dalloc(m), *Dlly = dalloc(m), *dlly = dalloc(m);
+    double *dx = dalloc(n), *dy = dalloc(m), *dlam = dalloc(m), *dgsi = dalloc(n), *dita = dalloc(n), *dmu = dalloc(m), *ds = dalloc(m);
+    double *xp = dalloc(n), *yp = dalloc(m), *lp = dalloc(m), *gp = dalloc(n), *ip = dalloc(n), *mp = dalloc(m), *sp = dalloc(m);
+    int steps = 0;
+    for (int l = 0; eps > 1.0e-7; l++) {
+        steps++;
+        for (int j = 0; j < n; j++) {
+            pl[j] = p0[j];
+            ql[j] = q0[j];
+            for (int i = 0; i < m; i++) { pl[j] += lambda[i] * p[(long)i * n + j]; ql[j] += lambda[i] * q[(long)i * n + j]; }
+        }
+        for (int i = 0; i < m; i++)
+            for (int j = 0; j < n; j++) G[(long)i * n + j] = p[(long)i * n + j] - q[(long)i * n + j] / pow2(x[j]);
+        for (int j = 0; j < n; j++) {
+            Dx[j] = 2.0 * ql[j] / pow(x[j], 3.0) + gsi[j] / (x[j] - alpha[j]) + ita[j] / (beta[j] - x[j]);
+            dlx[j] = pl[j] - ql[j] / pow2(x[j]) - eps / (x[j] - alpha[j]) + eps / (beta[j] - x[j]);
+        }
+        for (int i = 0; i < m; i++) {
+            Dy[i] = s->d[i] + mu[i] / y[i];
+            Dl[i] = sv[i] / lambda[i];
+            dly[i] = s->c[i] + s->d[i] * y[i] - lambda[i] - eps / y[i];
+        }
+        double la = 0.0;
+        for (int i = 0; i < m; i++) la = la + lambda[i] * s->a[i];
+        double dlz = s->a0 - eps / z - la;
+        for (int i = 0; i < m; i++) {
+            dll[i] = -s->a[i] * z - y[i] - b[i] + eps / lambda[i];
+            for (int j = 0; j < n; j++) dll[i] += p[(long)i * n + j] * x[j] + q[(long)i * n + j] / x[j];
+            Dlly[i] = Dl[i] + 1.0 / Dy[i];
+            dlly[i] = dll[i] + dly[i] / Dy[i];
+        }
+        double dz = 0.0, dzeta = 0.0;
+        for (int i = 0; i < m; i++) { dy[i] = 0.0; dlam[i] = 0.0; dmu[i] = 0.0; ds[i] = 0.0; }
+        for (int j = 0; j < n; j++) { dx[j] = 0.0; dgsi[j] = 0.0; dita[j] = 0.0; }
+        if (n > m) {
+            int nb = m + 1;
+            double* A = dalloc((long)nb * nb);
+            double* B = dalloc(nb);
+            double* sol = dalloc(nb);
+            for (int ii = 0; ii < m; ii++) {
+                for (int jj = 0; jj < m; jj++)
+                    for (int kk = 0; kk < n; kk++) A[ii * nb + jj] += G[(long)ii * n + kk] * G[(long)jj * n + kk] / Dx[kk];
+                A[ii * nb + ii] += Dlly[ii];
+                A[ii * nb + m] = s->a[ii];
+                A[m * nb + ii] = s->a[ii];
+            }
+            A[m * nb + m] = -zeta / z;
+            for (int ii = 0; ii < m; ii++) {
+                B[ii] = dlly[ii];
+                for (int jj = 0; jj < n; jj++) B[ii] -= G[(long)ii * n + jj] * dlx[jj] / Dx[jj];
+            }
+            B[m] = dlz;
+            mma_solvels(nb, A, B, sol);
+            for (int i = 0; i < m; i++) dlam[i] = sol[i];
+            dz = sol[m];
+            for (int j = 0; j < n; j++) {
+                dx[j] = -dlx[j] / Dx[j];
+                for (int i = 0; i < m; i++) dx[j] -= G[(long)i * n + j] * dlam[i] / Dx[j];
+            }
+            free(A); free(B); free(sol);
+        } else {
+            int nb = n + 1;
+            double* A = dalloc((long)nb * nb);
+            double* B = dalloc(nb);
+            double* sol = dalloc(nb);
+            for (int ii = 0; ii < n; ii++) {
+                for (int jj = 0; jj < n; jj++)
+                    for (int kk = 0; kk < m; kk++) A[ii * nb + jj] += G[(long)kk * n + ii] * G[(long)kk * n + jj] / Dlly[kk];
+                A[ii * nb + ii] += Dx[ii];
+                for (int jj = 0; jj < m; jj++) {
+                    A[ii * nb + n] -= G[(long)jj * n + ii] * s->a[jj] / Dlly[jj];
+                    A[n * nb + ii] -= G[(long)jj * n + ii] * s->a[jj] / Dlly[jj];
+                    A[n * nb + n] += s->a[jj] * s->a[jj] / Dlly[jj];
+                }
+            }
+            A[n * nb + n] += zeta / z;
+            for (int ii = 0; ii < n; ii++) {
+                B[ii] = -dlx[ii];
+                for (int jj = 0; jj < m; jj++) B[ii] -= G[(long)jj * n + ii] * dlly[jj] / Dlly[jj];
+            }
+            B[n] = -dlz;
+            for (int jj = 0; jj < m; jj++) B[n] += s->a[jj] * dlly[jj] / Dlly[jj];
+            mma_solvels(nb, A, B, sol);
+            for (int j = 0; j < n; j++) dx[j] = sol[j];
+            dz = sol[n];
+            for (int i = 0; i < m; i++) {
+                dlam[i] = -s->a[i] * dz / Dlly[i] + dlly[i] / Dlly[i];
+                for (int j = 0; j < n; j++) dlam[i] += G[(long)i * n + j] * dx[j] / Dlly[i];
+            }
+            free(A); free(B); free(sol);
+        }
+        for (int i = 0; i < m; i++) {
+            dy[i] = dlam[i] / Dy[i] - dly[i] / Dy[i];
+            dmu[i] = -mu[i] * dy[i] / y[i] - mu[i] + eps / y[i];
+            ds[i] = -sv[i] * dlam[i] / lambda[i] - sv[i] + eps / lambda[i];
+        }
+        for (int j = 0; j < n; j++) {
+            dgsi[j] = -gsi[j] * dx[j] / (x[j] - alpha[j]) - gsi[j] + eps / (x[j] - alpha[j]);
+            dita[j] = ita[j] * dx[j] / (beta[j] - x[j]) - ita[j] + eps / (beta[j] - x[j]);
+        }
+        dzeta = -zeta * dz / z - zeta + eps / z;
+        double txmax = 0.0;
+        for (int j = 0; j < n; j++) {
+            double t = max4(-1.01 * dx[j] / (x[j] - alpha[j]), 1.01 * dx[j] / (beta[j] - x[j]), -1.01 * dgsi[j] / gsi[j],
+                            -1.01 * dita[j] / ita[j]);
+            if (txmax < t) txmax = t;
+        }
+        double tymax = 0.0;
+        for (int i = 0; i < m; i++) {
+            double t = max4(-1.01 * dy[i] / y[i], -1.01 * dlam[i] / lambda[i], -1.01 * dmu[i] / mu[i], -1.01 * ds[i] / sv[i]);
+            if (tymax < t) tymax = t;
+        }
+        double tau = 1.0 / max5(1.0, txmax, tymax, -1.01 * dz / z, -1.01 * dzeta / zeta);
+        double dw = conlin_kktnorm(s, p, q, p0, q0, alpha, beta, b, x, y, z, lambda, gsi, ita, mu, zeta, sv, eps);
+        double zp = 0.0, zetap = 0.0;
+        for (int ll = 0; ll < 50; ll++) {
+            for (int j = 0; j < n; j++) { xp[j] = x[j] + tau * dx[j]; gp[j] = gsi[j] + tau * dgsi[j]; ip[j] = ita[j] + tau * dita[j]; }
+            for (int i = 0; i < m; i++) {
+                yp[i] = y[i] + tau * dy[i]; lp[i] = lambda[i] + tau * dlam[i];
+                mp[i] = mu[i] + tau * dmu[i]; sp[i] = sv[i] + tau * ds[i];
+            }
+            zp = z + tau * dz;
+            zetap = zeta + tau * dzeta;
+            double dw1 = conlin_kktnorm(s, p, q, p0, q0, alpha, beta, b, xp, yp, zp, lp, gp, ip, mp, zetap, sp, eps);
+            if (dw1 < dw) break;
+            tau *= 0.5;
+        }
+        memcpy(x, xp, sizeof(double) * n); memcpy(y, yp, sizeof(double) * m); z = zp;
+        memcpy(lambda, lp, sizeof(double) * m); memcpy(gsi, gp, sizeof(double) * n); memcpy(ita, ip, sizeof(double) * n);
+        memcpy(mu, mp, sizeof(double) * m); zeta = zetap; memcpy(sv, sp, sizeof(double) * m);
+        double dw2 = conlin_kktnorm(s, p, q, p0, q0, alpha, beta, b, x, y, z, lambda, gsi, ita, mu, zeta, sv, eps);
+        if (dw2 < 0.9 * eps) eps *= 0.1;
+        else eps *= 1.0;
+    }
+    s->previousvalue = f;
+    s->k++;
+    s->newton_steps = steps;
+    memcpy(xk, x, sizeof(double) * n);
+    free(alpha); free(beta); free(p0); free(q0); free(p); free(q); free(b);
+    free(x); free(y); free(lambda); free(sv); free(gsi); free(ita); free(mu);
+    free(pl); free(ql); free(G); free(Dx); free(dlx); free(Dy); free(Dl); free(dly); free(dll); free(Dlly); free(dlly);
+    free(dx); free(dy); free(dlam); free(dgsi); free(dita); free(dmu); free(ds);
+    free(xp); free(yp); free(lp); free(gp); free(ip); free(mp); free(sp);
+}
+
 void orc_simp_default_params(orc_simp_params* p) {
     p->E1 = 1.0; p->E0 = 1.0e-9; p->nu = 0.3; p->t = 1.0; p->p = 3.0; p->vf = 0.5; p->R = 1.5;
     p->scale0 = 1.0; p->scale1 = 1.0; p->cg_eps = 1.0e-10; p->cg_itrmax = 100000; p->use_mma = 0;
@@ -908,7 +1164,16 @@ int orc_simp2d(int nx, int ny, int iters, const orc_simp_params* P, double* s, d
         orc_ke_planestrain_q4(X, 1.0, P->nu, P->t, ke1 + 64L * e);
     }
     orc_mma* mma = NULL;
-    if (P->use_mma) {
+    orc_conlin* conlin = NULL;
+    if (P->use_mma == 2) {
+        /* CONLIN with the CONLIN sample's constants (sample_optimize_density_CONLIN.cpp:80-85) */
+        double a = 0.0, c = P->mma_c, d = 0.0;
+        double *xmin = dalloc(ne), *xmax = dalloc(ne);
+        for (int i = 0; i < ne; i++) { xmin[i] = P->mma_xmin; xmax[i] = P->mma_xmax; }
+        conlin = orc_conlin_create(ne, 1, P->mma_a0, &a, &c, &d, xmin, xmax);
+        orc_conlin_set_parameters(conlin, P->mma_move, P->mma_eps);
+        free(xmin); free(xmax);
+    } else if (P->use_mma) {
         double a = 0.0, c = P->mma_c, d = 0.0;
         double *xmin = dalloc(ne), *xmax = dalloc(ne);
         for (int i = 0; i < ne; i++) { xmin[i] = P->mma_xmin; xmax[i] = P->mma_xmax; }
@@ -983,6 +1248,8 @@ int orc_simp2d(int nx, int ny, int iters, const orc_simp_params* P, double* s, d
         if (!P->use_mma) {
             orc_oc_update_beta(ne, s, dfds, dgds, P->oc_iota, P->oc_lmin, P->oc_lmax, P->oc_leps, P->oc_move, fptr, fidx,
                                fw, P->vf, P->scale1, P->heaviside ? beta : 0.0, &lam, NULL, 0);
+        } else if (conlin) {
+            orc_conlin_update(conlin, s, f, dfds, &g, dgds);
         } else {
             orc_mma_update(mma, s, f, dfds, &g, dgds);
         }
@@ -993,6 +1260,7 @@ int orc_simp2d(int nx, int ny, int iters, const orc_simp_params* P, double* s, d
         else orc_filter_apply(ne, fptr, fidx, fw, s, rho_out);
     }
     orc_mma_destroy(mma);
+    orc_conlin_destroy(conlin);
     free(xy); free(conn); free(fn); free(fd); free(n2g); free(cg); free(fptr); free(fidx); free(fw);
     free(rho); free(dgdrho); free(dfdrho); free(dfds); free(dgds); free(ke_all); free(F); free(ufix); free(uf); free(u);
     free(rf); free(ke1);

@@ -118,6 +118,21 @@ int orc_mma_is_convergence(orc_mma* s, double f);
 void orc_mma_update(orc_mma* s, double* xk, double f, const double* dfdx, const double* g, const double* dgdx);
 void orc_mma_destroy(orc_mma* s);
 
+/* CONLIN<T> state (src/Optimize/Solver/CONLIN.h:18-68) and its calls
+ * (SetParameters :71-74, IsConvergence :82-87, UpdateVariables :90-365). */
+typedef struct orc_conlin {
+    int n, m, k;
+    double previousvalue, epsvalue, move, a0;
+    double *a, *c, *d, *xmin, *xmax;
+    int newton_steps;
+} orc_conlin;
+orc_conlin* orc_conlin_create(int n, int m, double a0, const double* a, const double* c, const double* d,
+                              const double* xmin, const double* xmax);
+void orc_conlin_set_parameters(orc_conlin* s, double move, double epsvalue);
+int orc_conlin_is_convergence(orc_conlin* s, double f);
+void orc_conlin_update(orc_conlin* s, double* xk, double f, const double* dfdx, const double* g, const double* dgdx);
+void orc_conlin_destroy(orc_conlin* s);
+
 /* The SIMP loop of the sample driver (sample_optimize_density_oc.cpp:83-208,
  * sample_optimize_density_mma.cpp:88-204) on an nx x ny grid of unit quad4
  * elements with the density filter, left edge clamped and a unit downward
@@ -127,7 +142,7 @@ void orc_mma_destroy(orc_mma* s);
 typedef struct orc_simp_params {
     double E1, E0, nu, t, p, vf, R, scale0, scale1;
     double cg_eps; int cg_itrmax;
-    int use_mma;
+    int use_mma;  /* 0: OC, 1: MMA, 2: CONLIN (with the mma_* constants it shares: a0, c, bounds, move, eps) */
     double oc_iota, oc_lmin, oc_lmax, oc_leps, oc_move;
     double mma_a0, mma_c, mma_xmin, mma_xmax, mma_raa0, mma_albefa, mma_move, mma_asyinit, mma_asydecr, mma_asyincr, mma_eps;
     /* HeavisideFilter instead of DensityFilter, with beta multiplied by

@@ -10,10 +10,12 @@
 // tests/golden/make_golden.py packs into fixtures.
 //
 // Modes
-//   simp <nx> <ny> <iters> <oc|mma|ocH|mmaH> <outdir> [nsave]
+//   simp <nx> <ny> <iters> <oc|mma|conlin|ocH|mmaH|conlinH> <outdir> [nsave]
 //       plane-strain SIMP compliance loop (DensityFilter R=1.5, E1=1,
 //       E0=1e-9, nu=0.3, t=1, p=3, vf=0.5, s0=0.5, left edge clamped, unit
 //       downward load at the bottom-right node, ScalingCG(100000, 1e-10)).
+//       conlin: CONLIN with the CONLIN sample's constants (a0 1, a 0, c 1e4,
+//       d 0, bounds [0.01, 1], move 0.2, eps 1e-6).
 //       The H variants use HeavisideFilter with the density samples' beta
 //       continuation (beta = 0.5, doubled at every 40th iteration from k = 0,
 //       sample_optimize_density_oc.cpp:78,89-93).
@@ -63,6 +65,7 @@
 #include "LinearAlgebra/Solvers/CG.h"
 #include "Optimize/Solver/OC.h"
 #include "Optimize/Solver/MMA.h"
+#include "Optimize/Solver/CONLIN.h"
 #include "Optimize/Filter/DensityFilter.h"
 #include "Optimize/Filter/HeavisideFilter.h"
 #include "Optimize/Filter/SensitivityFilter.h"
@@ -204,7 +207,8 @@ struct LoopFilter {
     }
 };
 
-int run_simp(int nx, int ny, int iters, bool use_mma, bool heaviside, const std::string& out, int nsave) {
+int run_simp(int nx, int ny, int iters, int optimizer, bool heaviside, const std::string& out, int nsave) {
+    const bool use_mma = optimizer != 0;
     Problem2D P = make_problem(nx, ny);
     const int ne = (int)P.elements.size();
     const double R = 1.5, E0 = 1.0e-9, E1 = 1.0, p = 3.0, weightlimit = 0.5, scale0 = 1.0, scale1 = 1.0;
@@ -237,6 +241,10 @@ int run_simp(int nx, int ny, int iters, bool use_mma, bool heaviside, const std:
     MMA<double> mma = MMA<double>(ne, 1, 1.0, std::vector<double>(1, 0.0), std::vector<double>(1, 10000.0),
                                   std::vector<double>(1, 0.0), std::vector<double>(ne, 0.01), std::vector<double>(ne, 1.0));
     mma.SetParameters(1.0e-5, 0.1, 0.2, 0.5, 0.7, 1.2, 1.0e-6);
+    CONLIN<double> conlin = CONLIN<double>(ne, 1, 1.0, std::vector<double>(1, 0.0), std::vector<double>(1, 10000.0),
+                                           std::vector<double>(1, 0.0), std::vector<double>(ne, 0.01),
+                                           std::vector<double>(ne, 1.0));
+    conlin.SetParameters(0.2, 1.0e-6);
 
     std::vector<double> hist_f, hist_g, hist_lambda, hist_conv, hist_nbis;
     std::vector<int32_t> hist_decisions;  // bisection decisions (1: g>0) of every OC update, flattened
@@ -350,6 +358,10 @@ int run_simp(int nx, int ny, int iters, bool use_mma, bool heaviside, const std:
             std::cout.rdbuf(old);
             hist_lambda.push_back(std::stod(cap.str()));
             hist_nbis.push_back(nbis);
+        } else if (optimizer == 2) {
+            conlin.UpdateVariables(s, f, dfds, {g}, {dgds});
+            hist_lambda.push_back(0.0);
+            hist_nbis.push_back(0);
         } else {
             mma.UpdateVariables(s, f, dfds, {g}, {dgds});
             hist_lambda.push_back(0.0);
@@ -631,10 +643,12 @@ int run_time(int nx, int ny, int cg_itrmax, int oc_cap) {
 int main(int argc, char** argv) {
     if (argc < 2) { std::fprintf(stderr, "usage: ref_harness <mode> ...\n"); return 1; }
     std::string mode = argv[1];
-    if (mode == "simp" && argc >= 7)
-        return run_simp(std::atoi(argv[2]), std::atoi(argv[3]), std::atoi(argv[4]),
-                        std::string(argv[5]) == "mma" || std::string(argv[5]) == "mmaH",
-                        std::string(argv[5]).back() == 'H', argv[6], argc >= 8 ? std::atoi(argv[7]) : 0);
+    if (mode == "simp" && argc >= 7) {
+        const std::string opt = argv[5];
+        const int optimizer = opt.rfind("conlin", 0) == 0 ? 2 : (opt.rfind("mma", 0) == 0 ? 1 : 0);
+        return run_simp(std::atoi(argv[2]), std::atoi(argv[3]), std::atoi(argv[4]), optimizer, opt.back() == 'H',
+                        argv[6], argc >= 8 ? std::atoi(argv[7]) : 0);
+    }
     if (mode == "operator" && argc >= 6)
         return run_operator(std::atoi(argv[2]), std::atoi(argv[3]), std::strtoull(argv[4], nullptr, 10), argv[5]);
     if (mode == "solid" && argc >= 4) return run_solid(argv[2], argv[3]);

@@ -110,6 +110,8 @@ def lib():
             "pf_oc_update": ([c_void_p, _f64p, _f64p, _f64p, P(OcParams), P(c_double), P(c_int)], c_int),
             "pf_mma_init": ([c_void_p, P(MmaParams)], c_int),
             "pf_mma_update": ([c_void_p, _f64p, c_double, _f64p, c_double, _f64p, P(c_int)], c_int),
+            "pf_conlin_init": ([c_void_p, P(MmaParams)], c_int),
+            "pf_conlin_update": ([c_void_p, _f64p, c_double, _f64p, c_double, _f64p, P(c_int)], c_int),
             "pf_simp_run": ([c_void_p, P(SimpParams), c_int, _f64p, c_void_p, P(SimpHistory)], c_int),
             "pf_simp_default_params": ([P(SimpParams)], None),
             "pf_profile_kernels": ([c_void_p, c_int, c_int, _f64p], c_int),
@@ -349,6 +351,19 @@ class Context:
         x = np.array(x, dtype=np.float64)
         steps = ctypes.c_int(0)
         self._ck(lib().pf_mma_update(self._h, x, f, _f64(dfdx), g, _f64(dgdx), ctypes.byref(steps)))
+        return x, steps.value
+
+    def conlin_init(self, **kw):
+        """CONLIN<T>(n, 1, a0, a, c, d, xmin, xmax) + SetParameters(move, epsvalue)."""
+        p = simp_default_params().mma
+        for k, v in kw.items():
+            setattr(p, k, v)
+        self._ck(lib().pf_conlin_init(self._h, ctypes.byref(p)))
+
+    def conlin_update(self, x, f, dfdx, g, dgdx):
+        x = np.array(x, dtype=np.float64)
+        steps = ctypes.c_int(0)
+        self._ck(lib().pf_conlin_update(self._h, x, f, _f64(dfdx), g, _f64(dgdx), ctypes.byref(steps)))
         return x, steps.value
 
     def profile_kernels(self, op=PF_OP_EBE64, reps=50):

@@ -1226,6 +1226,7 @@ int comp_sens_device(pf_ctx* ctx, const double* u, double scale0, double* dfdrho
 
 struct MmaState {
     pf_mma_params prm{};
+    bool conlin = false;  // CONLIN's approximation (same interior-point driver)
     int k = 0;
     double previousvalue = 0.0;
     long n = 0;
@@ -1302,9 +1303,13 @@ int mma_update_device(pf_ctx* ctx, double* xk, double f, const double* dfdx, dou
     const double a = P.a, c = P.c, d = P.d, a0 = P.a0;
     const unsigned gg = gsize(n);
     double* part = ctx->red.partials;
-    k_mma_prep<<<gg, kBlock, 0, ctx->stream>>>(n, M->k < 2, xk, M->xkm1, M->xkm2, M->L, M->U, P.xmin, P.xmax,
-                                              P.asyinit, P.asydecr, P.asyincr, P.albefa, P.move, P.raa0, dfdx, dgdx,
-                                              M->alpha, M->beta, M->p0, M->q0, M->p, M->q, part);
+    if (M->conlin)
+        k_conlin_prep<<<gg, kBlock, 0, ctx->stream>>>(n, xk, P.xmin, P.xmax, P.move, dfdx, dgdx, M->alpha, M->beta,
+                                                     M->p0, M->q0, M->p, M->q, part);
+    else
+        k_mma_prep<<<gg, kBlock, 0, ctx->stream>>>(n, M->k < 2, xk, M->xkm1, M->xkm2, M->L, M->U, P.xmin, P.xmax,
+                                                  P.asyinit, P.asydecr, P.asyincr, P.albefa, P.move, P.raa0, dfdx,
+                                                  dgdx, M->alpha, M->beta, M->p0, M->q0, M->p, M->q, part);
     PF_TRY(check_launch(ctx));
     double Sb;
     PF_TRY(host_partials(ctx, part, (int)gg, 1, &Sb));
@@ -1328,12 +1333,18 @@ int mma_update_device(pf_ctx* ctx, double* xk, double f, const double* dfdx, dou
     const int max_newton = P.max_newton > 0 ? P.max_newton : 500;
     for (; eps > 1.0e-7; l++) {
         if (l >= max_newton) {
-            ctx->err = "MMA interior-point loop reached max_newton";
+            ctx->err = M->conlin ? "CONLIN interior-point loop reached max_newton"
+                                 : "MMA interior-point loop reached max_newton";
             if (steps_out) *steps_out = l;
             return PF_NOT_CONVERGED;
         }
-        k_mma_newton_a<<<gg, kBlock, 0, ctx->stream>>>(n, lam, eps, M->x, M->gsi, M->ita, M->alpha, M->beta, M->L,
-                                                      M->U, M->p0, M->q0, M->p, M->q, M->G, M->Dx, M->dlx, part);
+        if (M->conlin)
+            k_conlin_newton_a<<<gg, kBlock, 0, ctx->stream>>>(n, lam, eps, M->x, M->gsi, M->ita, M->alpha, M->beta,
+                                                             M->p0, M->q0, M->p, M->q, M->G, M->Dx, M->dlx, part);
+        else
+            k_mma_newton_a<<<gg, kBlock, 0, ctx->stream>>>(n, lam, eps, M->x, M->gsi, M->ita, M->alpha, M->beta,
+                                                          M->L, M->U, M->p0, M->q0, M->p, M->q, M->G, M->Dx, M->dlx,
+                                                          part);
         PF_TRY(check_launch(ctx));
         double S[4];
         PF_TRY(host_partials(ctx, part, (int)gg, 4, S));
@@ -1378,9 +1389,14 @@ int mma_update_device(pf_ctx* ctx, double* xk, double f, const double* dfdx, dou
             sp = s + tau * ds;
             zp = z + tau * dz;
             zetap = zeta + tau * dzeta;
-            k_mma_trial<<<gg, kBlock, 0, ctx->stream>>>(n, tau, lp, eps, M->x, M->gsi, M->ita, M->dx, M->dgsi,
-                                                       M->dita, M->alpha, M->beta, M->L, M->U, M->p0, M->q0, M->p,
-                                                       M->q, M->xp, M->gp, M->ip, part);
+            if (M->conlin)
+                k_conlin_trial<<<gg, kBlock, 0, ctx->stream>>>(n, tau, lp, eps, M->x, M->gsi, M->ita, M->dx,
+                                                              M->dgsi, M->dita, M->alpha, M->beta, M->p0, M->q0,
+                                                              M->p, M->q, M->xp, M->gp, M->ip, part);
+            else
+                k_mma_trial<<<gg, kBlock, 0, ctx->stream>>>(n, tau, lp, eps, M->x, M->gsi, M->ita, M->dx, M->dgsi,
+                                                           M->dita, M->alpha, M->beta, M->L, M->U, M->p0, M->q0,
+                                                           M->p, M->q, M->xp, M->gp, M->ip, part);
             PF_TRY(check_launch(ctx));
             double T[2];
             PF_TRY(host_partials(ctx, part, (int)gg, 2, T));
@@ -1406,7 +1422,7 @@ int mma_update_device(pf_ctx* ctx, double* xk, double f, const double* dfdx, dou
     return PF_OK;
 }
 
-int mma_init_device(pf_ctx* ctx, const pf_mma_params* prm) {
+int mma_init_device(pf_ctx* ctx, const pf_mma_params* prm, bool conlin = false) {
     PF_CHECK(ctx->dim > 0, PF_ERR_STATE, "no grid");
     PF_CHECK(prm && prm->m == 1, PF_ERR_UNSUPPORTED, "MMA on the device supports one constraint (m = 1)");
     PF_CHECK(prm->xmax > prm->xmin, PF_ERR_ARG, "MMA: xmax must exceed xmin");
@@ -1418,6 +1434,7 @@ int mma_init_device(pf_ctx* ctx, const pf_mma_params* prm) {
     }
     MmaState* M = ctx->mma;
     M->prm = *prm;
+    M->conlin = conlin;
     M->k = 0;
     M->previousvalue = 0.0;
     const size_t eb = sizeof(double) * ctx->ne;
@@ -1913,6 +1930,19 @@ int pf_mma_init(pf_ctx* ctx, const pf_mma_params* prm) {
     return mma_init_device(ctx, prm);
 }
 
+int pf_conlin_init(pf_ctx* ctx, const pf_mma_params* prm) {
+    if (!ctx || !prm) return PF_ERR_ARG;
+    PF_TRY(set_device(ctx));
+    return mma_init_device(ctx, prm, true);
+}
+
+int pf_conlin_update(pf_ctx* ctx, double* x, double f, const double* dfdx, double g, const double* dgdx,
+                     int* newton_steps) {
+    if (!ctx) return PF_ERR_ARG;
+    PF_CHECK(ctx->mma && ctx->mma->conlin, PF_ERR_STATE, "CONLIN not initialised: call pf_conlin_init first");
+    return pf_mma_update(ctx, x, f, dfdx, g, dgdx, newton_steps);
+}
+
 int pf_mma_update(pf_ctx* ctx, double* x, double f, const double* dfdx, double g, const double* dgdx,
                   int* newton_steps) {
     if (!ctx || !x || !dfdx || !dgdx) return PF_ERR_ARG;
@@ -1977,7 +2007,8 @@ int pf_simp_run(pf_ctx* ctx, const pf_simp_params* prm, int iters, double* s, do
     oc.scale1 = prm->scale1;
     oc.beta = 0.0;
     double beta = 0.0;
-    if (prm->use_mma) PF_TRY(mma_init_device(ctx, &prm->mma));
+    PF_CHECK(prm->use_mma >= 0 && prm->use_mma <= 2, PF_ERR_ARG, "use_mma: 0 OC, 1 MMA, 2 CONLIN");
+    if (prm->use_mma) PF_TRY(mma_init_device(ctx, &prm->mma, prm->use_mma == 2));
     double prev_f = 0.0;
     int it = 0;
     for (; it < iters; it++) {

@@ -1,4 +1,4 @@
-// MMA (Svanberg) for one constraint on the device: the O(n) passes of
+// MMA (Svanberg) and CONLIN for one constraint on the device: the O(n) passes of
 // MMA<T>::UpdateVariables (src/Optimize/Solver/MMA.h:116-419) as kernels
 // with fixed-order workgroup partials; the O(1) scalar algebra of the
 // primal-dual interior-point iteration (the (m+1)^2 = 2x2 Newton system
@@ -206,6 +206,118 @@ __global__ void __launch_bounds__(256) k_mma_trial(long n, double tau, double la
         ip[j] = iv;
         double gs;
         acc[0] += mma_kkt_terms(xv, gv, iv, lam, eps, alpha[j], beta[j], L[j], U[j], p0[j], q0[j], p[j], q[j], gs);
+        acc[1] += gs;
+    }
+    store_partials<2>(acc, lds, part);
+}
+
+// ------------------------------------------------------------- CONLIN ----
+// CONLIN<T> (src/Optimize/Solver/CONLIN.h:90-365) shares the interior-point
+// driver; only the approximation differs: p x + q / x with p, q the positive
+// and negative parts of the derivative (q scaled by x_k^2), move limits
+// without asymptotes.
+
+// Move limits (CONLIN.h:92-96), p0, q0 (99-105), p, q and the b-sum terms
+// (108-120).
+__global__ void __launch_bounds__(256) k_conlin_prep(long n, const double* __restrict__ xk, double xmin, double xmax,
+                                                     double move, const double* __restrict__ dfdx,
+                                                     const double* __restrict__ dgdx, double* __restrict__ alpha,
+                                                     double* __restrict__ beta, double* __restrict__ p0,
+                                                     double* __restrict__ q0, double* __restrict__ p,
+                                                     double* __restrict__ q, double* __restrict__ part) {
+    __shared__ double lds[32];
+    double acc[1] = {0.0};
+    const double dx = xmax - xmin;
+    for (long j = blockIdx.x * (long)blockDim.x + threadIdx.x; j < n; j += (long)gridDim.x * blockDim.x) {
+        const double x = xk[j];
+        // std::max({xmin, x - move dx}), std::min({xmax, x + move dx})
+        double al = xmin, t1 = x - move * dx;
+        if (al < t1) al = t1;
+        double be = xmax, t2 = x + move * dx;
+        if (t2 < be) be = t2;
+        alpha[j] = al;
+        beta[j] = be;
+        const double x2 = x * x;  // pow(x, 2.0)
+        const double df = dfdx[j], dg = dgdx[j];
+        p0[j] = (df < 0.0) ? 0.0 : df;
+        q0[j] = ((-df < 0.0) ? 0.0 : -df) * x2;
+        const double pj = (dg < 0.0) ? 0.0 : dg;
+        const double qj = ((-dg < 0.0) ? 0.0 : -dg) * x2;
+        p[j] = pj;
+        q[j] = qj;
+        acc[0] += pj * x + qj / x;
+    }
+    store_partials<1>(acc, lds, part);
+}
+
+// KKT residual terms of equations (5.9a,e,f) (CONLIN.h:392-396).
+__device__ inline double conlin_kkt_terms(double x, double gs, double it, double lam, double eps, double al,
+                                          double be, double p0, double q0, double p, double q, double& gsum) {
+    const double pl = p0 + lam * p, ql = q0 + lam * q;
+    gsum = p * x + q / x;
+    const double a = pl - ql / (x * x) - gs + it;
+    const double e = gs * (x - al) - eps;
+    const double f = it * (be - x) - eps;
+    double s = 0.0;
+    s += a * a;
+    s += e * e;
+    s += f * f;
+    return s;
+}
+
+// Newton coefficients (CONLIN.h:143-170) and the n-sums of the 2x2 system,
+// as k_mma_newton_a.
+__global__ void __launch_bounds__(256) k_conlin_newton_a(long n, double lam, double eps, const double* __restrict__ x,
+                                                         const double* __restrict__ gsi, const double* __restrict__ ita,
+                                                         const double* __restrict__ alpha,
+                                                         const double* __restrict__ beta, const double* __restrict__ p0,
+                                                         const double* __restrict__ q0, const double* __restrict__ p,
+                                                         const double* __restrict__ q, double* __restrict__ G,
+                                                         double* __restrict__ Dx, double* __restrict__ dlx,
+                                                         double* __restrict__ part) {
+    __shared__ double lds[64];
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (long j = blockIdx.x * (long)blockDim.x + threadIdx.x; j < n; j += (long)gridDim.x * blockDim.x) {
+        const double xv = x[j], al = alpha[j], be = beta[j];
+        const double pl = p0[j] + lam * p[j], ql = q0[j] + lam * q[j];
+        const double x2 = xv * xv;
+        const double g = p[j] - q[j] / x2;
+        const double d = 2.0 * ql / cube_rn(xv) + gsi[j] / (xv - al) + ita[j] / (be - xv);
+        const double dl = pl - ql / x2 - eps / (xv - al) + eps / (be - xv);
+        G[j] = g;
+        Dx[j] = d;
+        dlx[j] = dl;
+        double gs;
+        const double kt = conlin_kkt_terms(xv, gsi[j], ita[j], lam, eps, al, be, p0[j], q0[j], p[j], q[j], gs);
+        acc[0] += gs;
+        acc[1] += g * g / d;
+        acc[2] += g * dl / d;
+        acc[3] += kt;
+    }
+    store_partials<4>(acc, lds, part);
+}
+
+// Trial point of the line search (CONLIN.h:318-340), as k_mma_trial.
+__global__ void __launch_bounds__(256) k_conlin_trial(long n, double tau, double lam, double eps,
+                                                      const double* __restrict__ x, const double* __restrict__ gsi,
+                                                      const double* __restrict__ ita, const double* __restrict__ dx,
+                                                      const double* __restrict__ dgsi, const double* __restrict__ dita,
+                                                      const double* __restrict__ alpha, const double* __restrict__ beta,
+                                                      const double* __restrict__ p0, const double* __restrict__ q0,
+                                                      const double* __restrict__ p, const double* __restrict__ q,
+                                                      double* __restrict__ xp, double* __restrict__ gp,
+                                                      double* __restrict__ ip, double* __restrict__ part) {
+    __shared__ double lds[32];
+    double acc[2] = {0.0, 0.0};
+    for (long j = blockIdx.x * (long)blockDim.x + threadIdx.x; j < n; j += (long)gridDim.x * blockDim.x) {
+        const double xv = x[j] + tau * dx[j];
+        const double gv = gsi[j] + tau * dgsi[j];
+        const double iv = ita[j] + tau * dita[j];
+        xp[j] = xv;
+        gp[j] = gv;
+        ip[j] = iv;
+        double gs;
+        acc[0] += conlin_kkt_terms(xv, gv, iv, lam, eps, alpha[j], beta[j], p0[j], q0[j], p[j], q[j], gs);
         acc[1] += gs;
     }
     store_partials<2>(acc, lds, part);

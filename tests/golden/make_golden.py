@@ -61,6 +61,8 @@ def main():
                      cg_itrmax=100000, cg_eps=1e-10, clamp="x=0 nodes, dofs {0,1}",
                      load="node at (nx, 0), dof 1, value -1", lx="nx", ly="ny")
     oc_meta = dict(iota=0.5, lambdamin=0.0, lambdamax=1e9, lambdaeps=1e-3, move=0.2)
+    conlin_meta = dict(optimizer="CONLIN", a0=1.0, a=[0.0], c=[1e4], d=[0.0], xmin=0.01, xmax=1.0, move=0.2,
+                       epsvalue=1e-6, source="sample/optimize/sample_optimize_density_CONLIN.cpp:80-85")
     heaviside_meta = dict(filter="HeavisideFilter", beta0=0.5, beta_factor=2.0, beta_every=40,
                           beta_rule="beta *= 2 at k % 40 == 0, before the filter (k = 0 gives beta = 1)")
     mma_meta = dict(a0=1.0, a=[0.0], c=[1e4], d=[0.0], xmin=0.01, xmax=1.0, raa0=1e-5, albefa=0.1, move=0.2,
@@ -72,6 +74,9 @@ def main():
         ("simp_120x40_oc", ["simp", 120, 40, 50, "oc"], 0, dict(simp_meta, **oc_meta, nx=120, ny=40, iters=50)),
         ("simp_120x40_mma", ["simp", 120, 40, 50, "mma"], 0, dict(simp_meta, **mma_meta, nx=120, ny=40, iters=50)),
         ("simp_16x8_ocH", ["simp", 16, 8, 5, "ocH"], 5, dict(simp_meta, **oc_meta, **heaviside_meta, nx=16, ny=8, iters=5)),
+        ("simp_16x8_conlin", ["simp", 16, 8, 5, "conlin"], 5, dict(simp_meta, **conlin_meta, nx=16, ny=8, iters=5)),
+        ("simp_120x40_conlin", ["simp", 120, 40, 50, "conlin"], 0,
+         dict(simp_meta, **conlin_meta, nx=120, ny=40, iters=50)),
         ("simp_120x40_ocH", ["simp", 120, 40, 50, "ocH"], 0,
          dict(simp_meta, **oc_meta, **heaviside_meta, nx=120, ny=40, iters=50)),
     ]

@@ -296,3 +296,18 @@ def test_sensitivity_filters_against_reference(pa):
             ctx.filter_set_lists(fp, fi, fw)
         assert np.array_equal(ctx.sensitivity_filter(g["s"], g["dfds"], 1), g["out1"])
         assert np.array_equal(ctx.sensitivity_filter(g["s"], g["dfds"], 2), g["out2"])
+
+
+def test_simp_conlin_loop_c1_against_reference(pa):
+    """50 CONLIN iterations at 120x40 (the CONLIN sample's constants) against
+    the reference's own CONLIN."""
+    case = "simp_120x40_conlin"
+    g = gi.load(case)
+    ctx = pa.cantilever2d(120, 40)
+    r = ctx.simp_run(50, pa.simp_default_params(use_mma=2))
+    from oracle import pyoracle as po
+    ref = po.simp2d(120, 40, 50, po.simp_params(use_mma=2))
+    assert gi.sha(ref["f"]) == gi.digest(case, "hist_f")
+    assert np.max(np.abs(r["f"] - ref["f"]) / ref["f"]) < 1e-9
+    assert np.max(np.abs(r["s"] - g["s_final"])) < 1e-6
+    assert np.max(np.abs(r["rho"] - g["rho_final"])) < 1e-6

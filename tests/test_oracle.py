@@ -61,11 +61,16 @@ def test_iteration0_against_reference(case, nx, ny):
     assert np.array_equal(po.filter_sens(fp, fi, fw, g["dfdrho0"]), g["dfds0"])
 
 
+def optimizer_of(case):
+    return 2 if "conlin" in case else (1 if "mma" in case else 0)
+
+
 @pytest.mark.parametrize("case,nx,ny,iters", [("simp_4x2_oc", 4, 2, 3), ("simp_8x4_oc", 8, 4, 5),
-                                              ("simp_16x8_mma", 16, 8, 5), ("simp_16x8_ocH", 16, 8, 5)])
+                                              ("simp_16x8_mma", 16, 8, 5), ("simp_16x8_ocH", 16, 8, 5),
+                                              ("simp_16x8_conlin", 16, 8, 5)])
 def test_simp_loop_small_against_reference(case, nx, ny, iters):
     g = gi.load(case)
-    prm = po.simp_params(use_mma=1 if "mma" in case else 0, heaviside=1 if case.endswith("H") else 0)
+    prm = po.simp_params(use_mma=optimizer_of(case), heaviside=1 if case.endswith("H") else 0)
     r = po.simp2d(nx, ny, iters, prm)
     assert gi.sha(r["f"]) == gi.digest(case, "hist_f")
     assert gi.sha(r["g"]) == gi.digest(case, "hist_g")
@@ -77,12 +82,12 @@ def test_simp_loop_small_against_reference(case, nx, ny, iters):
 
 
 @pytest.mark.slow
-@pytest.mark.parametrize("case", ["simp_120x40_oc", "simp_120x40_mma", "simp_120x40_ocH"])
+@pytest.mark.parametrize("case", ["simp_120x40_oc", "simp_120x40_mma", "simp_120x40_ocH", "simp_120x40_conlin"])
 def test_simp_loop_c1_against_reference(case):
-    """All 50 OC (or MMA) iterations of the 120x40 workload, bit for bit
+    """All 50 OC (MMA, CONLIN) iterations of the 120x40 workload, bit for bit
     (ocH: HeavisideFilter with beta continuation, beta 1 then 2 at k = 40)."""
     g = gi.load(case)
-    prm = po.simp_params(use_mma=1 if "mma" in case else 0, heaviside=1 if case.endswith("H") else 0)
+    prm = po.simp_params(use_mma=optimizer_of(case), heaviside=1 if case.endswith("H") else 0)
     r = po.simp2d(120, 40, 50, prm)
     assert gi.sha(r["f"]) == gi.digest(case, "hist_f")
     assert gi.sha(r["g"]) == gi.digest(case, "hist_g")
