@@ -80,6 +80,23 @@ def kernels_only(ctx, pa, torch, nx, ny, reps=20, copy_mib=256):
                       "N": ctx.N, "ne": ctx.ne, "kernels_ms": prof}))
 
 
+def stream_copy_gbs(torch, mib=256, reps=20):
+    """STREAM-like device-to-device copy of `mib` MiB on torch's stream, timed
+    with events: bytes read + written per second, the bandwidth a plain
+    streaming kernel reaches on this GPU."""
+    src = torch.ones(mib * 2**20 // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    sec = e0.elapsed_time(e1) * 1e-3 / reps
+    return 2 * mib * 2**20 / sec / 1e9
+
+
 def learned_vs_jacobi(pa, nx, ny, train_iters=30, collect_every=20, collect_max=16, steps=500, lr=2e-3):
     """Learned-preconditioner workload: the network is trained on (residual,
     error) pairs that the plain FP64 Jacobi-PCG produces during the first
@@ -135,8 +152,8 @@ def main():
                          "plain device copy of known size: the program the counter runs profile")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU decomposition is "
-              "described in DESIGN.md and is not part of this build.", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); a multi-GPU decomposition of "
+              "this path is not built.", file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -171,6 +188,7 @@ def main():
     ms, byts = kern[name]
     achieved = byts / (ms * 1e-3) / 1e9
     iter_ms = prof["k1_ms"] + prof["k2_ms"]
+    copy_gbs = stream_copy_gbs(torch)
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "traffic_pcg_q4_1024x512.json")
     if os.path.exists(tfile):
@@ -203,7 +221,8 @@ def main():
         "roofline": {"bound": "hbm", "kernel": name, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "avg_ms": ms, "algorithmic_bytes": byts,
                      "pcg_iteration": {"ms": iter_ms, "bytes": bytes_k1 + bytes_k2,
-                                       "achieved": (bytes_k1 + bytes_k2) / (iter_ms * 1e-3) / 1e9}},
+                                       "achieved": (bytes_k1 + bytes_k2) / (iter_ms * 1e-3) / 1e9},
+                     "stream_copy_GBs": copy_gbs, "frac_of_copy": achieved / copy_gbs},
         "kernels_ms": prof,
         "pcg_time_to_1e-8": extras,
         "cpu_baseline": cpu,

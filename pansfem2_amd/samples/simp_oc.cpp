@@ -11,7 +11,9 @@
 // at the bottom-right node, E1 = 1, E0 = 1e-9, nu = 0.3, p = 3, volume
 // fraction 0.5, filter radius 1.5, ScalingCG(100000, 1e-10), OC(0.5, 0,
 // 1e9, 1e-3, 0.2).  Prints "k f g lambda" per iteration; with out_prefix,
-// writes the final design variables and their filtered densities as text.
+// writes the final design variables and their filtered densities as text,
+// and the last iteration's mesh, u, r and densities as
+// <out_prefix>_result<k>.vtk in the reference sample's layout.
 // With "H": HeavisideFilter with the density samples' beta continuation
 // (beta 0.5, doubled at every 40th iteration from k = 0) instead of
 // DensityFilter.
@@ -145,6 +147,19 @@ int main(int argc, char** argv) {
             }
             std::vector<double> dfds = sens(s, dfdrho);
             std::vector<double> dgds = sens(s, dgdrho);
+
+            // post process of the last iteration, as the reference sample
+            // writes every iteration (sample_optimize_density_oc.cpp:175-182)
+            if (argc > 4 && k == iters - 1) {
+                std::ofstream fout(std::string(argv[4]) + "_result" + std::to_string(k) + ".vtk");
+                MakeHeadderToVTK(fout);
+                AddPointsToVTK(x, fout);
+                AddElementToVTK(elements, fout);
+                AddElementTypes(std::vector<int>(elements.size(), 9), fout);
+                AddPointVectors(u, "u", fout, true);
+                AddPointVectors(r, "r", fout, false);
+                AddElementScalers(rho, "s", fout, true);
+            }
 
             if (optimizer.IsConvergence(f)) break;
             optimizer.UpdateVariables(s, f, dfds, g, dgds, [&](std::vector<double> xk1) {

@@ -77,6 +77,49 @@ def test_sample_driver_against_reference(tmp_path):
     rho = np.loadtxt(prefix + "_rho.txt")
     assert np.max(np.abs(s - g["s_final"])) < 1e-6
     assert np.max(np.abs(rho - g["rho_final"])) < 1e-6
+    # the last iteration's VTK file: the reference sample's layout, and its
+    # u and r (printed to 6 significant digits) give that iteration's compliance
+    vtk = read_vtk(prefix + "_result49.vtk")
+    assert vtk["points"].shape == (121 * 41, 3) and vtk["cells"].shape == (4800, 5)
+    assert np.all(vtk["cells"][:, 0] == 4) and np.all(vtk["types"] == 9)
+    f49 = float(np.sum(vtk["u"] * vtk["r"]))
+    assert abs(f49 - g["hist_f"][49]) / g["hist_f"][49] < 1e-4
+    assert vtk["s"].shape == (4800,) and np.all((vtk["s"] >= 0) & (vtk["s"] <= 1))
+
+
+def read_vtk(path):
+    """Legacy-ASCII VTK of ExportToVTK.h's layout: POINTS, CELLS, CELL_TYPES,
+    POINT_DATA vectors u and r, CELL_DATA scalars s."""
+    tok = open(path).read().split()
+    out, i = {}, 0
+    while i < len(tok):
+        t = tok[i]
+        if t == "POINTS":
+            n = int(tok[i + 1])
+            out["points"] = np.array(tok[i + 3:i + 3 + 3 * n], float).reshape(n, 3)
+            i += 3 + 3 * n
+        elif t == "CELLS":
+            n, m = int(tok[i + 1]), int(tok[i + 2])
+            out["cells"] = np.array(tok[i + 3:i + 3 + m], int).reshape(n, m // n)
+            i += 3 + m
+        elif t == "CELL_TYPES":
+            n = int(tok[i + 1])
+            out["types"] = np.array(tok[i + 2:i + 2 + n], int)
+            i += 2 + n
+        elif t in ("VECTORS", "SCALARS"):
+            name, j = tok[i + 1], i + 3
+            if t == "SCALARS":
+                assert tok[j] == "LOOKUP_TABLE"
+                j += 2
+            k = j
+            while k < len(tok) and tok[k][0] in "-+.0123456789":
+                k += 1
+            vals = np.array(tok[j:k], float)
+            out[name] = vals.reshape(-1, 3) if t == "VECTORS" else vals
+            i = k
+        else:
+            i += 1
+    return out
 
 
 @pytest.mark.gpu
