@@ -6,7 +6,10 @@ clamped, unit downward load at the bottom-right node, E1 = 1, E0 = 1e-9,
 nu = 0.3, p = 3, volume fraction 0.5, density filter R = 1.5, FP64
 matrix-free Jacobi-PCG to a 1e-10 recursive residual, OC update (move 0.2,
 damping 0.5).  A step is one OC iteration (filter, volume, moduli, PCG solve,
-compliance and sensitivities, filtered sensitivities, OC bisection).
+compliance and sensitivities, filtered sensitivities, OC bisection).  The
+PCG runs in its single-reduction form (one launch per iteration, the same
+iterates as ScalingCG's order in exact arithmetic); the loop is timed again
+with ScalingCG's own order beside it.
 
 Prints one JSON line: OC iterations/sec over the timed steps, the roofline
 of the dominant kernel measured live with HIP events, the PCG
@@ -105,6 +108,7 @@ def learned_vs_jacobi(pa, nx, ny, train_iters=30, collect_every=20, collect_max=
     pairs; then both solvers take the system of OC iteration train_iters + 1
     to a 1e-8 FP64 true relative residual."""
     ctx = pa.cantilever2d(nx, ny)
+    ctx.set_pcg_form(pa.PF_PCG_SINGLE_REDUCTION)  # the Jacobi leg at its fastest
     ctx.precond_init(seed=11)
     t0 = time.perf_counter()
     r = ctx.simp_run(train_iters, pa.simp_default_params(collect=collect_every, collect_max=collect_max))
@@ -166,6 +170,7 @@ def main():
         kernels_only(ctx, pa, torch, nx, ny)
         return
     # warmup OC iterations (untimed), then K timed iterations continuing the same trajectory
+    ctx.set_pcg_form(pa.PF_PCG_SINGLE_REDUCTION)
     w = ctx.simp_run(args.warmup) if args.warmup > 0 else {"s": np.full(ne, 0.5)}
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -176,16 +181,23 @@ def main():
     value = args.steps / wall
     n_cg = float(np.mean(r["cg_iters"]))
     n_bis = float(np.mean(r["bisect"]))
+    # the same steps with ScalingCG's two-launch order
+    ctx.set_pcg_form(pa.PF_PCG_SCALINGCG)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    r0 = ctx.simp_run(args.steps, s0=w["s"])
+    torch.cuda.synchronize()
+    wall0 = time.perf_counter() - t0
+    ctx.set_pcg_form(pa.PF_PCG_SINGLE_REDUCTION)
 
     # dominant kernel: live HIP-event timing of the PCG kernels on the final design
     ctx.set_density(r["rho"])
     prof = ctx.profile_kernels(pa.PF_OP_EBE64, reps=100)
     bytes_k1 = 8 * ne + 4 * 8 * N   # p_old, z read; p_new, q written; E read
     bytes_k2 = 8 * 8 * N            # x, r, p, q, D read; x, r, z written
-    kern = {"pcg_k1_q4 (p update + EbE K p + p.q)": (prof["k1_ms"], bytes_k1),
-            "pcg_k2 (x, r, z update + z.r, r.r)": (prof["k2_ms"], bytes_k2)}
-    name = max(kern, key=lambda k: kern[k][0])
-    ms, byts = kern[name]
+    bytes_cg1 = 8 * ne + 11 * 8 * N  # r, w, s, D, p, x read; r, w, s, p, x written; E read
+    name = "pcg_cg1_q4 (single-reduction PCG iteration: s, r, p, x update + EbE K u + r.u, u.Ku, r.r)"
+    ms, byts = prof["cg1_ms"], bytes_cg1
     achieved = byts / (ms * 1e-3) / 1e9
     iter_ms = prof["k1_ms"] + prof["k2_ms"]
     copy_gbs = stream_copy_gbs(torch)
@@ -200,10 +212,14 @@ def main():
     extras = {}
     if not args.no_extras:
         b = ctx.load()
-        for label, op in (("jacobi_fp64", pa.PF_OP_EBE64), ("mixed_fp32_ir", pa.PF_OP_EBE32)):
+        for label, op, form in (("jacobi_fp64", pa.PF_OP_EBE64, pa.PF_PCG_SINGLE_REDUCTION),
+                                ("jacobi_fp64_scalingcg_order", pa.PF_OP_EBE64, pa.PF_PCG_SCALINGCG),
+                                ("mixed_fp32_ir", pa.PF_OP_EBE32, pa.PF_PCG_SINGLE_REDUCTION)):
+            ctx.set_pcg_form(form)
             x, st = ctx.pcg(b, op=op, eps=1e-8)
             extras[label] = {"seconds": st["seconds"], "iterations": st["iterations"],
                              "true_residual": st["true_residual"], "status": st["status"], "outer": st["outer"]}
+        ctx.set_pcg_form(pa.PF_PCG_SINGLE_REDUCTION)
         extras["speedup_mixed_vs_jacobi"] = extras["jacobi_fp64"]["seconds"] / extras["mixed_fp32_ir"]["seconds"]
         if not args.no_learned:
             extras["learned"] = learned_vs_jacobi(pa, nx, ny)
@@ -216,12 +232,17 @@ def main():
         "vs_baseline": None, "dtype": "f64", "data": "synthetic (uniform initial design 0.5, deterministic load case)",
         "config": {"workload": f"SIMP OC loop, plane-strain quad4 {nx}x{ny} cantilever, FP64 matrix-free "
                                "Jacobi-PCG to 1e-10, density filter R=1.5, OC move 0.2",
-                   "nx": nx, "ny": ny, "elements": ne, "free_dofs": N, "solver": "EBE64 Jacobi-PCG",
-                   "mean_cg_iters": n_cg, "mean_bisection_steps": n_bis},
+                   "nx": nx, "ny": ny, "elements": ne, "free_dofs": N,
+                   "solver": "EBE64 Jacobi-PCG, single-reduction form",
+                   "mean_cg_iters": n_cg, "mean_bisection_steps": n_bis,
+                   "scalingcg_order": {"value": args.steps / wall0, "ms_per_step": 1000.0 * wall0 / args.steps,
+                                       "mean_cg_iters": float(np.mean(r0["cg_iters"])),
+                                       "compliance_rel_diff": float(np.max(np.abs(r0["f"] - r["f"]) / r["f"]))}},
         "roofline": {"bound": "hbm", "kernel": name, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "avg_ms": ms, "algorithmic_bytes": byts,
-                     "pcg_iteration": {"ms": iter_ms, "bytes": bytes_k1 + bytes_k2,
-                                       "achieved": (bytes_k1 + bytes_k2) / (iter_ms * 1e-3) / 1e9},
+                     "scalingcg_order": {"k1_ms": prof["k1_ms"], "k2_ms": prof["k2_ms"], "iteration_ms": iter_ms,
+                                         "bytes": bytes_k1 + bytes_k2,
+                                         "achieved": (bytes_k1 + bytes_k2) / (iter_ms * 1e-3) / 1e9},
                      "stream_copy_GBs": copy_gbs, "frac_of_copy": achieved / copy_gbs},
         "kernels_ms": prof,
         "pcg_time_to_1e-8": extras,

@@ -176,6 +176,17 @@ int pf_apply_K(pf_ctx* ctx, int op, const double* x_free, double* y_free);
  * operator; PF_OP_EBE32 runs FP32 inner solves inside FP64 iterative
  * refinement to a true FP64 residual of eps. */
 int pf_pcg(pf_ctx* ctx, int op, const double* b_free, double* x_free, double eps, int itrmax, pf_solve_stats* st);
+/* Form of the Jacobi-PCG recurrences for the quad4 matrix-free solves of
+ * this context (pf_pcg and pf_pcg_dev with PF_OP_EBE64 / PF_OP_EBE32, and
+ * pf_simp_run), until changed.  PF_PCG_SCALINGCG (the default) is
+ * ScalingCG's own order (CG.h:419-453), two launches per iteration.
+ * PF_PCG_SINGLE_REDUCTION is Chronopoulos and Gear's form (s = K p carried
+ * as a vector, p.Kp from u.Ku): one launch and one reduction per iteration,
+ * the same iterates in exact arithmetic, the same stopping test on the
+ * recursive residual.  Hex8, CSR and learned solves keep ScalingCG's order.
+ * Has no counterpart in the reference. */
+enum { PF_PCG_SCALINGCG = 0, PF_PCG_SINGLE_REDUCTION = 1 };
+int pf_set_pcg_form(pf_ctx* ctx, int form);
 /* Same on device-resident free-DOF vectors. */
 int pf_pcg_dev(pf_ctx* ctx, int op, const double* b_free_dev, double* x_free_dev, double eps, int itrmax,
                pf_solve_stats* st);
@@ -255,7 +266,8 @@ void pf_simp_default_params(pf_simp_params* prm);
  * back-to-back launches each) of the two PCG iteration kernels of operator
  * `op` on the current moduli: ms[0] = operator kernel (p update + K p +
  * p.q), ms[1] = vector-update kernel (x, r, z and the z.r, r.r partials),
- * ms[2] = density filter, ms[3] = one OC bisection step (two launches).
+ * ms[2] = density filter, ms[3] = one OC bisection step (two launches),
+ * ms[4] = the single-reduction iteration kernel (PF_OP_EBE64 only, else 0).
  * Diagnostic only (no counterpart in the reference); it overwrites the
  * solver's work vectors. */
 int pf_profile_kernels(pf_ctx* ctx, int op, int reps, double* ms);

@@ -16,6 +16,7 @@ LIB_PATH = os.path.join(HERE, "libpansfem2_amd.so")
 PF_OK, PF_NOT_CONVERGED = 0, 1
 PF_OP_CSR64, PF_OP_EBE64, PF_OP_EBE32, PF_OP_LEARNED = 0, 1, 2, 3
 PF_FILTER_VARIABLES, PF_FILTER_SENSITIVITIES = 0, 1
+PF_PCG_SCALINGCG, PF_PCG_SINGLE_REDUCTION = 0, 1
 
 _i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
 _f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
@@ -100,6 +101,7 @@ def lib():
             "pf_apply_K": ([c_void_p, c_int, _f64p, _f64p], c_int),
             "pf_pcg": ([c_void_p, c_int, _f64p, _f64p, c_double, c_int, P(SolveStats)], c_int),
             "pf_pcg_dev": ([c_void_p, c_int, c_void_p, c_void_p, c_double, c_int, P(SolveStats)], c_int),
+            "pf_set_pcg_form": ([c_void_p, c_int], c_int),
             "pf_compliance_sens": ([c_void_p, _f64p, c_double, P(c_double), _f64p], c_int),
             "pf_filter_setup": ([c_void_p, c_double], c_int),
             "pf_filter": ([c_void_p, c_int, _f64p, _f64p], c_int),
@@ -277,6 +279,12 @@ class Context:
         self._ck(lib().pf_apply_K(self._h, op, _f64(x), y))
         return y
 
+    def set_pcg_form(self, form):
+        """PF_PCG_SCALINGCG (ScalingCG's order, the default) or
+        PF_PCG_SINGLE_REDUCTION (one launch per iteration) for the quad4
+        matrix-free solves of this context."""
+        self._ck(lib().pf_set_pcg_form(self._h, form))
+
     def pcg(self, b, op=PF_OP_EBE64, eps=1e-10, itrmax=100000):
         """ScalingCG(K, b, itrmax, eps) (CG.h:419-453) on the device."""
         x = np.zeros(self.N)
@@ -368,10 +376,11 @@ class Context:
 
     def profile_kernels(self, op=PF_OP_EBE64, reps=50):
         """Average device ms of the PCG operator kernel, the PCG vector kernel,
-        the filter and one OC bisection step (HIP events on the library stream)."""
-        ms = np.zeros(4)
+        the filter, one OC bisection step and the single-reduction PCG kernel
+        (HIP events on the library stream)."""
+        ms = np.zeros(5)
         self._ck(lib().pf_profile_kernels(self._h, op, reps, ms))
-        return dict(k1_ms=ms[0], k2_ms=ms[1], filter_ms=ms[2], oc_step_ms=ms[3])
+        return dict(k1_ms=ms[0], k2_ms=ms[1], filter_ms=ms[2], oc_step_ms=ms[3], cg1_ms=ms[4])
 
     # ---- learned preconditioner ----------------------------------------------
     def precond_init(self, seed=20261204):

@@ -34,6 +34,17 @@ int grid_cap() {
 #define PF_K1_TJ 64
 #endif
 constexpr int kTileJ = PF_K1_TJ, kTileI = PF_K1_TI;
+// tile of the single-reduction kernel (k_cg1_q4_tile)
+#ifndef PF_CG1_TI
+#define PF_CG1_TI 8
+#endif
+#ifndef PF_CG1_TJ
+#define PF_CG1_TJ 64
+#endif
+#ifndef PF_CG1_EDGE_FIRST
+#define PF_CG1_EDGE_FIRST 1
+#endif
+constexpr int kCg1TI = PF_CG1_TI, kCg1TJ = PF_CG1_TJ;
 
 // Launch shape of the quad4 operator kernel: tiles, and workgroups (a
 // multiple of 8, one group per XCD) so that each takes about kTilesPerBlock
@@ -119,6 +130,7 @@ void free_grid(pf_ctx* ctx) {
     dfree(ctx->d_x); dfree(ctx->d_r); dfree(ctx->d_z); dfree(ctx->d_q); dfree(ctx->d_p0); dfree(ctx->d_p1);
     dfree(ctx->d_D); dfree(ctx->d_b); dfree(ctx->d_t0); dfree(ctx->d_t1);
     dfree(ctx->f_x); dfree(ctx->f_r); dfree(ctx->f_z); dfree(ctx->f_q); dfree(ctx->f_p0); dfree(ctx->f_p1);
+    dfree(ctx->d_s0); dfree(ctx->d_s1); dfree(ctx->f_s0); dfree(ctx->f_s1);
     dfree(ctx->f_D); dfree(ctx->f_b);
     dfree(ctx->d_e0); dfree(ctx->d_e1); dfree(ctx->d_e2); dfree(ctx->d_e3); dfree(ctx->d_e4); dfree(ctx->d_e5);
     dfree(ctx->d_e6);
@@ -141,9 +153,10 @@ int alloc_grid(pf_ctx* ctx) {
     PF_TRY(dalloc(ctx, &ctx->d_E, ne));
     PF_TRY(dalloc(ctx, &ctx->d_Ef, ne));
     double** dv[] = {&ctx->d_x, &ctx->d_r, &ctx->d_z, &ctx->d_q, &ctx->d_p0, &ctx->d_p1, &ctx->d_D, &ctx->d_b,
-                     &ctx->d_t0, &ctx->d_t1, &ctx->d_load};
+                     &ctx->d_t0, &ctx->d_t1, &ctx->d_load, &ctx->d_s0, &ctx->d_s1};
     for (double** p : dv) PF_TRY(dalloc(ctx, p, nv));
-    float** fv[] = {&ctx->f_x, &ctx->f_r, &ctx->f_z, &ctx->f_q, &ctx->f_p0, &ctx->f_p1, &ctx->f_D, &ctx->f_b};
+    float** fv[] = {&ctx->f_x, &ctx->f_r, &ctx->f_z, &ctx->f_q, &ctx->f_p0, &ctx->f_p1, &ctx->f_D, &ctx->f_b,
+                    &ctx->f_s0, &ctx->f_s1};
     for (float** p : fv) PF_TRY(dalloc(ctx, p, nv));
     double** de[] = {&ctx->d_e0, &ctx->d_e1, &ctx->d_e2, &ctx->d_e3, &ctx->d_e4, &ctx->d_e5, &ctx->d_e6};
     for (double** p : de) PF_TRY(dalloc(ctx, p, ne));
@@ -261,9 +274,91 @@ struct SnapHook {
     std::vector<int> iters;
 };
 
+// Workgroups of k_cg1_q4_tile: full tiles, then edge tiles (launched first).
+Cg1Tiles cg1_tiles(const pf_ctx* ctx, unsigned* grid) {
+    Cg1Tiles G;
+    G.fullI = (ctx->nx + 1) / kCg1TI;
+    G.fullJ = (ctx->ny + 1) / kCg1TJ;
+    G.tilesI = (ctx->nx + 1 + kCg1TI - 1) / kCg1TI;
+    G.tilesJ = (ctx->ny + 1 + kCg1TJ - 1) / kCg1TJ;
+    G.perF = (G.fullI * G.fullJ + 7) / 8;
+    G.nEdgePad = 8 * ((G.tilesI * G.tilesJ - G.fullI * G.fullJ + 7) / 8);
+    G.edgeFirst = PF_CG1_EDGE_FIRST;
+    *grid = (unsigned)(8 * G.perF + G.nEdgePad);
+    return G;
+}
+
+// Single-reduction form on quad4 grids (k_cg1_q4_tile, one launch per
+// iteration).  Work vectors: x and p in place; r, w = K u and s = K p each in
+// two buffers that alternate between launches (r|z, p1|q, s0|s1).
+template <class T>
+int pcg_device_cg1(pf_ctx* ctx, const T* b, T* x, const T* D, T* r, T* z, T* p0, T* p1, T* q, double eps,
+                   int itrmax, PcgRun& out) {
+    PcgDev* S = (PcgDev*)ctx->d_pcg;
+    double* part[2] = {ctx->red.partials, ctx->red.partials + 3 * kMaxPartials};
+    unsigned g1;
+    const Cg1Tiles G = cg1_tiles(ctx, &g1);
+    PF_CHECK(g1 <= (unsigned)kMaxPartials, PF_ERR_ARG, "grid too large for the reduction scratch");
+    const bool f32 = std::is_same<T, float>::value;
+    const T* Ev = f32 ? (const T*)ctx->d_Ef : (const T*)ctx->d_E;
+    const T* Kv = f32 ? (const T*)ctx->d_K0f : (const T*)ctx->d_K0;
+    T* s0 = f32 ? (T*)ctx->f_s0 : (T*)ctx->d_s0;
+    T* s1 = f32 ? (T*)ctx->f_s1 : (T*)ctx->d_s1;
+    T* R[2] = {r, z};
+    T* W[2] = {p1, q};
+    T* Sv[2] = {s0, s1};
+    k_cg1_start<<<1, 1, 0, ctx->stream>>>(S);
+    PF_TRY(check_launch(ctx));
+    const int chunk = 32;
+    const int mmax = itrmax + 1;  // launch m finishes iterate m; iterate itrmax is the last
+    int m = 0;
+    hipEvent_t ev[2];
+    PF_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+    PF_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+    volatile int* hflag = (volatile int*)ctx->h_red;
+    hflag[0] = hflag[1] = 0;
+    int slot = 0, pending = 0;
+    bool done = false;
+    while (!done && m < mmax) {
+        const int mend = std::min(mmax, m + chunk);
+        for (; m < mend; m++) {
+            const int c = m & 1, o = c ^ 1;
+            Cg1Vecs<T> V{m == 0 ? b : R[o], W[o], Sv[o], R[c], W[c], Sv[c], p0, x, D};
+            k_cg1_q4_tile<T, kCg1TI, kCg1TJ><<<g1, kBlock, 0, ctx->stream>>>(
+                ctx->nx, ctx->ny, G, m, V, Ev, Kv, ctx->d_fix, part[o], (int)g1, part[c], S, eps);
+        }
+        PF_TRY(check_launch(ctx));
+        PF_HIP(hipMemcpyAsync((void*)&hflag[slot], &S->done, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        PF_HIP(hipEventRecord(ev[slot], ctx->stream));
+        if (pending) {
+            PF_HIP(hipEventSynchronize(ev[slot ^ 1]));
+            if (hflag[slot ^ 1]) done = true;
+        }
+        pending = 1;
+        slot ^= 1;
+    }
+    PF_HIP(hipStreamSynchronize(ctx->stream));
+    hipEventDestroy(ev[0]);
+    hipEventDestroy(ev[1]);
+    k_cg1_final<<<1, kBlock, 0, ctx->stream>>>(m, part[(m - 1) & 1], (int)g1, S, eps);
+    PF_TRY(check_launch(ctx));
+    PF_HIP(hipMemcpyAsync(ctx->h_pcg, S, sizeof(PcgDev), hipMemcpyDeviceToHost, ctx->stream));
+    PF_TRY(sync(ctx));
+    const PcgDev* H = (const PcgDev*)ctx->h_pcg;
+    out.iterations = H->iterations;
+    out.status = H->status;
+    out.rnorm = H->rnorm;
+    out.bnorm = H->bnorm;
+    return PF_OK;
+}
+
 template <class T>
 int pcg_device(pf_ctx* ctx, bool csr, const T* b, T* x, const T* D, T* r, T* z, T* p0, T* p1, T* q, double eps,
                int itrmax, PcgRun& out, SnapHook* snap = nullptr) {
+    // FP32 solves keep ScalingCG's order: with FP32 vectors the carried s = K p
+    // drifts and the single-reduction form needs about a third more iterations
+    if (!csr && is2d(ctx) && ctx->pcg_form == PF_PCG_SINGLE_REDUCTION && !snap && std::is_same<T, double>::value)
+        return pcg_device_cg1<T>(ctx, b, x, D, r, z, p0, p1, q, eps, itrmax, out);
     const long n = csr ? ctx->N : ctx->nv;
     PcgDev* S = (PcgDev*)ctx->d_pcg;
     double* partA = ctx->red.partials;
@@ -1464,7 +1559,7 @@ int pf_create(pf_ctx** out, int device) {
     ctx->device = device;
     int rc = set_device(ctx);
     if (rc == PF_OK && hipStreamCreate(&ctx->stream) != hipSuccess) rc = PF_ERR_HIP;
-    if (rc == PF_OK) rc = dalloc(ctx, &ctx->red.partials, 4L * kMaxPartials);
+    if (rc == PF_OK) rc = dalloc(ctx, &ctx->red.partials, 8L * kMaxPartials);
     if (rc == PF_OK) rc = dalloc(ctx, &ctx->red.counter, 1);
     if (rc == PF_OK) rc = dalloc(ctx, &ctx->red.out, 16);
     if (rc == PF_OK && hipMalloc((void**)&ctx->d_pcg, sizeof(PcgDev)) != hipSuccess) rc = PF_ERR_NOMEM;
@@ -1784,6 +1879,13 @@ int pf_pcg_dev(pf_ctx* ctx, int op, const double* b_free_dev, double* x_free_dev
     PF_TRY(to_free(ctx, ctx->d_x, x_free_dev));
     PF_TRY(sync(ctx));
     return rc;
+}
+
+int pf_set_pcg_form(pf_ctx* ctx, int form) {
+    if (!ctx) return PF_ERR_ARG;
+    PF_CHECK(form == PF_PCG_SCALINGCG || form == PF_PCG_SINGLE_REDUCTION, PF_ERR_ARG, "pf_set_pcg_form: unknown form");
+    ctx->pcg_form = form;
+    return PF_OK;
 }
 
 int pf_pcg(pf_ctx* ctx, int op, const double* b_free, double* x_free, double eps, int itrmax, pf_solve_stats* st) {
@@ -2184,6 +2286,27 @@ int pf_profile_kernels(pf_ctx* ctx, int op, int reps, double* ms) {
         ms[1] = timed([&] {
             k_pcg_k2<double><<<g2, kBlock, 0, ctx->stream>>>(ctx->nv, 0, ctx->d_x, ctx->d_r, ctx->d_z, ctx->d_p1,
                                                            ctx->d_q, ctx->d_D, partA, (int)g1, partB, S);
+        });
+    }
+    ms[4] = 0.0;
+    if (!f32) {
+        // the single-reduction kernel: launch 0 from r = b, then launch 1 (no
+        // convergence test) repeated on the same inputs
+        unsigned gc;
+        const Cg1Tiles G = cg1_tiles(ctx, &gc);
+        PF_CHECK(gc <= (unsigned)kMaxPartials, PF_ERR_ARG, "grid too large for the reduction scratch");
+        double* pc0 = ctx->red.partials;
+        double* pc1 = ctx->red.partials + 3 * kMaxPartials;
+        k_cg1_start<<<1, 1, 0, ctx->stream>>>(S);
+        Cg1Vecs<double> V0{ctx->d_load, ctx->d_p1, ctx->d_s1, ctx->d_r, ctx->d_p1, ctx->d_s0, ctx->d_p0, ctx->d_x,
+                           ctx->d_D};
+        k_cg1_q4_tile<double, kCg1TI, kCg1TJ><<<gc, kBlock, 0, ctx->stream>>>(
+            ctx->nx, ctx->ny, G, 0, V0, ctx->d_E, ctx->d_K0, ctx->d_fix, pc1, (int)gc, pc0, S, 0.0);
+        Cg1Vecs<double> V1{ctx->d_r, ctx->d_p1, ctx->d_s0, ctx->d_z, ctx->d_q, ctx->d_s1, ctx->d_p0, ctx->d_x,
+                           ctx->d_D};
+        ms[4] = timed([&] {
+            k_cg1_q4_tile<double, kCg1TI, kCg1TJ><<<gc, kBlock, 0, ctx->stream>>>(
+                ctx->nx, ctx->ny, G, 1, V1, ctx->d_E, ctx->d_K0, ctx->d_fix, pc0, (int)gc, pc1, S, 0.0);
         });
     }
     ms[2] = ms[3] = 0.0;

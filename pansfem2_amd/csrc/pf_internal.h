@@ -89,6 +89,10 @@ struct pf_ctx {
     double *d_x = nullptr, *d_r = nullptr, *d_z = nullptr, *d_q = nullptr, *d_p0 = nullptr, *d_p1 = nullptr;
     double *d_D = nullptr, *d_b = nullptr, *d_t0 = nullptr, *d_t1 = nullptr;
     float *f_x = nullptr, *f_r = nullptr, *f_z = nullptr, *f_q = nullptr, *f_p0 = nullptr, *f_p1 = nullptr;
+    int pcg_form = 0;  // PF_PCG_SCALINGCG or PF_PCG_SINGLE_REDUCTION (quad4 matrix-free solves)
+    // second buffers of the single-reduction solver's s = K p
+    double *d_s0 = nullptr, *d_s1 = nullptr;
+    float *f_s0 = nullptr, *f_s1 = nullptr;
     float *f_D = nullptr, *f_b = nullptr;
 
     // element work vectors (ne)

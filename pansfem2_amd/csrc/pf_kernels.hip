@@ -181,6 +181,7 @@ struct PcgDev {
     double bnorm;
     double pq, alpha, beta, rnorm;
     int iterations, status, done, pad;
+    double g[2], a[2];  // single-reduction form: gamma = r.u and alpha of the last two steps
 };
 
 // beta for K1(k) from the sums (z.r, r.r) of K2(k-1) (init: (z.r, b.b)).
@@ -418,6 +419,319 @@ __global__ void __launch_bounds__(256) k_pcg_k1_q4_tile(int nx, int ny, int tile
         }
     }
     store_partials<1>(acc, lds, partA);
+}
+
+// ------------------------------------------- single-reduction Jacobi-PCG ----
+// Chronopoulos & Gear's form of preconditioned CG (J. Comput. Appl. Math. 25,
+// 1989), one launch per iteration on quad4 grids.  With u = r ./ D, w = K u
+// and s = K p carried as a vector, the recurrences
+//   p_i = u_i + beta_i p_{i-1}      s_i = w_i + beta_i s_{i-1}
+//   x_{i+1} = x_i + alpha_i p_i     r_{i+1} = r_i - alpha_i s_i
+//   gamma_i = r_i.u_i, delta_i = u_i.w_i,
+//   beta_i = gamma_i / gamma_{i-1}, alpha_i = gamma_i / (delta_i - beta_i gamma_i / alpha_{i-1})
+// generate ScalingCG's iterates (CG.h:397-453) in exact arithmetic (p_i.K p_i
+// = delta_i - beta_i gamma_i / alpha_{i-1}), and the three sums of an iterate
+// (gamma, delta, r.r) come out of one reduction.
+//
+// Launch m (m >= 1) reduces the sums of iterate m-1, stops if |r_{m-1}| <
+// eps |b| (iterations = m-1, as ScalingCG counts), else forms alpha_{m-1},
+// beta_{m-1}, updates p, s, x, r, and computes w_m = K u_m with its sums.
+// Launch 0 starts from r = b, x = 0 (alpha = beta = 0, nothing else read).
+// A tile needs u_m on its one-node halo, so it recomputes s and r there from
+// the previous launch's r, w, s (which neighbouring tiles are rewriting):
+// r, w and s alternate between two buffers; p and x are touched only at a
+// tile's own nodes and are updated in place.
+template <class T>
+struct Cg1Vecs {
+    const T* rin;  // r_{m-1} (b at m = 0)
+    const T* win;  // w_{m-1}
+    const T* sin;  // s_{m-2}
+    T* rout;       // r_m
+    T* wout;       // w_m
+    T* sout;       // s_{m-1}
+    T* p;          // p_{m-2} -> p_{m-1}
+    T* x;          // x_{m-1} -> x_m
+    const T* D;
+};
+
+// the head of launch m >= 1 from the sums v = (gamma, delta, r.r) of iterate m-1
+__device__ inline void cg1_head(int m, const double (&v)[3], PcgDev* __restrict__ S, double eps, double& alpha,
+                                double& beta, bool& stop) {
+    stop = false;
+    const bool w0 = threadIdx.x == 0 && blockIdx.x == 0;
+    const double rnorm = sqrt(v[2]);
+    if (m == 1) {
+        beta = 0.0;
+        alpha = v[0] / v[1];
+        if (w0) S->bnorm = rnorm;
+        if (!(v[0] > 0.0)) {  // b = 0: x = 0 is the solution
+            stop = true;
+            if (w0) {
+                S->rnorm = rnorm;
+                S->iterations = 0;
+                S->status = 1;
+                S->done = 1;
+            }
+            return;
+        }
+    } else {
+        if (rnorm < eps * S->bnorm) {  // CG.h:445-449
+            stop = true;
+            if (w0) {
+                S->rnorm = rnorm;
+                S->iterations = m - 1;
+                S->status = 1;
+                S->done = 1;
+            }
+            return;
+        }
+        beta = v[0] / S->g[m & 1];                       // gamma_{m-2} (slot (m-2)&1)
+        alpha = v[0] / (v[1] - beta * v[0] / S->a[m & 1]);  // alpha_{m-2}
+    }
+    if (w0) {
+        S->g[(m - 1) & 1] = v[0];
+        S->a[(m - 1) & 1] = alpha;
+        S->rnorm = rnorm;
+    }
+}
+
+// One tile of TI x TJ nodes per workgroup (tile (I, J) covers node rows
+// [I TI, ...) and columns [J TJ, ...), clipped to the grid); TJ need not
+// divide the workgroup: the own nodes are numbered q = r TJ + c and dealt
+// out 256 at a time.
+// Which tile a workgroup takes.  Full tiles (TI x TJ nodes inside the grid)
+// and edge tiles (the last, partial tile row and column) are launched as two
+// groups; edgeFirst puts the edge tiles (little work each) first, so that
+// the full tiles that do not fit in the first wave of resident workgroups
+// start as soon as the edge tiles are done.  Full tiles are given out
+// XCD-contiguously (workgroup b runs on XCD b mod 8) as in k_pcg_k1_q4_tile;
+// the edge group is padded to a multiple of 8 workgroups to keep that
+// correspondence.
+struct Cg1Tiles {
+    int fullI, fullJ;    // full tile rows and columns
+    int tilesI, tilesJ;  // all tile rows and columns
+    int perF;            // full tiles per XCD (ceil(fullI fullJ / 8))
+    int nEdgePad;        // edge tiles, rounded up to a multiple of 8
+    int edgeFirst;
+};
+
+// tile (I, J) of workgroup b; false if b has no tile
+__device__ inline bool cg1_tile(const Cg1Tiles& G, int b, int& I, int& J) {
+    const int nFullPad = 8 * G.perF;
+    int f = -1, e = -1;
+    if (G.edgeFirst) {
+        if (b < G.nEdgePad) e = b; else f = b - G.nEdgePad;
+    } else {
+        if (b < nFullPad) f = b; else e = b - nFullPad;
+    }
+    if (f >= 0) {
+        const int id = (f & 7) * G.perF + (f >> 3);
+        if (id >= G.fullI * G.fullJ) return false;
+        I = id / G.fullJ;
+        J = id - I * G.fullJ;
+        return true;
+    }
+    const int ec = G.tilesJ - G.fullJ;  // edge columns (0 or 1)
+    if (e < G.tilesI * ec) {
+        I = e / ec;
+        J = G.fullJ + (e - I * ec);
+        return true;
+    }
+    e -= G.tilesI * ec;
+    if (e < (G.tilesI - G.fullI) * G.fullJ) {
+        I = G.fullI + e / G.fullJ;
+        J = e - (I - G.fullI) * G.fullJ;
+        return true;
+    }
+    return false;
+}
+
+template <class T, int TI, int TJ>
+__global__ void __launch_bounds__(256) k_cg1_q4_tile(int nx, int ny, Cg1Tiles G, int m,
+                                                     Cg1Vecs<T> V, const T* __restrict__ E,
+                                                     const T* __restrict__ K0, const uint8_t* __restrict__ fix,
+                                                     const double* __restrict__ partIn, int nbIn,
+                                                     double* __restrict__ partOut, PcgDev* __restrict__ S,
+                                                     double eps) {
+#pragma clang fp contract(fast)  // not a check-path kernel: FMA allowed
+    constexpr int PER = ((TI + 2) * (TJ + 2) + 255) / 256;  // staged entries per thread
+    constexpr int PE = ((TI + 1) * (TJ + 1) + 255) / 256;   // element moduli per thread
+    constexpr int OWN = (TI * TJ + 255) / 256;              // own nodes per thread
+    __shared__ double lds[32];
+    __shared__ Vec2<T> su[(TI + 2) * (TJ + 2)];  // u_m on the tile and its halo, row stride tj + 2
+    __shared__ Vec2<T> uo[TI * TJ];              // u_{m-1} = r_{m-1} ./ D at the own nodes, stride tj
+    __shared__ T se[(TI + 1) * (TJ + 1)];        // moduli of the tile's elements, stride tj + 1
+    constexpr int ti = TI, tj = TJ;
+    constexpr int halo = (ti + 2) * (tj + 2), es = (ti + 1) * (tj + 1), own = ti * tj;
+    int TIr, TJr;
+    const bool has = cg1_tile(G, blockIdx.x, TIr, TJr);
+    const int i0 = TIr * TI, j0 = TJr * TJ;
+    const bool prev = m > 0;  // launch 0 reads only r = b
+
+    // staged entries: r, w, s, D; the tile's elements: E.  p and x (own
+    // nodes only) are loaded later, in the operator phase's mapping, so they
+    // are not live during the head.
+    Vec2<T> rv[PER], wv[PER], sv[PER], dv[PER];
+    T ev[PE];
+    auto load = [&]() {
+#pragma unroll
+        for (int u = 0; u < PER; u++) {
+            const int h = threadIdx.x + u * 256;
+            const int a = h / (tj + 2), b = h - a * (tj + 2);
+            const int ii = i0 - 1 + a, jj = j0 - 1 + b;
+            rv[u] = wv[u] = sv[u] = Vec2<T>{0, 0};
+            dv[u] = Vec2<T>{1, 1};
+            if (h < halo && ii >= 0 && ii <= nx && jj >= 0 && jj <= ny) {
+                const long n = (long)(ny + 1) * ii + jj;
+                rv[u] = load2(V.rin, n);
+                dv[u] = load2(V.D, n);
+                if (prev) {
+                    wv[u] = load2(V.win, n);
+                    sv[u] = load2(V.sin, n);
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < PE; u++) {
+            const int h = threadIdx.x + u * 256;
+            const int a = h / (tj + 1), b = h - a * (tj + 1);
+            const int ei = i0 - 1 + a, ej = j0 - 1 + b;
+            ev[u] = (h < es && ei >= 0 && ei < nx && ej >= 0 && ej < ny) ? E[(long)ny * ei + ej] : (T)0;
+        }
+    };
+    // the first tile's loads, the previous launch's partials and the done
+    // flag are issued before the first wait
+    if (has) load();
+    PartRegs<3, 5> pr;  // 1280 partials in registers, the rest read in sum_partials
+    if (prev) load_partials(partIn, nbIn, pr);
+    if (S->done) return;
+    double alpha = 0.0, beta = 0.0;
+    if (prev) {
+        double v[3];
+        bool stop;
+        sum_partials(pr, partIn, nbIn, lds, v);
+        cg1_head(m, v, S, eps, alpha, beta, stop);
+        if (stop) return;
+    }
+    const T ta = (T)alpha, tb = (T)beta;
+    constexpr int ss = tj + 2;  // LDS row strides
+    double acc[3] = {0.0, 0.0, 0.0};  // r.u, u.w, r.r of iterate m
+    if (has) {
+#pragma unroll
+        for (int u = 0; u < PER; u++) {
+            const int h = threadIdx.x + u * 256;
+            if (h < halo) {
+                const int a = h / (tj + 2), b = h - a * (tj + 2);
+                const int ii = i0 - 1 + a, jj = j0 - 1 + b;
+                Vec2<T> sn, rn, un;
+                sn.x = wv[u].x + tb * sv[u].x;  // s = w + beta s
+                sn.y = wv[u].y + tb * sv[u].y;
+                rn.x = rv[u].x - ta * sn.x;     // r -= alpha s
+                rn.y = rv[u].y - ta * sn.y;
+                un.x = rn.x / dv[u].x;          // Scaling (CG.h:412)
+                un.y = rn.y / dv[u].y;
+                su[h] = un;
+                if (a >= 1 && a <= ti && b >= 1 && b <= tj && ii <= nx && jj <= ny) {
+                    const long n = (long)(ny + 1) * ii + jj;
+                    uo[(a - 1) * tj + (b - 1)] = Vec2<T>{rv[u].x / dv[u].x, rv[u].y / dv[u].y};
+                    store2(V.sout, n, sn);
+                    store2(V.rout, n, rn);
+                    acc[0] += (double)rn.x * (double)un.x;
+                    acc[0] += (double)rn.y * (double)un.y;
+                    acc[2] += (double)rn.x * (double)rn.x;
+                    acc[2] += (double)rn.y * (double)rn.y;
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < PE; u++) {
+            const int h = threadIdx.x + u * 256;
+            if (h < es) se[h] = ev[u];
+        }
+        // p_{m-2} and x_{m-1} of this thread's own nodes: in flight during the operator
+        Vec2<T> pv[OWN], xv[OWN];
+#pragma unroll
+        for (int o = 0; o < OWN; o++) {
+            const int q = threadIdx.x + o * 256;
+            const int r = q / tj, c = q - r * tj;
+            const int i = i0 + r, j = j0 + c;
+            pv[o] = xv[o] = Vec2<T>{0, 0};
+            if (prev && q < own && i <= nx && j <= ny) {
+                const long n = (long)(ny + 1) * i + j;
+                pv[o] = load2(V.p, n);
+                xv[o] = load2(V.x, n);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int o = 0; o < OWN; o++) {
+            const int q = threadIdx.x + o * 256;
+            const int r = q / tj, c = q - r * tj;
+            const int i = i0 + r, j = j0 + c;
+            if (q >= own || i > nx || j > ny) continue;
+            const Vec2<T>* s0 = su + r * ss + c;  // LDS rows r, r+1, r+2 from column c
+            T ax = 0, ay = 0, ex, ey;
+            if (i >= 1 && j >= 1) {  // element (i-1, j-1): this node is TR
+                elem_rows_q4(K0, 2, s0[0], s0[ss], s0[ss + 1], s0[1], ex, ey);
+                T Ee = se[r * (tj + 1) + c];
+                ax += Ee * ex; ay += Ee * ey;
+            }
+            if (i >= 1 && j < ny) {  // element (i-1, j): BR
+                elem_rows_q4(K0, 1, s0[1], s0[ss + 1], s0[ss + 2], s0[2], ex, ey);
+                T Ee = se[r * (tj + 1) + c + 1];
+                ax += Ee * ex; ay += Ee * ey;
+            }
+            if (i < nx && j >= 1) {  // element (i, j-1): TL
+                elem_rows_q4(K0, 3, s0[ss], s0[2 * ss], s0[2 * ss + 1], s0[ss + 1], ex, ey);
+                T Ee = se[(r + 1) * (tj + 1) + c];
+                ax += Ee * ex; ay += Ee * ey;
+            }
+            if (i < nx && j < ny) {  // element (i, j): BL
+                elem_rows_q4(K0, 0, s0[ss + 1], s0[2 * ss + 1], s0[2 * ss + 2], s0[ss + 2], ex, ey);
+                T Ee = se[(r + 1) * (tj + 1) + c + 1];
+                ax += Ee * ex; ay += Ee * ey;
+            }
+            const long n = (long)(ny + 1) * i + j;
+            const uint8_t f = fix[n];
+            if (f & 1) ax = 0;
+            if (f & 2) ay = 0;
+            const Vec2<T> un = s0[ss + 1];
+            store2(V.wout, n, Vec2<T>{ax, ay});
+            acc[1] += (double)un.x * (double)ax;
+            acc[1] += (double)un.y * (double)ay;
+            const Vec2<T> uold = uo[q];
+            Vec2<T> pn, xn;
+            pn.x = uold.x + tb * pv[o].x;  // p = u + beta p
+            pn.y = uold.y + tb * pv[o].y;
+            xn.x = xv[o].x + ta * pn.x;    // x += alpha p
+            xn.y = xv[o].y + ta * pn.y;
+            store2(V.p, n, pn);
+            store2(V.x, n, xn);
+        }
+    }
+    store_partials<3>(acc, lds, partOut);
+}
+
+__global__ void k_cg1_start(PcgDev* __restrict__ S) {
+    S->done = 0;
+    S->status = 0;
+    S->iterations = 0;
+}
+
+// After the host's last launch (M launches: iterate M-1 is the last one):
+// the convergence test of that iterate and the itrmax exit (CG.h:451-452).
+__global__ void k_cg1_final(int M, const double* __restrict__ partIn, int nbIn, PcgDev* __restrict__ S, double eps) {
+    __shared__ double lds[32];
+    if (S->done) return;
+    double v[3];
+    reduce_partials<3>(partIn, nbIn, lds, v);
+    if (threadIdx.x == 0) {
+        const double rnorm = sqrt(v[2]);
+        S->rnorm = rnorm;
+        S->iterations = M - 1;
+        S->status = (M > 1 && rnorm < eps * S->bnorm) ? 1 : 2;
+        S->done = 1;
+    }
 }
 
 // CSR flavour of K1 (free-DOF vectors): q_i = sum_j a_ij (beta pold_j + z_j),

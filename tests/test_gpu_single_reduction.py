@@ -1,0 +1,132 @@
+"""GPU parity of the single-reduction Jacobi-PCG form (PF_PCG_SINGLE_REDUCTION:
+Chronopoulos and Gear's recurrences, one launch per iteration) against the
+reference's ScalingCG results and against the device's ScalingCG-order form.
+
+Bars as for ScalingCG itself: displacement within 1e-8 of the reference's at
+eps = 1e-10; the 50-iteration SIMP loop within 1e-9 (compliance) and 1e-6
+(densities); CG iteration counts within max(3, 0.5%).  The two forms generate
+the same iterates in exact arithmetic, so at grids beyond the oracle's reach
+they must agree with each other to what their residuals allow.  Grid shapes
+cover every tile layout of the kernel: only edge tiles (4x2, 120x40), full
+tiles only (127x63), and both (1024x512, 135x70).
+"""
+import numpy as np
+import pytest
+
+from tests import golden_io as gi
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def pa():
+    import pansfem2_amd
+    return pansfem2_amd
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))
+
+
+def close_counts(a, b):
+    return abs(a - b) <= max(3, 0.005 * b)
+
+
+@pytest.mark.parametrize("case", ["simp_4x2_oc", "simp_8x4_oc", "simp_120x40_oc"])
+def test_solve_against_reference(pa, case):
+    from oracle import pyoracle as po
+    g = gi.load(case)
+    m = gi.meta(case)
+    print(f"single-reduction solve {case}", flush=True)
+    ctx = pa.cantilever2d(m["nx"], m["ny"])
+    ctx.set_density(g["rho0"])
+    ctx.set_pcg_form(pa.PF_PCG_SINGLE_REDUCTION)
+    u, st = ctx.pcg(g["F"], op=pa.PF_OP_EBE64, eps=1e-10)
+    assert st["status"] == 1
+    assert rel(u, g["u0_free"]) < 1e-8
+    assert st["true_residual"] < 1e-9
+    ref_iters = int(po.simp2d(m["nx"], m["ny"], 1)["cg_iters"][0])
+    assert close_counts(st["iterations"], ref_iters), (st["iterations"], ref_iters)
+    ctx.close()
+
+
+def test_simp_loop_c1_against_reference(pa):
+    """50 OC iterations at 120x40 with the single-reduction solver."""
+    from oracle import pyoracle as po
+    case = "simp_120x40_oc"
+    g = gi.load(case)
+    print("single-reduction SIMP loop 120x40", flush=True)
+    ctx = pa.cantilever2d(120, 40)
+    ctx.set_pcg_form(pa.PF_PCG_SINGLE_REDUCTION)
+    r = ctx.simp_run(50)
+    ref = po.simp2d(120, 40, 50)
+    assert gi.sha(ref["f"]) == gi.digest(case, "hist_f")
+    assert np.max(np.abs(r["f"] - ref["f"]) / ref["f"]) < 1e-9
+    assert np.max(np.abs(r["rho"] - g["rho_final"])) < 1e-6
+    assert np.max(np.abs(r["s"] - g["s_final"])) < 1e-6
+    assert np.all(np.abs(r["cg_iters"] - ref["cg_iters"]) <= np.maximum(3, 0.005 * ref["cg_iters"]))
+    ctx.close()
+
+
+@pytest.mark.parametrize("nx,ny", [(127, 63), (135, 70), (1024, 512)])
+def test_agrees_with_scalingcg_order(pa, nx, ny):
+    """Beyond the oracle's sizes: both forms on an i.i.d. density field."""
+    print(f"single-reduction vs ScalingCG order {nx}x{ny}", flush=True)
+    ctx = pa.cantilever2d(nx, ny)
+    rng = np.random.default_rng(20261206)
+    ctx.set_density(rng.uniform(1e-3, 1.0, nx * ny))
+    b = ctx.load()
+    ctx.set_pcg_form(pa.PF_PCG_SCALINGCG)
+    u0, s0 = ctx.pcg(b, op=pa.PF_OP_EBE64, eps=1e-8)
+    ctx.set_pcg_form(pa.PF_PCG_SINGLE_REDUCTION)
+    u1, s1 = ctx.pcg(b, op=pa.PF_OP_EBE64, eps=1e-8)
+    print(f"  iterations {s0['iterations']} vs {s1['iterations']}, |u1-u0|/|u0| {rel(u1, u0):.2e}, "
+          f"true residuals {s0['true_residual']:.2e} {s1['true_residual']:.2e}", flush=True)
+    assert s0["status"] == 1 and s1["status"] == 1
+    assert s1["rel_residual"] < 1e-8
+    assert close_counts(s1["iterations"], s0["iterations"])
+    assert rel(u1, u0) < 1e-6
+    # compliance u.b agrees far more tightly than the displacement
+    assert abs(u1 @ b - u0 @ b) / abs(u0 @ b) < 1e-9
+    ctx.close()
+
+
+def test_zero_load_and_itrmax_exit(pa):
+    ctx = pa.cantilever2d(64, 32)
+    rng = np.random.default_rng(5)
+    ctx.set_density(rng.uniform(0.1, 1.0, 64 * 32))
+    b = ctx.load()
+    ctx.set_pcg_form(pa.PF_PCG_SINGLE_REDUCTION)
+    # b = 0: x = 0 without iterating
+    x, st = ctx.pcg(np.zeros_like(b), op=pa.PF_OP_EBE64, eps=1e-10)
+    assert st["status"] == 1 and st["iterations"] == 0 and not np.any(x)
+    # itrmax reached: the last iterate, flagged as not converged (CG.h:451-452),
+    # equal to the ScalingCG-order iterate after the same number of steps
+    x1, st1 = ctx.pcg(b, op=pa.PF_OP_EBE64, eps=1e-10, itrmax=7)
+    assert st1["status"] == 2 and st1["iterations"] == 7
+    ctx.set_pcg_form(pa.PF_PCG_SCALINGCG)
+    x0, st0 = ctx.pcg(b, op=pa.PF_OP_EBE64, eps=1e-10, itrmax=7)
+    assert st0["status"] == 2 and st0["iterations"] == 7
+    assert rel(x1, x0) < 1e-12
+    ctx.close()
+
+
+def test_mixed_precision_with_single_reduction(pa):
+    """FP32 inner solves keep ScalingCG's order; the FP64 corrections use
+    the single-reduction form: the refinement still ends at a 1e-8 true
+    residual."""
+    ctx = pa.cantilever2d(256, 128)
+    rng = np.random.default_rng(7)
+    ctx.set_density(rng.uniform(1e-3, 1.0, 256 * 128))
+    b = ctx.load()
+    ctx.set_pcg_form(pa.PF_PCG_SINGLE_REDUCTION)
+    x, st = ctx.pcg(b, op=pa.PF_OP_EBE32, eps=1e-8)
+    assert st["status"] == 1 and st["true_residual"] <= 1.0000001e-8
+    ctx.close()
+
+
+def test_form_argument_checked(pa):
+    ctx = pa.cantilever2d(8, 4)
+    with pytest.raises(pa.PfError):
+        ctx.set_pcg_form(7)
+    ctx.close()
