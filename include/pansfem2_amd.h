@@ -40,8 +40,8 @@ enum {
     PF_ERR_UNSUPPORTED = -6
 };
 
-/* operators for pf_apply_K / pf_pcg */
-enum { PF_OP_CSR64 = 0, PF_OP_EBE64 = 1, PF_OP_EBE32 = 2, PF_OP_LEARNED = 3 };
+/* operators for pf_apply_K / pf_pcg (PF_OP_MG: pf_pcg only) */
+enum { PF_OP_CSR64 = 0, PF_OP_EBE64 = 1, PF_OP_EBE32 = 2, PF_OP_LEARNED = 3, PF_OP_MG = 4 };
 /* filter modes for pf_filter */
 enum { PF_FILTER_VARIABLES = 0, PF_FILTER_SENSITIVITIES = 1 };
 
@@ -260,6 +260,30 @@ int pf_simp_run(pf_ctx* ctx, const pf_simp_params* prm, int iters, double* s, do
  * 1e9, 1e-3, 0.2), MMA(a0 1, a 0, c 1e4, d 0, [0.01,1], 1e-5, 0.1, 0.2, 0.5,
  * 0.7, 1.2, 1e-6), density filter; Heaviside continuation 0.5, 2, 40). */
 void pf_simp_default_params(pf_simp_params* prm);
+
+/* ---- multigrid base preconditioner (quad4) ------------------------------ */
+/* No counterpart in the reference.  pf_mg_setup makes a geometric multigrid
+ * V-cycle the base step of the flexible CG: pf_pcg with PF_OP_MG (FCG with
+ * the V-cycle alone) and PF_OP_LEARNED (the network correction over the
+ * V-cycle instead of over the Jacobi step; train after pf_mg_setup so the
+ * network learns against the same base).  Coarse levels halve both element
+ * counts down to at most 512 nodal DOFs (or an odd count); a coarse
+ * element's matrix is the Galerkin product of its four children's
+ * (bilinear prolongation, fixed DOFs removed at the finest level), built in
+ * FP64 and applied in FP32; nu damped-Jacobi sweeps (weight omega) before
+ * and after the coarse correction; a dense solve on the coarsest level.  The
+ * hierarchy is rebuilt from the current moduli at the start of every solve.
+ * nu = 0 switches the multigrid off. */
+int pf_mg_setup(pf_ctx* ctx, int nu, double omega);
+/* Build the hierarchy for the current moduli now (solves do it themselves). */
+int pf_mg_build(pf_ctx* ctx);
+/* One V-cycle z = V(r) on free-DOF host vectors, for the current moduli. */
+int pf_mg_apply(pf_ctx* ctx, const double* r_free, double* z_free);
+/* Level `level` of the built hierarchy: its element counts, and (when the
+ * pointers are not null) its FP64 element matrices (ne x 64, row-major per
+ * element, nodes BL, BR, TR, TL, x before y; level >= 1), FP32 diagonal and
+ * nodal fixity bits (bit 0 x, bit 1 y).  *levels receives the level count. */
+int pf_mg_level(pf_ctx* ctx, int level, int* levels, int* nx, int* ny, double* Ke, float* D, uint8_t* fix);
 
 /* ---- measurement -------------------------------------------------------- */
 /* Average device duration (HIP events on the context's stream, `reps`

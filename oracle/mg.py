@@ -1,0 +1,173 @@
+"""CPU restatement of the multigrid V-cycle used as the flexible CG's base
+preconditioner (pansfem2_amd/csrc/pf_mg.hip).  Test infrastructure only: the
+tests compare the device hierarchy and V-cycle with it.
+
+The multigrid has no counterpart in the reference (PANSFEM2 solves with
+ScalingCG, src/LinearAlgebra/Solvers/CG.h:397-453), so this restatement is
+pinned by mathematics instead of reference vectors (tests/test_mg_oracle.py):
+  - the element-wise Galerkin coarsening equals the assembled (M P)^T A (M P)
+    built from scipy sparse matrices (P bilinear, M the free-DOF mask);
+  - the V-cycle is a symmetric operator (<V a, b> = <a, V b>), as symmetric
+    smoothing around a Galerkin coarse correction must be;
+  - MG-preconditioned CG converges in a grid-independent handful of
+    iterations where Jacobi-PCG needs hundreds.
+
+Layout as in the device code: nodes n = (ny + 1) i + j, nodal vectors of 2
+entries per node, element (i, j) with nodes BL (i, j), BR (i+1, j),
+TR (i+1, j+1), TL (i, j+1), element index ny i + j.
+"""
+import numpy as np
+import scipy.sparse as sp
+
+LX = np.array([0, 1, 1, 0])
+LY = np.array([0, 0, 1, 1])
+
+
+def element_dofs(nx, ny):
+    I, J = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")
+    I, J = I.ravel(), J.ravel()
+    nodes = np.stack([(ny + 1) * (I + LX[k]) + J + LY[k] for k in range(4)], 1)
+    return np.stack([2 * nodes, 2 * nodes + 1], 2).reshape(-1, 8)
+
+
+def assemble(nx, ny, Ke):
+    """Sum of element matrices Ke (ne x 8 x 8) as a sparse nodal matrix."""
+    dofs = element_dofs(nx, ny)
+    r = np.repeat(dofs, 8, 1).ravel()
+    c = np.tile(dofs, (1, 8)).ravel()
+    n = 2 * (nx + 1) * (ny + 1)
+    return sp.csr_matrix((Ke.reshape(-1), (r, c)), shape=(n, n))
+
+
+def fix_dofs(fix):
+    """Nodal DOF mask (True = fixed) from fixity bits per node."""
+    f = np.asarray(fix, np.uint8)
+    return np.stack([(f & 1) > 0, (f & 2) > 0], 1).reshape(-1)
+
+
+def level0_matrices(nx, ny, E, K0, fix):
+    """E_e K0 with the rows and columns of fixed DOFs zeroed."""
+    Ke = np.asarray(E, float)[:, None, None] * np.asarray(K0, float).reshape(8, 8)[None]
+    fixed = fix_dofs(fix)[element_dofs(nx, ny)]          # ne x 8
+    keep = (~fixed).astype(float)
+    return Ke * keep[:, :, None] * keep[:, None, :]
+
+
+def child_prolongation(a0, a1):
+    """8 x 8 map from a coarse element's DOFs to child (a0, a1)'s DOFs."""
+    P = np.zeros((8, 8))
+    for k in range(4):
+        x, y = 0.5 * (a0 + LX[k]), 0.5 * (a1 + LY[k])
+        w = [(1 - x) * (1 - y), x * (1 - y), x * y, (1 - x) * y]
+        for q in range(4):
+            P[2 * k, 2 * q] = w[q]
+            P[2 * k + 1, 2 * q + 1] = w[q]
+    return P
+
+
+def coarsen(nx, ny, Ke):
+    """Galerkin coarse element matrices: Kc = sum over the 4 children of
+    P_a^T K_a P_a."""
+    cnx, cny = nx // 2, ny // 2
+    Kf = Ke.reshape(nx, ny, 8, 8)
+    Kc = np.zeros((cnx, cny, 8, 8))
+    for a0 in range(2):
+        for a1 in range(2):
+            P = child_prolongation(a0, a1)
+            Kc += np.einsum("ki,IJkl,lj->IJij", P, Kf[a0::2, a1::2], P)
+    return Kc.reshape(-1, 8, 8)
+
+
+def prolongation(nx, ny):
+    """Bilinear P from the (nx/2, ny/2) grid's nodes to the (nx, ny) grid's."""
+    cny = ny // 2
+    rows, cols, vals = [], [], []
+    for i in range(nx + 1):
+        for j in range(ny + 1):
+            I0, J0 = i // 2, j // 2
+            if i % 2 == 0 and j % 2 == 0:
+                ws = [(I0, J0, 1.0)]
+            elif i % 2 == 1 and j % 2 == 0:
+                ws = [(I0, J0, 0.5), (I0 + 1, J0, 0.5)]
+            elif i % 2 == 0:
+                ws = [(I0, J0, 0.5), (I0, J0 + 1, 0.5)]
+            else:
+                ws = [(I0, J0, 0.25), (I0 + 1, J0, 0.25), (I0, J0 + 1, 0.25), (I0 + 1, J0 + 1, 0.25)]
+            f = (ny + 1) * i + j
+            for I, J, w in ws:
+                c = (cny + 1) * I + J
+                for d in range(2):
+                    rows.append(2 * f + d)
+                    cols.append(2 * c + d)
+                    vals.append(w)
+    n_f = 2 * (nx + 1) * (ny + 1)
+    n_c = 2 * (nx // 2 + 1) * (ny // 2 + 1)
+    return sp.csr_matrix((vals, (rows, cols)), shape=(n_f, n_c))
+
+
+def build(nx, ny, E, K0, fix, coarse_dofs=90):
+    """Levels as the device builds them: halve while both element counts
+    are even and the level has more than `coarse_dofs` nodal DOFs."""
+    Ke = level0_matrices(nx, ny, E, K0, fix)
+    free0 = ~fix_dofs(fix)
+    levels = []
+    while True:
+        A = assemble(nx, ny, Ke)
+        d = A.diagonal().copy()
+        free = free0 if not levels else d > 0
+        levels.append(dict(nx=nx, ny=ny, Ke=Ke, A=A, free=free, D=np.where(free, d, 1.0)))
+        if 2 * (nx + 1) * (ny + 1) <= coarse_dofs or nx % 2 or ny % 2:
+            break
+        levels[-1]["P"] = prolongation(nx, ny)
+        Ke = coarsen(nx, ny, Ke)
+        nx, ny = nx // 2, ny // 2
+    c = levels[-1]
+    f = c["free"]
+    c["inv"] = np.linalg.inv(c["A"][f][:, f].toarray())
+    return levels
+
+
+def apply_level(lv, x):
+    return (lv["A"] @ (x * lv["free"])) * lv["free"]
+
+
+def vcycle(levels, r, nu=1, omega=0.6, k=0):
+    """Symmetric V-cycle: nu damped-Jacobi sweeps, Galerkin coarse correction
+    (restriction P^T), nu sweeps; dense solve on the coarsest level."""
+    lv = levels[k]
+    r = r * lv["free"]
+    if "inv" in lv:
+        x = np.zeros_like(r)
+        x[lv["free"]] = lv["inv"] @ r[lv["free"]]
+        return x
+    dinv = np.where(lv["free"], 1.0 / lv["D"], 0.0)
+    x = np.zeros_like(r)
+    for _ in range(nu):
+        x = x + omega * dinv * (r - apply_level(lv, x))
+    rc = lv["P"].T @ (r - apply_level(lv, x))
+    x = x + (lv["P"] @ vcycle(levels, rc, nu, omega, k + 1)) * lv["free"]
+    for _ in range(nu):
+        x = x + omega * dinv * (r - apply_level(lv, x))
+    return x
+
+
+def pcg(apply_A, b, M, eps=1e-8, itmax=100000):
+    """Preconditioned CG to a relative recursive residual eps; (x, iterations)."""
+    x = np.zeros_like(b)
+    r = b.copy()
+    z = M(r)
+    p = z.copy()
+    rz = r @ z
+    bn = np.linalg.norm(b)
+    for k in range(itmax):
+        q = apply_A(p)
+        a = rz / (p @ q)
+        x += a * p
+        r -= a * q
+        if np.linalg.norm(r) < eps * bn:
+            return x, k + 1
+        z = M(r)
+        rz1 = r @ z
+        p = z + rz1 / rz * p
+        rz = rz1
+    return x, itmax

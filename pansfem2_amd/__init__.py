@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libpansfem2_amd.so")
 
 PF_OK, PF_NOT_CONVERGED = 0, 1
-PF_OP_CSR64, PF_OP_EBE64, PF_OP_EBE32, PF_OP_LEARNED = 0, 1, 2, 3
+PF_OP_CSR64, PF_OP_EBE64, PF_OP_EBE32, PF_OP_LEARNED, PF_OP_MG = 0, 1, 2, 3, 4
 PF_FILTER_VARIABLES, PF_FILTER_SENSITIVITIES = 0, 1
 PF_PCG_SCALINGCG, PF_PCG_SINGLE_REDUCTION = 0, 1
 
@@ -102,6 +102,10 @@ def lib():
             "pf_pcg": ([c_void_p, c_int, _f64p, _f64p, c_double, c_int, P(SolveStats)], c_int),
             "pf_pcg_dev": ([c_void_p, c_int, c_void_p, c_void_p, c_double, c_int, P(SolveStats)], c_int),
             "pf_set_pcg_form": ([c_void_p, c_int], c_int),
+            "pf_mg_setup": ([c_void_p, c_int, c_double], c_int),
+            "pf_mg_build": ([c_void_p], c_int),
+            "pf_mg_apply": ([c_void_p, _f64p, _f64p], c_int),
+            "pf_mg_level": ([c_void_p, c_int, P(c_int), P(c_int), P(c_int), c_void_p, c_void_p, c_void_p], c_int),
             "pf_compliance_sens": ([c_void_p, _f64p, c_double, P(c_double), _f64p], c_int),
             "pf_filter_setup": ([c_void_p, c_double], c_int),
             "pf_filter": ([c_void_p, c_int, _f64p, _f64p], c_int),
@@ -278,6 +282,32 @@ class Context:
         y = np.zeros(self.N)
         self._ck(lib().pf_apply_K(self._h, op, _f64(x), y))
         return y
+
+    def mg_setup(self, nu=1, omega=0.6):
+        """Geometric multigrid V-cycle as the flexible CG's base step
+        (PF_OP_MG, and PF_OP_LEARNED's base); nu = 0 switches it off."""
+        self._ck(lib().pf_mg_setup(self._h, nu, omega))
+
+    def mg_build(self):
+        self._ck(lib().pf_mg_build(self._h))
+
+    def mg_apply(self, r):
+        z = np.zeros(self.N)
+        self._ck(lib().pf_mg_apply(self._h, _f64(r), z))
+        return z
+
+    def mg_level(self, level, want_ke=True):
+        """(nx, ny, Ke (ne x 8 x 8) or None, D, fix) of a built level."""
+        nl, nx, ny = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(0)
+        self._ck(lib().pf_mg_level(self._h, level, ctypes.byref(nl), ctypes.byref(nx), ctypes.byref(ny),
+                                   None, None, None))
+        nn = (nx.value + 1) * (ny.value + 1)
+        ke = np.zeros((nx.value * ny.value, 8, 8)) if (want_ke and level > 0) else None
+        D = np.zeros(2 * nn, np.float32)
+        fix = np.zeros(nn, np.uint8)
+        self._ck(lib().pf_mg_level(self._h, level, ctypes.byref(nl), ctypes.byref(nx), ctypes.byref(ny),
+                                   ke.ctypes.data if ke is not None else None, D.ctypes.data, fix.ctypes.data))
+        return dict(levels=nl.value, nx=nx.value, ny=ny.value, Ke=ke, D=D, fix=fix)
 
     def set_pcg_form(self, form):
         """PF_PCG_SCALINGCG (ScalingCG's order, the default) or

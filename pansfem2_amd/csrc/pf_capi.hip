@@ -5,6 +5,7 @@
 #include "pf_mma.hip"
 #include "pf_hex8.hip"
 #include "pf_learned.hip"
+#include "pf_mg.hip"
 
 #include <algorithm>
 #include <chrono>
@@ -17,14 +18,8 @@ using namespace pf;
 
 namespace {
 
-int grid_cap() {
-    static int cap = [] {
-        const char* s = std::getenv("PF_GRID_CAP");
-        int v = s ? std::atoi(s) : 1024;
-        return std::max(64, std::min(v, kMaxPartials));
-    }();
-    return cap;
-}
+// workgroups of the grid-stride 1D kernels
+constexpr int grid_cap() { return 1024; }
 
 // nodes per LDS tile of the quad4 PCG operator kernel (TI rows x TJ along j)
 #ifndef PF_K1_TI
@@ -41,25 +36,17 @@ constexpr int kTileJ = PF_K1_TJ, kTileI = PF_K1_TI;
 #ifndef PF_CG1_TJ
 #define PF_CG1_TJ 64
 #endif
-#ifndef PF_CG1_EDGE_FIRST
-#define PF_CG1_EDGE_FIRST 1
-#endif
+
 constexpr int kCg1TI = PF_CG1_TI, kCg1TJ = PF_CG1_TJ;
 
 // Launch shape of the quad4 operator kernel: tiles, and workgroups (a
-// multiple of 8, one group per XCD) so that each takes about kTilesPerBlock
-// tiles of its XCD's range.
+// multiple of 8, one group per XCD), one tile each (several tiles per
+// workgroup with the next tile's loads in flight measured slower).
 struct K1Shape {
     int tilesJ, ntiles;
     unsigned grid;
 };
-int k1_tiles_per_block() {
-    static int v = [] {
-        const char* s = std::getenv("PF_K1_TPB");
-        return s ? std::max(1, std::atoi(s)) : 1;
-    }();
-    return v;
-}
+constexpr int k1_tiles_per_block() { return 1; }
 K1Shape k1_shape(int nx, int ny) {
     K1Shape sh;
     sh.tilesJ = (ny + 1 + kTileJ - 1) / kTileJ;
@@ -72,14 +59,7 @@ K1Shape k1_shape(int nx, int ny) {
 
 // workgroups of the streaming PCG vector kernel (few, so that the operator
 // kernel's redundant read of its partials stays small)
-int k2_cap() {
-    static int cap = [] {
-        const char* s = std::getenv("PF_K2_GRID");
-        int v = s ? std::atoi(s) : 512;
-        return std::max(32, std::min(v, kMaxPartials));
-    }();
-    return cap;
-}
+constexpr int k2_cap() { return 512; }
 
 unsigned gsize(long n) { return (unsigned)std::max(1L, std::min((long)grid_cap(), (n + kBlock - 1) / kBlock)); }
 
@@ -124,7 +104,9 @@ bool is2d(pf_ctx* ctx) { return ctx->dim == 2; }
 void release_net(pf_ctx* ctx);
 G3 g3(pf_ctx* ctx) { return G3{ctx->nx, ctx->ny, ctx->nz}; }
 
+void mg_free(pf_ctx* ctx);
 void free_grid(pf_ctx* ctx) {
+    mg_free(ctx);
     dfree(ctx->d_n2g); dfree(ctx->d_g2n); dfree(ctx->d_fix);
     dfree(ctx->d_rho); dfree(ctx->d_E); dfree(ctx->d_Ef);
     dfree(ctx->d_x); dfree(ctx->d_r); dfree(ctx->d_z); dfree(ctx->d_q); dfree(ctx->d_p0); dfree(ctx->d_p1);
@@ -283,7 +265,7 @@ Cg1Tiles cg1_tiles(const pf_ctx* ctx, unsigned* grid) {
     G.tilesJ = (ctx->ny + 1 + kCg1TJ - 1) / kCg1TJ;
     G.perF = (G.fullI * G.fullJ + 7) / 8;
     G.nEdgePad = 8 * ((G.tilesI * G.tilesJ - G.fullI * G.fullJ + 7) / 8);
-    G.edgeFirst = PF_CG1_EDGE_FIRST;
+    G.edgeFirst = 1;
     *grid = (unsigned)(8 * G.perF + G.nEdgePad);
     return G;
 }
@@ -561,6 +543,29 @@ struct NetLayer {
     long size() const { return 9L * cin * cout; }
 };
 
+// Levels of the V-cycle (k_mg_* kernels): level 0 is the grid (E, K0, the
+// context's fixity); coarser levels hold FP64 and FP32 element matrices.
+struct MgLevel {
+    int nx = 0, ny = 0;
+    long nn = 0, nv = 0, ne = 0;
+    double* Ke64 = nullptr;  // levels >= 1, and level 0 when it is the coarsest
+    float* Ke32 = nullptr;   // levels >= 1
+    float* D = nullptr;
+    uint8_t* fix = nullptr;  // levels >= 1 (level 0 uses the context's)
+    float *r = nullptr, *x = nullptr, *x2 = nullptr, *res = nullptr, *res2 = nullptr;
+};
+struct MgState {
+    const int* done = nullptr;  // while set, the V-cycle's kernels return at once when *done
+    int tail = -1;              // first level of the one-workgroup tail (k_mg_tail), -1: none
+    int nu = 1;
+    float omega = 0.6f;
+    std::vector<MgLevel> L;
+    int nf = 0;             // free DOFs of the coarsest level
+    int* dof = nullptr;     // their nodal indices
+    double* A = nullptr;    // coarsest free-DOF block, inverted in place
+    double* W = nullptr;    // inversion scratch
+    std::vector<void*> owned;
+};
 struct LearnedNet {
     int H[4] = {0}, Wd[4] = {0};
     std::vector<NetLayer> L;  // c0a c0b c1a c1b c2a c2b c3a c3b d2 d1 d0 out
@@ -825,6 +830,304 @@ int net_backward(pf_ctx* ctx) {
     return check_launch(ctx);
 }
 
+
+// ------------------------------------------------------------ multigrid ----
+// levels with at least this many nodes use the LDS-tiled sweep
+constexpr long kMgTileNodes = 16384;
+// coarsening stops at the first level with at most this many nodal DOFs (a
+// 8 x 4 level: its free block is inverted in LDS)
+constexpr long kMgCoarse = 90;
+// largest coarsest level the dense solve takes (grids with odd sizes stop early)
+constexpr long kMgDenseMax = 2048;
+
+void mg_free(pf_ctx* ctx) {
+    if (!ctx->mg) return;
+    for (void* p : ctx->mg->owned) hipFree(p);
+    delete ctx->mg;
+    ctx->mg = nullptr;
+}
+
+template <class T>
+int mg_alloc(pf_ctx* ctx, T** p, long n) {
+    if (hipMalloc((void**)p, sizeof(T) * (size_t)std::max(1L, n)) != hipSuccess) {
+        ctx->err = "hipMalloc failed for the multigrid hierarchy";
+        return PF_ERR_NOMEM;
+    }
+    ctx->mg->owned.push_back(*p);
+    return PF_OK;
+}
+
+// levels and their storage for the current grid
+int mg_plan(pf_ctx* ctx, int nu, float omega) {
+    mg_free(ctx);
+    ctx->mg = new MgState;
+    MgState* S = ctx->mg;
+    S->nu = nu;
+    S->omega = omega;
+    int nx = ctx->nx, ny = ctx->ny;
+    for (;;) {
+        MgLevel a;
+        a.nx = nx;
+        a.ny = ny;
+        a.nn = (long)(nx + 1) * (ny + 1);
+        a.nv = 2 * a.nn;
+        a.ne = (long)nx * ny;
+        S->L.push_back(a);
+        if (a.nv <= kMgCoarse || nx % 2 || ny % 2) break;
+        nx /= 2;
+        ny /= 2;
+    }
+    const int c = (int)S->L.size() - 1;
+    PF_CHECK(S->L[c].nv <= kMgDenseMax, PF_ERR_UNSUPPORTED,
+             "multigrid: the grid does not coarsen to a small enough level (element counts must be divisible by "
+             "powers of two)");
+    for (int l = 0; l <= c; l++) {
+        MgLevel& a = S->L[l];
+        if (l > 0 || l == c) PF_TRY(mg_alloc(ctx, &a.Ke64, a.ne * 64));
+        if (l > 0) {
+            PF_TRY(mg_alloc(ctx, &a.Ke32, a.ne * 64));
+            PF_TRY(mg_alloc(ctx, &a.fix, a.nn));
+        }
+        float** v[] = {&a.D, &a.r, &a.x, &a.x2, &a.res, &a.res2};
+        for (float** q : v) PF_TRY(mg_alloc(ctx, q, a.nv));
+    }
+    // the small levels (and the coarsest solve) run in one workgroup
+    if (S->L[c].nv <= kMgLdsDense) {
+        int t = c;
+        while (t - 1 >= 1 && S->L[t - 1].nn <= kMgTailNodes && c - (t - 1) + 1 <= kMgTailMax) t--;
+        if (S->L[t].nn <= kMgTailNodes && t >= 1) S->tail = t;
+    }
+    const long nvc = S->L[c].nv;
+    PF_TRY(mg_alloc(ctx, &S->dof, nvc));
+    PF_TRY(mg_alloc(ctx, &S->A, nvc * nvc));
+    PF_TRY(mg_alloc(ctx, &S->W, nvc * nvc));
+    return PF_OK;
+}
+
+// coarse operators, diagonals and the coarsest inverse for the current moduli
+int mg_build(pf_ctx* ctx) {
+    MgState* S = ctx->mg;
+    PF_CHECK(S, PF_ERR_STATE, "multigrid not set up: call pf_mg_setup first");
+    const int c = (int)S->L.size() - 1;
+    MgLevel& l0 = S->L[0];
+    k_mg_diag0<<<gsize(l0.nn), kBlock, 0, ctx->stream>>>(l0.nx, l0.ny, ctx->d_E, ctx->d_K0, ctx->d_fix, l0.D);
+    for (int l = 1; l <= c; l++) {
+        MgLevel& a = S->L[l];
+        const long t = a.ne * 64;
+        k_mg_coarsen<<<(unsigned)((t + kBlock - 1) / kBlock), kBlock, 0, ctx->stream>>>(
+            a.nx, a.ny, ctx->d_E, ctx->d_K0, ctx->d_fix, l == 1 ? nullptr : S->L[l - 1].Ke64, a.Ke64, a.Ke32);
+        k_mg_diag_full<<<gsize(a.nn), kBlock, 0, ctx->stream>>>(a.nx, a.ny, a.Ke64, a.D, a.fix);
+    }
+    MgLevel& C = S->L[c];
+    if (c == 0)
+        k_mg_ke0_full<<<(unsigned)((C.ne * 64 + kBlock - 1) / kBlock), kBlock, 0, ctx->stream>>>(C.ne, ctx->d_E,
+                                                                                             ctx->d_K0, C.Ke64);
+    PF_TRY(check_launch(ctx));
+    // free DOFs of the coarsest level (its fixity is known only now)
+    std::vector<uint8_t> hf(C.nn);
+    PF_HIP(hipMemcpyAsync(hf.data(), c == 0 ? ctx->d_fix : C.fix, C.nn, hipMemcpyDeviceToHost, ctx->stream));
+    PF_TRY(sync(ctx));
+    std::vector<int> dof;
+    for (long n = 0; n < C.nn; n++)
+        for (int d = 0; d < 2; d++)
+            if (!((hf[n] >> d) & 1)) dof.push_back((int)(2 * n + d));
+    S->nf = (int)dof.size();
+    PF_CHECK(S->nf > 0, PF_ERR_STATE, "multigrid: no free DOFs on the coarsest level");
+    PF_HIP(hipMemcpyAsync(S->dof, dof.data(), sizeof(int) * dof.size(), hipMemcpyHostToDevice, ctx->stream));
+    const long nf2 = (long)S->nf * S->nf;
+    k_mg_dense<<<(unsigned)((nf2 + kBlock - 1) / kBlock), kBlock, 0, ctx->stream>>>(C.nx, C.ny, C.Ke64, S->dof,
+                                                                                   S->nf, S->A);
+    if (S->nf <= kMgLdsDense)
+        k_mg_invert_lds<<<1, kBlock, 0, ctx->stream>>>(S->nf, S->A);
+    else
+        k_mg_invert<<<1, kBlock, 0, ctx->stream>>>(S->nf, S->A, S->W);
+    PF_TRY(check_launch(ctx));
+    return sync(ctx);  // dof (host vector) must outlive its copy
+}
+
+// one V-cycle from level l on S->L[l].r; returns the buffer holding x
+float* mg_cycle(pf_ctx* ctx, int l) {
+    MgState* S = ctx->mg;
+    MgLevel& a = S->L[l];
+    const int c = (int)S->L.size() - 1;
+    if (l == S->tail) {
+        MgTail T{};
+        T.nlev = c - l + 1;
+        for (int k = 0; k < T.nlev; k++) {
+            MgLevel& b = S->L[l + k];
+            T.L[k] = MgTailLevel{b.nx, b.ny, b.Ke32, b.D, b.fix, b.r, b.x, b.x2, b.res, b.res2};
+        }
+        T.nf = S->nf;
+        T.Ainv = S->A;
+        T.dof = S->dof;
+        T.nu = S->nu;
+        T.omega = S->omega;
+        k_mg_tail<<<1, kMgTailNodes, 0, ctx->stream>>>(T, S->done);
+        if (T.nlev == 1) return a.x;
+        return mg_tail_final_is_x2(S->nu) ? a.x2 : a.x;
+    }
+    if (l == c) {
+        k_fill<float><<<gsize(a.nv), kBlock, 0, ctx->stream>>>(a.nv, a.x, 0.f);
+        k_mg_coarse_solve<<<(unsigned)((S->nf + 63) / 64), 64, 0, ctx->stream>>>(S->nf, S->A, S->dof, a.r, a.x,
+                                                                                S->done);
+        return a.x;
+    }
+    const uint8_t* fx = l == 0 ? ctx->d_fix : a.fix;
+    const unsigned g = gsize(a.nn);
+    float *x = a.x, *res = a.res, *xo = a.x2, *ro = a.res2;
+    auto sweep = [&](auto ke, const float* xi, const float* ri, const float* xc, float* xout, float* rout) {
+        if (a.nn >= kMgTileNodes) {  // large levels: LDS-tiled
+            const int tilesJ = (a.ny + 1 + 63) / 64, tiles = tilesJ * ((a.nx + 1 + 7) / 8);
+            k_mg_sweep_tile<decltype(ke), 8, 64><<<tiles, kBlock, 0, ctx->stream>>>(
+                a.nx, a.ny, tilesJ, ke, a.D, fx, S->omega, a.r, xi, ri, xc, xout, rout, S->done);
+        } else {
+            k_mg_sweep<<<g, kBlock, 0, ctx->stream>>>(a.nx, a.ny, ke, a.D, fx, S->omega, a.r, xi, ri, xc, xout,
+                                                      rout, S->done);
+        }
+    };
+    auto jac = [&](auto ke, const float* xi, float* xout) {
+        k_mg_jacobi<<<g, kBlock, 0, ctx->stream>>>(a.nx, a.ny, ke, a.D, fx, S->omega, a.r, xi, xout, S->done);
+    };
+    auto run = [&](auto ke) {
+        sweep(ke, nullptr, nullptr, nullptr, x, res);
+        for (int k = 1; k < S->nu; k++) {
+            sweep(ke, x, res, nullptr, xo, ro);
+            std::swap(x, xo);
+            std::swap(res, ro);
+        }
+        MgLevel& b = S->L[l + 1];
+        k_mg_restrict<<<gsize(b.nn), kBlock, 0, ctx->stream>>>(b.nx, b.ny, res, b.fix, b.r, S->done);
+        const float* xc = mg_cycle(ctx, l + 1);
+        sweep(ke, x, nullptr, xc, xo, nullptr);
+        std::swap(x, xo);
+        for (int k = 1; k < S->nu; k++) {
+            jac(ke, x, xo);
+            std::swap(x, xo);
+        }
+    };
+    if (l == 0)
+        run(MgKeShared{ctx->d_Ef, ctx->d_K0f});
+    else
+        run(MgKeFull{a.Ke32});
+    return x;
+}
+
+// PCG with z = V(r) (the hierarchy built for the current moduli), stream
+// ordered like pcg_device: k_pcg_k1_q4_tile (beta, convergence test on the
+// recursive residual, p, K p), k_mg_pcg_update (alpha, x, r, r in FP32), the
+// V-cycle, k_mg_pcg_z (z and the sums z.r, r.r); the host reads the done
+// flag once per chunk.  b, x nodal FP64; x starts at 0.
+int pcg_mg_device(pf_ctx* ctx, const double* b, double* x, double eps, int itrmax, PcgRun& out) {
+    MgState* M = ctx->mg;
+    const long n = ctx->nv;
+    PcgDev* S = (PcgDev*)ctx->d_pcg;
+    double* partA = ctx->red.partials;
+    double* partB = ctx->red.partials + 2 * kMaxPartials;
+    const K1Shape sh = k1_shape(ctx->nx, ctx->ny);
+    const unsigned g1 = sh.grid;
+    const unsigned g2 = std::min<unsigned>(gsize(n), (unsigned)k2_cap());
+    PF_CHECK(g1 <= (unsigned)kMaxPartials, PF_ERR_ARG, "grid too large for the reduction scratch");
+    double *r = ctx->d_r, *z = ctx->d_z, *p0 = ctx->d_p0, *p1 = ctx->d_p1, *q = ctx->d_q;
+    float* r32 = M->L[0].r;
+    // x = 0, r = b (k_pcg_init's z = b ./ D and its sums are replaced by V(b)'s)
+    k_pcg_init<double><<<g2, kBlock, 0, ctx->stream>>>(n, b, ctx->d_D, x, r, z, p0, partB, S);
+    k_d2f<<<gsize(n), kBlock, 0, ctx->stream>>>(n, b, r32);
+    M->done = &S->done;
+    k_mg_pcg_z<<<g2, kBlock, 0, ctx->stream>>>(n, mg_cycle(ctx, 0), r, z, partB, nullptr);
+    PF_TRY(check_launch(ctx));
+    const int chunk = 8;
+    int k = 0;
+    hipEvent_t ev[2];
+    PF_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+    PF_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+    volatile int* hflag = (volatile int*)ctx->h_red;
+    hflag[0] = hflag[1] = 0;
+    int slot = 0, pending = 0;
+    bool done = false;
+    int rc = PF_OK;
+    while (!done && k < itrmax) {
+        const int kend = std::min(itrmax, k + chunk);
+        for (; k < kend; k++) {
+            double* pold = (k & 1) ? p1 : p0;
+            double* pnew = (k & 1) ? p0 : p1;
+            k_pcg_k1_q4_tile<double, kTileI, kTileJ><<<g1, kBlock, 0, ctx->stream>>>(
+                ctx->nx, ctx->ny, sh.tilesJ, sh.ntiles, k, pold, z, pnew, q, ctx->d_E, ctx->d_K0, ctx->d_fix, partB,
+                (int)g2, partA, S, eps);
+            k_mg_pcg_update<<<g2, kBlock, 0, ctx->stream>>>(n, k, x, r, pnew, q, r32, partA, (int)g1, S);
+            k_mg_pcg_z<<<g2, kBlock, 0, ctx->stream>>>(n, mg_cycle(ctx, 0), r, z, partB, &S->done);
+        }
+        if ((rc = check_launch(ctx)) < 0) break;
+        if (hipMemcpyAsync((void*)&hflag[slot], &S->done, sizeof(int), hipMemcpyDeviceToHost, ctx->stream) !=
+                hipSuccess ||
+            hipEventRecord(ev[slot], ctx->stream) != hipSuccess) {
+            rc = PF_ERR_HIP;
+            break;
+        }
+        if (pending) {
+            hipEventSynchronize(ev[slot ^ 1]);
+            if (hflag[slot ^ 1]) done = true;
+        }
+        pending = 1;
+        slot ^= 1;
+    }
+    M->done = nullptr;
+    hipStreamSynchronize(ctx->stream);
+    hipEventDestroy(ev[0]);
+    hipEventDestroy(ev[1]);
+    PF_TRY(rc);
+    k_pcg_final<<<1, kBlock, 0, ctx->stream>>>(k, partB, (int)g2, S, eps);
+    PF_TRY(check_launch(ctx));
+    PF_HIP(hipMemcpyAsync(ctx->h_pcg, S, sizeof(PcgDev), hipMemcpyDeviceToHost, ctx->stream));
+    PF_TRY(sync(ctx));
+    const PcgDev* H = (const PcgDev*)ctx->h_pcg;
+    out.iterations = H->iterations;
+    out.status = H->status;
+    out.rnorm = H->rnorm;
+    out.bnorm = H->bnorm;
+    return PF_OK;
+}
+
+// Multigrid-preconditioned solve to an FP64 true residual eps ||b||: the
+// device PCG above stops on its recursive residual; while the true residual
+// is above eps ||b|| (at most 4 rounds), a correction K d = b - K x is solved
+// the same way and added.
+int solve_mg(pf_ctx* ctx, const double* b, double* x, double eps, int itrmax, pf_solve_stats& st) {
+    PF_TRY(diag_nodal(ctx, false));
+    PF_TRY(mg_build(ctx));
+    int rc;
+    const double bn = norm_nodal(ctx, b, &rc);
+    PF_TRY(rc);
+    PcgRun run;
+    PF_TRY(pcg_mg_device(ctx, b, x, eps, itrmax, run));
+    int total = run.iterations, rounds = 1;
+    double tn;
+    PF_TRY(true_residual(ctx, b, x, ctx->d_t0, &tn));
+    while (tn > eps * bn && rounds < 4 && total < itrmax && std::isfinite(tn)) {
+        PF_HIP(hipMemcpyAsync(ctx->d_t1, ctx->d_t0, sizeof(double) * ctx->nv, hipMemcpyDeviceToDevice, ctx->stream));
+        double* d = ctx->d_s0;  // the correction (the single-reduction form's buffer, free here)
+        PF_TRY(pcg_mg_device(ctx, ctx->d_t1, d, eps * bn / tn, itrmax - total, run));
+        k_axpy_d<<<gsize(ctx->nv), kBlock, 0, ctx->stream>>>(ctx->nv, 1.0, d, x);
+        PF_TRY(check_launch(ctx));
+        total += run.iterations;
+        rounds++;
+        PF_TRY(true_residual(ctx, b, x, ctx->d_t0, &tn));
+    }
+    st.iterations = total;
+    st.outer = rounds;
+    st.status = (tn <= eps * bn) ? 1 : 2;
+    st.rel_residual = bn > 0 ? tn / bn : 0.0;
+    return PF_OK;
+}
+
+// z = V(r) for nodal FP64 vectors (the hierarchy built for the current moduli)
+int mg_apply(pf_ctx* ctx, const double* r, double* z) {
+    MgLevel& a = ctx->mg->L[0];
+    k_d2f<<<gsize(a.nv), kBlock, 0, ctx->stream>>>(a.nv, r, a.r);
+    const float* x = mg_cycle(ctx, 0);
+    k_f2d<<<gsize(a.nv), kBlock, 0, ctx->stream>>>(a.nv, x, z);
+    return check_launch(ctx);
+}
+
 double dot_nodal(pf_ctx* ctx, const double* a, const double* b, int* rc);
 int optimal_pair(pf_ctx* ctx, const double* r, double* a, double* c);
 
@@ -845,7 +1148,10 @@ int net_apply(pf_ctx* ctx, const double* rho, const double* r, double* d, double
     k_net_input<<<gsize(ctx->nn), kBlock, 0, ctx->stream>>>(ctx->nx, ctx->ny, rho, r, rs, N->x0);
     PF_TRY(net_forward(ctx));
     k_net_y_nodal<<<gsize(ctx->nn), kBlock, 0, ctx->stream>>>(ctx->nn, N->y, ctx->d_fix, N->w);
-    k_div_d<<<gsize(nv), kBlock, 0, ctx->stream>>>(nv, r, ctx->d_D, N->dd);   // t = r ./ D
+    if (ctx->mg)
+        PF_TRY(mg_apply(ctx, r, N->dd));  // t = V(r)
+    else
+        k_div_d<<<gsize(nv), kBlock, 0, ctx->stream>>>(nv, r, ctx->d_D, N->dd);   // t = r ./ D
     PF_TRY(apply_nodal(ctx, false, N->dd, N->gd, true));                     // K t
     PF_TRY(apply_nodal(ctx, false, N->w, N->Kw, true));                      // K y
     double a = 1.0, c = 0.0;
@@ -979,7 +1285,12 @@ int net_train_step(pf_ctx* ctx, int s, float lr, double* loss) {
     PF_TRY(net_forward(ctx));
     k_net_y_nodal<<<gsize(ctx->nn), kBlock, 0, ctx->stream>>>(ctx->nn, N->y, ctx->d_fix, N->w);   // y
     PF_TRY(apply_nodal(ctx, false, N->w, N->Kw, true));                                           // K y
-    k_div_d<<<gsize(nv), kBlock, 0, ctx->stream>>>(nv, N->rr, ctx->d_D, N->dd);                   // t
+    if (ctx->mg) {
+        PF_TRY(mg_build(ctx));                                                                     // this pair's moduli
+        PF_TRY(mg_apply(ctx, N->rr, N->dd));                                                       // t = V(r)
+    } else {
+        k_div_d<<<gsize(nv), kBlock, 0, ctx->stream>>>(nv, N->rr, ctx->d_D, N->dd);               // t = r ./ D
+    }
     PF_TRY(apply_nodal(ctx, false, N->dd, N->gd, true));                                          // K t
     const double er = dot_nodal(ctx, N->ee, N->rr, &rc);
     PF_TRY(rc);
@@ -1030,21 +1341,33 @@ int net_collect_random(pf_ctx* ctx, int count, uint64_t seed, int* total) {
     return PF_OK;
 }
 
-// Stored directions of the flexible CG.  With m = 8 the 96 x 48 cantilever
-// needs about 1.4x the iterations of m = 32; CoefPack/PtrPack hold 32.
+// Stored directions of the flexible CG.  With the network over the Jacobi
+// step, m = 8 needs about 1.4x the iterations of m = 32 on the 96 x 48
+// cantilever; over the V-cycle the preconditioner varies far less and a few
+// directions suffice.  CoefPack/PtrPack hold 32.
 constexpr int kFcgDirections = 32;
+constexpr int kFcgDirectionsMg = 8;
+constexpr int kFcgDirectionsMgOnly = 2;
 
-// Flexible CG (Notay's FCG(m)) with M = the network correction: each new
-// direction is made K-conjugate to the last m stored ones, which keeps the
-// Krylov-like acceleration that a varying preconditioner would otherwise
-// lose.  FP64 vectors; stops on the FP64 true residual; when the network
-// breaks down, diverges (residual 10x above its best) or stagnates (no
-// halving in `stall` steps) Jacobi-PCG takes over from the current iterate.
-int pcg_learned(pf_ctx* ctx, const double* b, double* x, double eps, int itrmax, pf_solve_stats& st) {
+// Flexible CG (Notay's FCG(m)): each new direction is made K-conjugate to
+// the last m stored ones, which keeps the Krylov-like acceleration that a
+// varying preconditioner would otherwise lose.  M is the network correction
+// over the base step (use_net), or the base step alone; the base step is the
+// multigrid V-cycle after pf_mg_setup, else the Jacobi step r ./ D.  FP64
+// vectors; stops on the FP64 true residual; when M breaks down, diverges
+// (residual 10x above its best) or stagnates (no halving in `stall` steps)
+// Jacobi-PCG takes over from the current iterate.
+int pcg_learned(pf_ctx* ctx, const double* b, double* x, double eps, int itrmax, pf_solve_stats& st,
+                bool use_net = true) {
     LearnedNet* N = ctx->net;
-    PF_CHECK(N && N->trained, PF_ERR_STATE, "learned preconditioner has not been trained");
+    if (use_net) PF_CHECK(N && N->trained, PF_ERR_STATE, "learned preconditioner has not been trained");
+    PF_CHECK(use_net || ctx->mg, PF_ERR_STATE, "multigrid not set up: call pf_mg_setup first");
+    if (!N) {
+        PF_TRY(net_build(ctx, 1));  // work buffers for the directions (the network itself stays untrained)
+        N = ctx->net;
+    }
     const long nv = ctx->nv;
-    const int m = kFcgDirections;
+    const int m = !ctx->mg ? kFcgDirections : (use_net ? kFcgDirectionsMg : kFcgDirectionsMgOnly);
     if (N->fcg_m != m) {
         N->fP.assign(m, nullptr);
         N->fQ.assign(m, nullptr);
@@ -1057,6 +1380,7 @@ int pcg_learned(pf_ctx* ctx, const double* b, double* x, double eps, int itrmax,
     std::vector<double> pq(m, 0.0);
     int rc;
     PF_TRY(diag_nodal(ctx, false));
+    if (ctx->mg) PF_TRY(mg_build(ctx));
     const double bn = norm_nodal(ctx, b, &rc);
     PF_TRY(rc);
     double *r = ctx->d_r, *z = ctx->d_z;
@@ -1067,7 +1391,10 @@ int pcg_learned(pf_ctx* ctx, const double* b, double* x, double eps, int itrmax,
     const int stall = 2000;
     bool fallback = false;
     for (; k < itrmax; k++) {
-        PF_TRY(net_apply(ctx, ctx->d_rho, r, z, nullptr, rn));
+        if (use_net)
+            PF_TRY(net_apply(ctx, ctx->d_rho, r, z, nullptr, rn));
+        else
+            PF_TRY(mg_apply(ctx, r, z));
         const int nst = std::min(k, m);
         const int slot = k % m;
         double* p = N->fP[slot];
@@ -1209,6 +1536,9 @@ int solve_nodal(pf_ctx* ctx, int op, const double* b, double* x, double eps, int
         PF_TRY(pcg_ir(ctx, b, x, eps, itrmax, inner_eps, s));
     } else if (op == PF_OP_LEARNED) {
         PF_TRY(pcg_learned(ctx, b, x, eps, itrmax, s));
+    } else if (op == PF_OP_MG) {
+        PF_CHECK(ctx->mg, PF_ERR_STATE, "multigrid not set up: call pf_mg_setup first");
+        PF_TRY(solve_mg(ctx, b, x, eps, itrmax, s));
     } else {
         ctx->err = "unknown operator";
         return PF_ERR_ARG;
@@ -1879,6 +2209,62 @@ int pf_pcg_dev(pf_ctx* ctx, int op, const double* b_free_dev, double* x_free_dev
     PF_TRY(to_free(ctx, ctx->d_x, x_free_dev));
     PF_TRY(sync(ctx));
     return rc;
+}
+
+int pf_mg_setup(pf_ctx* ctx, int nu, double omega) {
+    if (!ctx) return PF_ERR_ARG;
+    PF_TRY(require_ready(ctx, false));
+    PF_TRY(set_device(ctx));
+    if (nu == 0) {
+        mg_free(ctx);
+        return PF_OK;
+    }
+    PF_CHECK(is2d(ctx), PF_ERR_UNSUPPORTED, "pf_mg_setup: quad4 grids only");
+    PF_CHECK(nu >= 1 && nu <= 8, PF_ERR_ARG, "pf_mg_setup: nu must be 1..8 (0 to switch off)");
+    PF_CHECK(omega > 0.0 && omega <= 1.0, PF_ERR_ARG, "pf_mg_setup: omega must be in (0, 1]");
+    const int rc = mg_plan(ctx, nu, (float)omega);
+    if (rc < 0) mg_free(ctx);
+    return rc;
+}
+
+int pf_mg_build(pf_ctx* ctx) {
+    if (!ctx) return PF_ERR_ARG;
+    PF_TRY(require_ready(ctx, true));
+    PF_TRY(set_device(ctx));
+    return mg_build(ctx);
+}
+
+int pf_mg_apply(pf_ctx* ctx, const double* r_free, double* z_free) {
+    if (!ctx || !r_free || !z_free) return PF_ERR_ARG;
+    PF_TRY(require_ready(ctx, true));
+    PF_TRY(set_device(ctx));
+    PF_TRY(mg_build(ctx));
+    PF_HIP(hipMemcpyAsync(ctx->d_t1, r_free, sizeof(double) * ctx->N, hipMemcpyHostToDevice, ctx->stream));
+    PF_TRY(to_nodal(ctx, ctx->d_t1, ctx->d_r));
+    PF_TRY(mg_apply(ctx, ctx->d_r, ctx->d_z));
+    PF_TRY(to_free(ctx, ctx->d_z, ctx->d_t0));
+    PF_HIP(hipMemcpyAsync(z_free, ctx->d_t0, sizeof(double) * ctx->N, hipMemcpyDeviceToHost, ctx->stream));
+    return sync(ctx);
+}
+
+int pf_mg_level(pf_ctx* ctx, int level, int* levels, int* nx, int* ny, double* Ke, float* D, uint8_t* fix) {
+    if (!ctx) return PF_ERR_ARG;
+    PF_CHECK(ctx->mg, PF_ERR_STATE, "multigrid not set up: call pf_mg_setup first");
+    PF_TRY(set_device(ctx));
+    MgState* S = ctx->mg;
+    if (levels) *levels = (int)S->L.size();
+    PF_CHECK(level >= 0 && level < (int)S->L.size(), PF_ERR_ARG, "pf_mg_level: no such level");
+    const MgLevel& a = S->L[level];
+    if (nx) *nx = a.nx;
+    if (ny) *ny = a.ny;
+    if (Ke) {
+        PF_CHECK(level >= 1, PF_ERR_ARG, "pf_mg_level: level 0's element matrices are E_e K0");
+        PF_HIP(hipMemcpyAsync(Ke, a.Ke64, sizeof(double) * a.ne * 64, hipMemcpyDeviceToHost, ctx->stream));
+    }
+    if (D) PF_HIP(hipMemcpyAsync(D, a.D, sizeof(float) * a.nv, hipMemcpyDeviceToHost, ctx->stream));
+    if (fix)
+        PF_HIP(hipMemcpyAsync(fix, level == 0 ? ctx->d_fix : a.fix, a.nn, hipMemcpyDeviceToHost, ctx->stream));
+    return sync(ctx);
 }
 
 int pf_set_pcg_form(pf_ctx* ctx, int form) {

@@ -130,6 +130,9 @@ struct pf_ctx {
     struct MmaState* mma = nullptr;
     // learned preconditioner
     struct LearnedNet* net = nullptr;
+
+    // multigrid base preconditioner of the flexible CG (pf_mg_setup)
+    struct MgState* mg = nullptr;
 };
 
 // ---------------------------------------------------------------- errors ----

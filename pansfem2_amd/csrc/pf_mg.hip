@@ -1,0 +1,664 @@
+// Geometric multigrid V-cycle for the quad4 matrix-free operator, used as the
+// base preconditioner of the flexible CG (no counterpart in the reference).
+//
+// Hierarchy: level 0 is the grid itself (moduli E_e times the shared K0);
+// level l+1 halves both element counts.  A coarse element's matrix is the
+// Galerkin product of its four children's matrices,
+//   Kc = sum_a P_a^T K_a P_a,
+// with P_a the bilinear interpolation from the coarse element's four nodes to
+// child a's four nodes (the same weights for both DOF directions).  Since a
+// child's nodes are interpolated from its own coarse element only, summing
+// these element products is exactly the assembled R A P with R = P^T.  Fixed
+// DOFs are removed at level 0 by zeroing their rows and columns in the
+// children's matrices, so the coarse operators are (M P)^T A (M P) whatever
+// the Dirichlet set; a coarse DOF with no stiffness left is treated as fixed.
+// Coarse matrices are built in FP64 and applied in FP32.
+//
+// V-cycle (symmetric, nu Jacobi sweeps before and after, weight omega):
+//   x = 0;  nu times: x += omega D^-1 (r - A x)
+//   r_c = P^T (r - A x);  x += P V(r_c);  nu times: x += omega D^-1 (r - A x)
+// and at the coarsest level a dense solve with the inverse of the free-DOF
+// block.  Vectors are nodal (2 entries per node, fixed entries 0); all
+// operator applications gather node by node over the (up to) four adjacent
+// elements, as the level-0 operator kernels do.
+#pragma once
+
+#include "pf_device.h"
+
+namespace pf {
+
+// element-matrix sources: row r, column c (node-major, x then y) of element e
+struct MgKeShared {  // level 0: E_e K0 (FP32)
+    const float* E;
+    const float* K0;
+    __device__ float operator()(long e, int r, int c) const { return E[e] * K0[r * 8 + c]; }
+};
+struct MgKeFull {  // coarse levels: one 8x8 matrix per element (FP32)
+    const float* Ke;
+    __device__ float operator()(long e, int r, int c) const { return Ke[e * 64 + r * 8 + c]; }
+};
+
+// (A x) at node (i, j); xf(ii, jj) gives x at a neighbouring node
+template <class KE, class XF>
+__device__ inline void mg_node_apply(int nx, int ny, int i, int j, const KE& ke, const XF& xf, float& ax,
+                                     float& ay) {
+#pragma clang fp contract(fast)  // preconditioner arithmetic: FMA allowed
+    ax = 0.f;
+    ay = 0.f;
+    auto elem = [&](int ei, int ej, int loc) {
+        const long e = (long)ny * ei + ej;
+        const Vec2<float> v[4] = {xf(ei, ej), xf(ei + 1, ej), xf(ei + 1, ej + 1), xf(ei, ej + 1)};
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            ax += ke(e, 2 * loc, 2 * c) * v[c].x + ke(e, 2 * loc, 2 * c + 1) * v[c].y;
+            ay += ke(e, 2 * loc + 1, 2 * c) * v[c].x + ke(e, 2 * loc + 1, 2 * c + 1) * v[c].y;
+        }
+    };
+    if (i >= 1 && j >= 1) elem(i - 1, j - 1, 2);  // this node is the element's TR
+    if (i >= 1 && j < ny) elem(i - 1, j, 1);      // BR
+    if (i < nx && j >= 1) elem(i, j - 1, 3);      // TL
+    if (i < nx && j < ny) elem(i, j, 0);          // BL
+}
+
+__device__ inline Vec2<float> mg_masked(Vec2<float> v, uint8_t f) {
+    if (f & 1) v.x = 0.f;
+    if (f & 2) v.y = 0.f;
+    return v;
+}
+
+// One Jacobi sweep at node n.  Pre-smoothing form (xc null): with
+// res = r - A x given,
+//   x' = x + omega D^-1 res,  res' = r - A x'
+// (x and res null: x = 0, res = r, the V-cycle's first sweep), x' formed on
+// the fly at the neighbours.  Post-smoothing form (xc non-null): x first
+// gets the prolongation P xc of the coarse correction, then
+//   x' = (x + P xc) + omega D^-1 (r - A (x + P xc)),
+// with no residual output (res and res_out must be null).
+template <class KE>
+__device__ inline void mg_sweep_at(long n, int nx, int ny, const KE& ke, const float* __restrict__ D,
+                                   const uint8_t* __restrict__ fix, float omega, const float* __restrict__ r,
+                                   const float* __restrict__ x, const float* __restrict__ res,
+                                   const float* __restrict__ xc, float* __restrict__ x_out,
+                                   float* __restrict__ res_out) {
+#pragma clang fp contract(fast)
+    const int cny = ny / 2;
+    // x at node (ii, jj) before the sweep: x + P xc (prolongation on the fly)
+    auto xin = [&](int ii, int jj) -> Vec2<float> {
+        const long m = (long)(ny + 1) * ii + jj;
+        Vec2<float> v = x ? load2(x, m) : Vec2<float>{0.f, 0.f};
+        if (xc) {
+            const int I0 = ii >> 1, J0 = jj >> 1, oi = ii & 1, oj = jj & 1;
+            auto cv = [&](int I, int J) { return load2(xc, (long)(cny + 1) * I + J); };
+            Vec2<float> p = cv(I0, J0);
+            if (oi && oj) {
+                const Vec2<float> a = cv(I0 + 1, J0), b = cv(I0, J0 + 1), c = cv(I0 + 1, J0 + 1);
+                p.x = 0.25f * (p.x + a.x + b.x + c.x);
+                p.y = 0.25f * (p.y + a.y + b.y + c.y);
+            } else if (oi) {
+                const Vec2<float> a = cv(I0 + 1, J0);
+                p.x = 0.5f * (p.x + a.x);
+                p.y = 0.5f * (p.y + a.y);
+            } else if (oj) {
+                const Vec2<float> a = cv(I0, J0 + 1);
+                p.x = 0.5f * (p.x + a.x);
+                p.y = 0.5f * (p.y + a.y);
+            }
+            v.x += p.x;
+            v.y += p.y;
+        }
+        return mg_masked(v, fix[m]);
+    };
+    // x' at a node in the pre-smoothing form: x + omega D^-1 res
+    auto xnew = [&](int ii, int jj) -> Vec2<float> {
+        const long m = (long)(ny + 1) * ii + jj;
+        const Vec2<float> rs = res ? load2(res, m) : load2(r, m);
+        const Vec2<float> d = load2(D, m);
+        Vec2<float> v = xin(ii, jj);
+        v.x += omega * rs.x / d.x;
+        v.y += omega * rs.y / d.y;
+        return mg_masked(v, fix[m]);
+    };
+    const int i = (int)(n / (ny + 1)), j = (int)(n - (long)i * (ny + 1));
+    const uint8_t f = fix[n];
+    const Vec2<float> rn = load2(r, n);
+    Vec2<float> xo;
+    if (xc) {
+        float ax, ay;
+        mg_node_apply(nx, ny, i, j, ke, xin, ax, ay);
+        const Vec2<float> d = load2(D, n);
+        xo = xin(i, j);
+        xo.x += omega * (rn.x - ax) / d.x;
+        xo.y += omega * (rn.y - ay) / d.y;
+        xo = mg_masked(xo, f);
+    } else {
+        xo = xnew(i, j);
+    }
+    store2(x_out, n, xo);
+    if (res_out) {
+        float ax, ay;
+        mg_node_apply(nx, ny, i, j, ke, xnew, ax, ay);
+        store2(res_out, n, mg_masked(Vec2<float>{rn.x - ax, rn.y - ay}, f));
+    }
+}
+
+template <class KE>
+__global__ void __launch_bounds__(256) k_mg_sweep(int nx, int ny, KE ke, const float* __restrict__ D,
+                                                  const uint8_t* __restrict__ fix, float omega,
+                                                  const float* __restrict__ r, const float* __restrict__ x,
+                                                  const float* __restrict__ res, const float* __restrict__ xc,
+                                                  float* __restrict__ x_out, float* __restrict__ res_out,
+                                                  const int* __restrict__ done) {
+    if (done && *done) return;
+    const long nn = (long)(nx + 1) * (ny + 1);
+    for (long n = blockIdx.x * (long)blockDim.x + threadIdx.x; n < nn; n += (long)gridDim.x * blockDim.x)
+        mg_sweep_at(n, nx, ny, ke, D, fix, omega, r, x, res, xc, x_out, res_out);
+}
+
+// k_mg_sweep LDS-tiled, for the large levels: a workgroup stages the sweep's
+// input vector (x' in the pre-smoothing form, x + P xc in the
+// post-smoothing form) for a TI x TJ tile and its one-node halo, then
+// applies the element matrices node by node from LDS (each element's rows
+// are read by its four nodes once each).
+template <class KE, int TI, int TJ>
+__global__ void __launch_bounds__(256) k_mg_sweep_tile(int nx, int ny, int tilesJ, KE ke, const float* __restrict__ D,
+                                                       const uint8_t* __restrict__ fix, float omega,
+                                                       const float* __restrict__ r, const float* __restrict__ x,
+                                                       const float* __restrict__ res, const float* __restrict__ xc,
+                                                       float* __restrict__ x_out, float* __restrict__ res_out,
+                                                       const int* __restrict__ done) {
+#pragma clang fp contract(fast)
+    if (done && *done) return;
+    constexpr int HALO = (TI + 2) * (TJ + 2), PER = (HALO + 255) / 256;
+    __shared__ Vec2<float> sx[TI + 2][TJ + 2];
+    const int t = blockIdx.x;
+    const int j0 = (t % tilesJ) * TJ, i0 = (t / tilesJ) * TI;
+    const int cny = ny / 2;
+#pragma unroll
+    for (int u = 0; u < PER; u++) {
+        const int h = threadIdx.x + u * 256;
+        if (h >= HALO) break;
+        const int a = h / (TJ + 2), b = h - a * (TJ + 2);
+        const int ii = i0 - 1 + a, jj = j0 - 1 + b;
+        Vec2<float> v{0.f, 0.f};
+        if (ii >= 0 && ii <= nx && jj >= 0 && jj <= ny) {
+            const long m = (long)(ny + 1) * ii + jj;
+            if (x) v = load2(x, m);
+            if (xc) {
+                // prolongation of the coarse correction
+                const int I0 = ii >> 1, J0 = jj >> 1, oi = ii & 1, oj = jj & 1;
+                auto cv = [&](int I, int J) { return load2(xc, (long)(cny + 1) * I + J); };
+                Vec2<float> p = cv(I0, J0);
+                if (oi && oj) {
+                    const Vec2<float> q1 = cv(I0 + 1, J0), q2 = cv(I0, J0 + 1), q3 = cv(I0 + 1, J0 + 1);
+                    p.x = 0.25f * (p.x + q1.x + q2.x + q3.x);
+                    p.y = 0.25f * (p.y + q1.y + q2.y + q3.y);
+                } else if (oi) {
+                    const Vec2<float> q1 = cv(I0 + 1, J0);
+                    p.x = 0.5f * (p.x + q1.x);
+                    p.y = 0.5f * (p.y + q1.y);
+                } else if (oj) {
+                    const Vec2<float> q1 = cv(I0, J0 + 1);
+                    p.x = 0.5f * (p.x + q1.x);
+                    p.y = 0.5f * (p.y + q1.y);
+                }
+                v.x += p.x;
+                v.y += p.y;
+            } else {
+                const Vec2<float> rs = res ? load2(res, m) : load2(r, m);
+                const Vec2<float> d = load2(D, m);
+                v.x += omega * rs.x / d.x;
+                v.y += omega * rs.y / d.y;
+            }
+            v = mg_masked(v, fix[m]);
+        }
+        sx[a][b] = v;
+    }
+    __syncthreads();
+    const int c = threadIdx.x % TJ;
+    for (int rr = threadIdx.x / TJ; rr < TI; rr += 256 / TJ) {
+        const int i = i0 + rr, j = j0 + c;
+        if (i > nx || j > ny) continue;
+        // (A v) at (i, j) from the staged neighbourhood
+        auto xf = [&](int ii, int jj) { return sx[ii - i0 + 1][jj - j0 + 1]; };
+        float ax, ay;
+        mg_node_apply(nx, ny, i, j, ke, xf, ax, ay);
+        const long n = (long)(ny + 1) * i + j;
+        const uint8_t f = fix[n];
+        const Vec2<float> rn = load2(r, n), xv = sx[rr + 1][c + 1];
+        if (xc) {
+            const Vec2<float> d = load2(D, n);
+            Vec2<float> xo{xv.x + omega * (rn.x - ax) / d.x, xv.y + omega * (rn.y - ay) / d.y};
+            store2(x_out, n, mg_masked(xo, f));
+        } else {
+            store2(x_out, n, xv);
+            if (res_out) store2(res_out, n, mg_masked(Vec2<float>{rn.x - ax, rn.y - ay}, f));
+        }
+    }
+}
+
+// plain Jacobi sweep x' = x + omega D^-1 (r - A x) at node n
+template <class KE>
+__device__ inline void mg_jacobi_at(long n, int nx, int ny, const KE& ke, const float* __restrict__ D,
+                                    const uint8_t* __restrict__ fix, float omega, const float* __restrict__ r,
+                                    const float* __restrict__ x, float* __restrict__ x_out) {
+#pragma clang fp contract(fast)
+    auto xf = [&](int ii, int jj) { return load2(x, (long)(ny + 1) * ii + jj); };
+    const int i = (int)(n / (ny + 1)), j = (int)(n - (long)i * (ny + 1));
+    float ax, ay;
+    mg_node_apply(nx, ny, i, j, ke, xf, ax, ay);
+    const Vec2<float> rn = load2(r, n), d = load2(D, n);
+    Vec2<float> v = load2(x, n);
+    v.x += omega * (rn.x - ax) / d.x;
+    v.y += omega * (rn.y - ay) / d.y;
+    store2(x_out, n, mg_masked(v, fix[n]));
+}
+
+template <class KE>
+__global__ void __launch_bounds__(256) k_mg_jacobi(int nx, int ny, KE ke, const float* __restrict__ D,
+                                                   const uint8_t* __restrict__ fix, float omega,
+                                                   const float* __restrict__ r, const float* __restrict__ x,
+                                                   float* __restrict__ x_out, const int* __restrict__ done) {
+    if (done && *done) return;
+    const long nn = (long)(nx + 1) * (ny + 1);
+    for (long n = blockIdx.x * (long)blockDim.x + threadIdx.x; n < nn; n += (long)gridDim.x * blockDim.x)
+        mg_jacobi_at(n, nx, ny, ke, D, fix, omega, r, x, x_out);
+}
+
+// res = r - A x (masked)
+template <class KE>
+__global__ void __launch_bounds__(256) k_mg_residual(int nx, int ny, KE ke, const uint8_t* __restrict__ fix,
+                                                     const float* __restrict__ r, const float* __restrict__ x,
+                                                     float* __restrict__ res) {
+#pragma clang fp contract(fast)
+    const long nn = (long)(nx + 1) * (ny + 1);
+    auto xf = [&](int ii, int jj) { return load2(x, (long)(ny + 1) * ii + jj); };
+    for (long n = blockIdx.x * (long)blockDim.x + threadIdx.x; n < nn; n += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(n / (ny + 1)), j = (int)(n - (long)i * (ny + 1));
+        float ax, ay;
+        mg_node_apply(nx, ny, i, j, ke, xf, ax, ay);
+        const Vec2<float> rn = load2(r, n);
+        store2(res, n, mg_masked(Vec2<float>{rn.x - ax, rn.y - ay}, fix[n]));
+    }
+}
+
+// r_c = P^T res: coarse node (I, J) gathers fine nodes (2I + a, 2J + b),
+// a, b in {-1, 0, 1}, with weights (1 - |a|/2)(1 - |b|/2)
+__device__ inline void mg_restrict_at(long n, int cnx, int cny, const float* __restrict__ res,
+                                      const uint8_t* __restrict__ cfix, float* __restrict__ rc) {
+#pragma clang fp contract(fast)
+    const int nx = 2 * cnx, ny = 2 * cny;
+    const int I = (int)(n / (cny + 1)), J = (int)(n - (long)I * (cny + 1));
+    float sx = 0.f, sy = 0.f;
+#pragma unroll
+    for (int a = -1; a <= 1; a++)
+#pragma unroll
+        for (int b = -1; b <= 1; b++) {
+            const int i = 2 * I + a, j = 2 * J + b;
+            if (i < 0 || i > nx || j < 0 || j > ny) continue;
+            const float w = (a ? 0.5f : 1.f) * (b ? 0.5f : 1.f);
+            const Vec2<float> v = load2(res, (long)(ny + 1) * i + j);
+            sx += w * v.x;
+            sy += w * v.y;
+        }
+    store2(rc, n, mg_masked(Vec2<float>{sx, sy}, cfix[n]));
+}
+
+__global__ void __launch_bounds__(256) k_mg_restrict(int cnx, int cny, const float* __restrict__ res,
+                                                     const uint8_t* __restrict__ cfix, float* __restrict__ rc,
+                                                     const int* __restrict__ done) {
+    if (done && *done) return;
+    const long cnn = (long)(cnx + 1) * (cny + 1);
+    for (long n = blockIdx.x * (long)blockDim.x + threadIdx.x; n < cnn; n += (long)gridDim.x * blockDim.x)
+        mg_restrict_at(n, cnx, cny, res, cfix, rc);
+}
+
+// Galerkin coarse element matrices: one thread per (coarse element, entry).
+// Children's matrices: level 0 (E, K0 and the fixity of the fine nodes,
+// whose rows and columns are zeroed) when Kf is null, else Kf (FP64).
+__global__ void __launch_bounds__(256) k_mg_coarsen(int cnx, int cny, const double* __restrict__ E,
+                                                    const double* __restrict__ K0, const uint8_t* __restrict__ fix,
+                                                    const double* __restrict__ Kf, double* __restrict__ Kc,
+                                                    float* __restrict__ Kc32) {
+    const long nce = (long)cnx * cny;
+    const int ny = 2 * cny;
+    const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (t >= nce * 64) return;
+    const long ce = t >> 6;
+    const int ent = (int)(t & 63), row = ent >> 3, col = ent & 7;
+    const int q = row >> 1, d = row & 1, q2 = col >> 1, d2 = col & 1;
+    const int CI = (int)(ce / cny), CJ = (int)(ce - (long)CI * cny);
+    // node k of an element at local offsets (0,0), (1,0), (1,1), (0,1)
+    const int lx[4] = {0, 1, 1, 0}, ly[4] = {0, 0, 1, 1};
+    double s = 0.0;
+    for (int a0 = 0; a0 < 2; a0++)
+        for (int a1 = 0; a1 < 2; a1++) {
+            const int fi = 2 * CI + a0, fj = 2 * CJ + a1;
+            const long fe = (long)ny * fi + fj;
+            // w[k]: weight of coarse node q (resp. q2) at child node k
+            double wq[4], wq2[4];
+            for (int k = 0; k < 4; k++) {
+                const double x = 0.5 * (a0 + lx[k]), y = 0.5 * (a1 + ly[k]);
+                const double w4[4] = {(1 - x) * (1 - y), x * (1 - y), x * y, (1 - x) * y};
+                wq[k] = w4[q];
+                wq2[k] = w4[q2];
+            }
+            uint8_t fk[4] = {0, 0, 0, 0};
+            if (!Kf) {
+                const long nb = (long)(ny + 1) * fi + fj;
+                fk[0] = fix[nb];
+                fk[1] = fix[nb + (ny + 1)];
+                fk[2] = fix[nb + (ny + 1) + 1];
+                fk[3] = fix[nb + 1];
+            }
+            for (int k = 0; k < 4; k++) {
+                if (wq[k] == 0.0 || ((fk[k] >> d) & 1)) continue;
+                for (int k2 = 0; k2 < 4; k2++) {
+                    if (wq2[k2] == 0.0 || ((fk[k2] >> d2) & 1)) continue;
+                    const int rr = 2 * k + d, cc = 2 * k2 + d2;
+                    const double kv = Kf ? Kf[fe * 64 + rr * 8 + cc] : E[fe] * K0[rr * 8 + cc];
+                    s += wq[k] * wq2[k2] * kv;
+                }
+            }
+        }
+    Kc[ce * 64 + ent] = s;
+    Kc32[ce * 64 + ent] = (float)s;
+}
+
+// Diagonal of a coarse level (level >= 1) and its fixity: a DOF whose
+// diagonal is not positive has no stiffness left and is fixed (D = 1).
+__global__ void __launch_bounds__(256) k_mg_diag_full(int nx, int ny, const double* __restrict__ Ke,
+                                                      float* __restrict__ D, uint8_t* __restrict__ fix) {
+    const long nn = (long)(nx + 1) * (ny + 1);
+    for (long n = blockIdx.x * (long)blockDim.x + threadIdx.x; n < nn; n += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(n / (ny + 1)), j = (int)(n - (long)i * (ny + 1));
+        double dx = 0.0, dy = 0.0;
+        auto add = [&](int ei, int ej, int loc) {
+            const long e = (long)ny * ei + ej;
+            dx += Ke[e * 64 + (2 * loc) * 8 + 2 * loc];
+            dy += Ke[e * 64 + (2 * loc + 1) * 8 + 2 * loc + 1];
+        };
+        if (i >= 1 && j >= 1) add(i - 1, j - 1, 2);
+        if (i >= 1 && j < ny) add(i - 1, j, 1);
+        if (i < nx && j >= 1) add(i, j - 1, 3);
+        if (i < nx && j < ny) add(i, j, 0);
+        uint8_t f = 0;
+        if (!(dx > 0.0)) f |= 1;
+        if (!(dy > 0.0)) f |= 2;
+        fix[n] = f;
+        D[2 * n] = (f & 1) ? 1.f : (float)dx;
+        D[2 * n + 1] = (f & 2) ? 1.f : (float)dy;
+    }
+}
+
+// level-0 diagonal in FP32 from E and K0 (fixed DOFs: 1)
+__global__ void __launch_bounds__(256) k_mg_diag0(int nx, int ny, const double* __restrict__ E,
+                                                  const double* __restrict__ K0, const uint8_t* __restrict__ fix,
+                                                  float* __restrict__ D) {
+    const long nn = (long)(nx + 1) * (ny + 1);
+    for (long n = blockIdx.x * (long)blockDim.x + threadIdx.x; n < nn; n += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(n / (ny + 1)), j = (int)(n - (long)i * (ny + 1));
+        double dx = 0.0, dy = 0.0;
+        auto add = [&](int ei, int ej, int loc) {
+            const double e = E[(long)ny * ei + ej];
+            dx += e * K0[(2 * loc) * 8 + 2 * loc];
+            dy += e * K0[(2 * loc + 1) * 8 + 2 * loc + 1];
+        };
+        if (i >= 1 && j >= 1) add(i - 1, j - 1, 2);
+        if (i >= 1 && j < ny) add(i - 1, j, 1);
+        if (i < nx && j >= 1) add(i, j - 1, 3);
+        if (i < nx && j < ny) add(i, j, 0);
+        const uint8_t f = fix[n];
+        D[2 * n] = (f & 1) ? 1.f : (float)dx;
+        D[2 * n + 1] = (f & 2) ? 1.f : (float)dy;
+    }
+}
+
+// Dense free-DOF block of the coarsest level: entry (a, b) for free DOFs
+// dof[a], dof[b] (nodal DOF indices), summed over the elements that hold both.
+__global__ void __launch_bounds__(256) k_mg_dense(int nx, int ny, const double* __restrict__ Ke,
+                                                  const int* __restrict__ dof, int nf, double* __restrict__ A) {
+    const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (t >= (long)nf * nf) return;
+    const int a = (int)(t / nf), b = (int)(t - (long)a * nf);
+    const int na = dof[a] >> 1, da = dof[a] & 1, nb = dof[b] >> 1, db = dof[b] & 1;
+    const int ia = na / (ny + 1), ja = na - ia * (ny + 1), ib = nb / (ny + 1), jb = nb - ib * (ny + 1);
+    double s = 0.0;
+    // elements around node a; node b must be one of their nodes
+    const int lx[4] = {0, 1, 1, 0}, ly[4] = {0, 0, 1, 1};
+    for (int ei = ia - 1; ei <= ia; ei++)
+        for (int ej = ja - 1; ej <= ja; ej++) {
+            if (ei < 0 || ei >= nx || ej < 0 || ej >= ny) continue;
+            int ka = -1, kb = -1;
+            for (int k = 0; k < 4; k++) {
+                if (ei + lx[k] == ia && ej + ly[k] == ja) ka = k;
+                if (ei + lx[k] == ib && ej + ly[k] == jb) kb = k;
+            }
+            if (ka < 0 || kb < 0) continue;
+            s += Ke[((long)ny * ei + ej) * 64 + (2 * ka + da) * 8 + 2 * kb + db];
+        }
+    A[t] = s;
+}
+
+// Largest free-DOF block inverted in LDS (90 x 90 doubles: 64.8 KB).
+constexpr int kMgLdsDense = 90;
+
+// In-place inverse of an SPD n x n matrix (row-major, FP64, n <= 90) by
+// Gauss-Jordan without pivoting, in LDS, one workgroup.
+__global__ void __launch_bounds__(256) k_mg_invert_lds(int n, double* __restrict__ A) {
+    __shared__ double M[kMgLdsDense * kMgLdsDense];
+    __shared__ double col[kMgLdsDense];
+    for (int t = threadIdx.x; t < n * n; t += blockDim.x) M[t] = A[t];
+    __syncthreads();
+    for (int k = 0; k < n; k++) {
+        const double piv = M[k * n + k];
+        for (int i = threadIdx.x; i < n; i += blockDim.x) col[i] = M[i * n + k];
+        __syncthreads();
+        for (int j = threadIdx.x; j < n; j += blockDim.x) M[k * n + j] = (j == k) ? 1.0 / piv : M[k * n + j] / piv;
+        __syncthreads();
+        for (int t = threadIdx.x; t < n * n; t += blockDim.x) {
+            const int i = t / n, j = t - i * n;
+            if (i == k) continue;
+            M[t] = (j == k) ? -col[i] / piv : M[t] - col[i] * M[k * n + j];
+        }
+        __syncthreads();
+    }
+    for (int t = threadIdx.x; t < n * n; t += blockDim.x) A[t] = M[t];
+}
+
+// In-place inverse of an SPD n x n matrix (row-major, FP64) by Gauss-Jordan
+// without pivoting, one workgroup in global memory (coarsest levels above
+// kMgLdsDense free DOFs, which only grids with odd element counts leave);
+// W is an n x n scratch.
+__global__ void __launch_bounds__(256) k_mg_invert(int n, double* __restrict__ A, double* __restrict__ W) {
+    // W = I
+    for (int t = threadIdx.x; t < n * n; t += blockDim.x) W[t] = (t / n == t % n) ? 1.0 : 0.0;
+    __syncthreads();
+    for (int k = 0; k < n; k++) {
+        const double piv = A[(long)k * n + k];
+        __syncthreads();
+        // scale row k
+        for (int c = threadIdx.x; c < n; c += blockDim.x) {
+            A[(long)k * n + c] /= piv;
+            W[(long)k * n + c] /= piv;
+        }
+        __syncthreads();
+        // eliminate column k from the other rows
+        for (int t = threadIdx.x; t < n * n; t += blockDim.x) {
+            const int r = t / n, c = t - r * n;
+            if (r == k) continue;
+            const double f = A[(long)r * n + k];
+            if (c != k) A[t] -= f * A[(long)k * n + c];
+            W[t] -= f * W[(long)k * n + c];
+        }
+        __syncthreads();
+        for (int r = threadIdx.x; r < n; r += blockDim.x)
+            if (r != k) A[(long)r * n + k] = 0.0;
+        __syncthreads();
+    }
+    for (int t = threadIdx.x; t < n * n; t += blockDim.x) A[t] = W[t];
+}
+
+// coarsest solve: x = Ainv r on the free DOFs (FP64 inverse, FP32
+// vectors), one thread per free DOF; the inverse is symmetric, so thread a
+// reads column a (coalesced across threads).  x's fixed entries are zeroed
+// beforehand.
+__global__ void __launch_bounds__(64) k_mg_coarse_solve(int nf, const double* __restrict__ Ainv,
+                                                        const int* __restrict__ dof, const float* __restrict__ r,
+                                                        float* __restrict__ x, const int* __restrict__ done) {
+    if (done && *done) return;
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= nf) return;
+    double s = 0.0;
+    for (int b = 0; b < nf; b++) s += Ainv[(long)b * nf + a] * (double)r[dof[b]];
+    x[dof[a]] = (float)s;
+}
+
+// level-0 element matrices in FP64 (E_e K0), for a grid that is its own
+// coarsest level
+__global__ void __launch_bounds__(256) k_mg_ke0_full(long ne, const double* __restrict__ E,
+                                                     const double* __restrict__ K0, double* __restrict__ Ke) {
+    const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (t < ne * 64) Ke[t] = E[t >> 6] * K0[t & 63];
+}
+
+// ------------------------------------------------------- the small levels ----
+// The V-cycle below a level of at most 1024 nodes in one workgroup of 1024
+// threads (one node per thread), the coarsest solve included: what would be
+// about 3 launches per level, each mostly launch latency, becomes one.
+// Phases are separated by barriers; the vectors stay in global memory (L2).
+constexpr int kMgTailMax = 6;    // levels handled by the tail kernel
+constexpr int kMgTailNodes = 1024;
+
+struct MgTailLevel {
+    int nx, ny;
+    const float* Ke;  // FP32 element matrices (a coarse level)
+    const float* D;
+    const uint8_t* fix;
+    float *r, *x, *x2, *res, *res2;
+};
+struct MgTail {
+    int nlev;  // levels; the last is the coarsest
+    MgTailLevel L[kMgTailMax];
+    int nf;
+    const double* Ainv;
+    const int* dof;
+    int nu;
+    float omega;
+};
+
+// Buffers: a level's pre-smoothing leaves x in L[l].x when nu is odd, else
+// in L[l].x2; the post-smoothing alternates nu more times from there.
+__host__ __device__ inline bool mg_pre_in_x2(int nu) { return ((nu - 1) & 1) != 0; }
+__host__ __device__ inline int mg_tail_final_is_x2(int nu) { return (mg_pre_in_x2(nu) ? 1 : 0) ^ (nu & 1); }
+
+__global__ void __launch_bounds__(1024) k_mg_tail(MgTail T, const int* __restrict__ done) {
+    if (done && *done) return;
+    __shared__ float rf[kMgLdsDense];
+    const int tid = threadIdx.x;
+    const MgTailLevel* L = T.L;
+    const bool pre_x2 = mg_pre_in_x2(T.nu);
+    // down: pre-smoothing and restriction
+#pragma unroll 1
+    for (int l = 0; l + 1 < T.nlev; l++) {
+        const MgTailLevel& a = L[l];
+        const MgKeFull ke{a.Ke};
+        const long nn = (long)(a.nx + 1) * (a.ny + 1);
+        float *x = a.x, *res = a.res, *xo = a.x2, *ro = a.res2;
+        if (tid < nn) mg_sweep_at(tid, a.nx, a.ny, ke, a.D, a.fix, T.omega, a.r, nullptr, nullptr, nullptr, x, res);
+        __syncthreads();
+#pragma unroll 1
+        for (int k = 1; k < T.nu; k++) {
+            if (tid < nn) mg_sweep_at(tid, a.nx, a.ny, ke, a.D, a.fix, T.omega, a.r, x, res, nullptr, xo, ro);
+            __syncthreads();
+            float* t0 = x; x = xo; xo = t0;
+            float* t1 = res; res = ro; ro = t1;
+        }
+        const MgTailLevel& b = L[l + 1];
+        const long cnn = (long)(b.nx + 1) * (b.ny + 1);
+        if (tid < cnn) mg_restrict_at(tid, b.nx, b.ny, res, b.fix, b.r);
+        __syncthreads();
+    }
+    // coarsest: x = Ainv r on the free DOFs
+    {
+        const MgTailLevel& c = L[T.nlev - 1];
+        const long nv = 2L * (c.nx + 1) * (c.ny + 1);
+        for (long q = tid; q < nv; q += blockDim.x) c.x[q] = 0.f;
+        if (tid < T.nf) rf[tid] = c.r[T.dof[tid]];
+        __syncthreads();
+        if (tid < T.nf) {
+            double sum = 0.0;
+#pragma unroll 8
+            for (int b = 0; b < T.nf; b++) sum += T.Ainv[(long)b * T.nf + tid] * (double)rf[b];
+            c.x[T.dof[tid]] = (float)sum;
+        }
+        __syncthreads();
+    }
+    // up: prolongation and post-smoothing
+    const float* xc = L[T.nlev - 1].x;
+#pragma unroll 1
+    for (int l = T.nlev - 2; l >= 0; l--) {
+        const MgTailLevel& a = L[l];
+        const MgKeFull ke{a.Ke};
+        const long nn = (long)(a.nx + 1) * (a.ny + 1);
+        float *x = pre_x2 ? a.x2 : a.x, *xo = pre_x2 ? a.x : a.x2;
+        if (tid < nn) mg_sweep_at(tid, a.nx, a.ny, ke, a.D, a.fix, T.omega, a.r, x, nullptr, xc, xo, nullptr);
+        __syncthreads();
+        { float* t0 = x; x = xo; xo = t0; }
+#pragma unroll 1
+        for (int k = 1; k < T.nu; k++) {
+            if (tid < nn) mg_jacobi_at(tid, a.nx, a.ny, ke, a.D, a.fix, T.omega, a.r, x, xo);
+            __syncthreads();
+            float* t0 = x; x = xo; xo = t0;
+        }
+        xc = x;
+    }
+}
+
+// ------------------------------------------- multigrid-preconditioned PCG ----
+// The PCG of the k_pcg_k1_q4_tile / k_pcg_k2 pair with z = V(r) in place of
+// z = r ./ D: the update kernel hands r to the V-cycle in FP32, and
+// k_mg_pcg_z brings z back in FP64 with the sums (z.r, r.r) that the next
+// k_pcg_k1_q4_tile reads (beta and the convergence test).
+
+// alpha = zr / p.q from K1's partials; x += alpha p; r -= alpha q; r32 = r
+__global__ void __launch_bounds__(256) k_mg_pcg_update(long n, int k, double* __restrict__ x, double* __restrict__ r,
+                                                       const double* __restrict__ p, const double* __restrict__ q,
+                                                       float* __restrict__ r32, const double* __restrict__ partA,
+                                                       int nbA, PcgDev* __restrict__ S) {
+#pragma clang fp contract(fast)
+    __shared__ double lds[32];
+    PartRegs<1, 8> pr;
+    load_partials(partA, nbA, pr);
+    if (S->done) return;
+    double v[1];
+    sum_partials(pr, partA, nbA, lds, v);
+    const double alpha = S->zr[k & 1] / v[0];  // CG.h:433
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        S->pq = v[0];
+        S->alpha = alpha;
+    }
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const double ri = r[i] - alpha * q[i];
+        x[i] = x[i] + alpha * p[i];
+        r[i] = ri;
+        r32[i] = (float)ri;
+    }
+}
+
+// z = V(r) in FP64 (the V-cycle's FP32 output); partials z.r and r.r
+__global__ void __launch_bounds__(256) k_mg_pcg_z(long n, const float* __restrict__ z32, const double* __restrict__ r,
+                                                  double* __restrict__ z, double* __restrict__ partB,
+                                                  const int* __restrict__ done) {
+    __shared__ double lds[32];
+    if (done && *done) return;
+    double acc[2] = {0.0, 0.0};
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const double zi = (double)z32[i], ri = r[i];
+        z[i] = zi;
+        acc[0] += zi * ri;
+        acc[1] += ri * ri;
+    }
+    store_partials<2>(acc, lds, partB);
+}
+
+}  // namespace pf

@@ -1,0 +1,141 @@
+"""GPU checks of the multigrid base preconditioner (pf_mg_*; no counterpart
+in the reference) against its CPU restatement (oracle/mg.py), and of the
+multigrid-preconditioned flexible CG (PF_OP_MG) against the reference's own
+solutions: displacement within 1e-8 at eps = 1e-10, and the 50-iteration
+SIMP loop within 1e-9 (compliance) and 1e-6 (densities)."""
+import numpy as np
+import pytest
+
+from tests import golden_io as gi
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def pa():
+    import pansfem2_amd
+    return pansfem2_amd
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))
+
+
+def oracle_levels(ctx, nx, ny, E):
+    from oracle import mg
+    fix = np.zeros((nx + 1) * (ny + 1), np.uint8)
+    fix[: ny + 1] = 3
+    K0 = ctx.element_stiffness(0, 1.0, 0.3, 1.0)
+    return mg.build(nx, ny, E, K0, fix), fix
+
+
+@pytest.mark.parametrize("nx,ny", [(16, 8), (120, 40), (64, 64)])
+def test_hierarchy_against_oracle(pa, nx, ny):
+    print(f"multigrid hierarchy {nx}x{ny}", flush=True)
+    rng = np.random.default_rng(nx * 1000 + ny)
+    rho = rng.uniform(1e-3, 1.0, nx * ny)
+    E = 1e-9 + (1 - 1e-9) * rho ** 3
+    ctx = pa.cantilever2d(nx, ny)
+    ctx.set_moduli(E)
+    ctx.mg_setup(1, 0.6)
+    ctx.mg_build()
+    lv, fix = oracle_levels(ctx, nx, ny, E)
+    info = ctx.mg_level(0)
+    assert info["levels"] == len(lv)
+    for k in range(len(lv)):
+        d = ctx.mg_level(k)
+        assert (d["nx"], d["ny"]) == (lv[k]["nx"], lv[k]["ny"])
+        free = ~((np.stack([d["fix"] & 1, d["fix"] & 2], 1).reshape(-1)) > 0)
+        assert np.array_equal(free, lv[k]["free"])
+        assert np.max(np.abs(d["D"] - lv[k]["D"]) / lv[k]["D"]) < 1e-6
+        if k > 0:
+            want = lv[k]["Ke"]
+            assert np.max(np.abs(d["Ke"] - want)) <= 1e-13 * np.max(np.abs(want))
+    ctx.close()
+
+
+@pytest.mark.parametrize("nx,ny", [(16, 8), (120, 40)])
+def test_vcycle_against_oracle(pa, nx, ny):
+    from oracle import mg
+    print(f"V-cycle {nx}x{ny}", flush=True)
+    rng = np.random.default_rng(7)
+    rho = rng.uniform(0.05, 1.0, nx * ny)
+    E = 1e-9 + (1 - 1e-9) * rho ** 3
+    ctx = pa.cantilever2d(nx, ny)
+    ctx.set_moduli(E)
+    ctx.mg_setup(1, 0.6)
+    lv, fix = oracle_levels(ctx, nx, ny, E)
+    n2g = ctx.dofmap()  # nodal DOF -> free index, -1 for fixed
+    r_free = rng.standard_normal(ctx.N)
+    z_free = ctx.mg_apply(r_free)
+    r_nodal = np.zeros(n2g.size)
+    r_nodal[n2g >= 0] = r_free[n2g[n2g >= 0]]
+    z_nodal = mg.vcycle(lv, r_nodal, 1, 0.6)
+    want = z_nodal[n2g >= 0][np.argsort(n2g[n2g >= 0])]
+    # FP32 V-cycle against the FP64 restatement
+    assert rel(z_free, want) < 1e-4
+    ctx.close()
+
+
+def test_solve_c1_against_reference(pa):
+    g = gi.load("simp_120x40_oc")
+    print("multigrid FCG 120x40", flush=True)
+    ctx = pa.cantilever2d(120, 40)
+    ctx.set_density(g["rho0"])
+    ctx.mg_setup(1, 0.6)
+    u, st = ctx.pcg(g["F"], op=pa.PF_OP_MG, eps=1e-10)
+    assert st["status"] == 1 and st["fallbacks"] == 0
+    assert st["iterations"] <= 40
+    assert rel(u, g["u0_free"]) < 1e-8
+    assert st["true_residual"] <= 1e-10
+    ctx.close()
+
+
+def test_simp_loop_c1_against_reference(pa):
+    """50 OC iterations at 120x40 with every solve multigrid-preconditioned."""
+    from oracle import pyoracle as po
+    g = gi.load("simp_120x40_oc")
+    print("multigrid SIMP loop 120x40", flush=True)
+    ctx = pa.cantilever2d(120, 40)
+    ctx.mg_setup(1, 0.6)
+    r = ctx.simp_run(50, pa.simp_default_params(solver=pa.PF_OP_MG))
+    ref = po.simp2d(120, 40, 50)
+    assert np.max(np.abs(r["f"] - ref["f"]) / ref["f"]) < 1e-9
+    assert np.max(np.abs(r["rho"] - g["rho_final"])) < 1e-6
+    assert np.max(np.abs(r["s"] - g["s_final"])) < 1e-6
+    assert np.max(r["cg_iters"]) <= 60
+    ctx.close()
+
+
+def test_large_against_jacobi(pa):
+    nx, ny = 1024, 512
+    print("multigrid FCG vs Jacobi-PCG 1024x512", flush=True)
+    ctx = pa.cantilever2d(nx, ny)
+    rng = np.random.default_rng(20261207)
+    ctx.set_density(rng.uniform(1e-3, 1.0, nx * ny))
+    b = ctx.load()
+    ctx.set_pcg_form(pa.PF_PCG_SINGLE_REDUCTION)
+    u0, s0 = ctx.pcg(b, op=pa.PF_OP_EBE64, eps=1e-8)
+    ctx.mg_setup(1, 0.6)
+    u1, s1 = ctx.pcg(b, op=pa.PF_OP_MG, eps=1e-8)
+    print(f"  jacobi {s0['iterations']} it {s0['seconds']:.3f} s; multigrid {s1['iterations']} it "
+          f"{s1['seconds']:.3f} s; |u1-u0|/|u0| {rel(u1, u0):.2e}", flush=True)
+    assert s1["status"] == 1 and s1["fallbacks"] == 0
+    assert s1["iterations"] < 300
+    assert rel(u1, u0) < 1e-6
+    assert abs(u1 @ b - u0 @ b) / abs(u0 @ b) < 1e-9
+    ctx.close()
+
+
+def test_setup_arguments(pa):
+    ctx = pa.cantilever2d(16, 8)
+    for nu, om in ((-1, 0.6), (9, 0.6), (1, 0.0), (1, 1.5)):
+        with pytest.raises(pa.PfError):
+            ctx.mg_setup(nu, om)
+    with pytest.raises(pa.PfError):
+        ctx.pcg(ctx.load(), op=pa.PF_OP_MG)  # not set up
+    ctx.mg_setup(1, 0.6)
+    ctx.mg_setup(0, 0.0)  # off again
+    with pytest.raises(pa.PfError):
+        ctx.pcg(ctx.load(), op=pa.PF_OP_MG)
+    ctx.close()
