@@ -105,11 +105,14 @@ def learned_vs_jacobi(pa, nx, ny, train_iters=30, collect_every=20, collect_max=
     error) pairs that the plain FP64 Jacobi-PCG produces during the first
     `train_iters` OC iterations (every `collect_every` chunks of 32 CG
     iterations, at most `collect_max` per solve), plus 120 synthetic (K v, v)
-    pairs; then both solvers take the system of OC iteration train_iters + 1
-    to a 1e-8 FP64 true relative residual."""
+    pairs, as a correction over the multigrid V-cycle (FP32); then three
+    solvers take the system of a later OC iteration to a 1e-8 FP64 true
+    relative residual: plain FP64 Jacobi-PCG, flexible CG with the V-cycle
+    alone, and flexible CG with the network over the V-cycle."""
     ctx = pa.cantilever2d(nx, ny)
     ctx.set_pcg_form(pa.PF_PCG_SINGLE_REDUCTION)  # the Jacobi leg at its fastest
     ctx.precond_init(seed=11)
+    ctx.mg_setup(1, 0.6)
     t0 = time.perf_counter()
     r = ctx.simp_run(train_iters, pa.simp_default_params(collect=collect_every, collect_max=collect_max))
     t_collect = time.perf_counter() - t0
@@ -129,14 +132,19 @@ def learned_vs_jacobi(pa, nx, ny, train_iters=30, collect_every=20, collect_max=
            "train_steps": steps, "loss_first": loss0, "loss_last": loss1, "train_seconds": t_train,
            "collect_oc_seconds": t_collect}
     comp = {}
-    for label, op in (("jacobi_fp64", pa.PF_OP_EBE64), ("learned_fcg", pa.PF_OP_LEARNED)):
+    for label, op in (("jacobi_fp64", pa.PF_OP_EBE64), ("multigrid_fcg", pa.PF_OP_MG),
+                      ("learned_fcg", pa.PF_OP_LEARNED)):
+        x, st = ctx.pcg(b, op=op, eps=1e-8)  # untimed first call: allocations
         x, st = ctx.pcg(b, op=op, eps=1e-8)
         comp[label] = float(x @ b)  # u^T K u = u^T f
         out[label] = {"seconds": st["seconds"], "iterations": st["iterations"], "true_residual": st["true_residual"],
                       "status": st["status"], "fallbacks": st["fallbacks"]}
     out["speedup_learned_vs_jacobi"] = out["jacobi_fp64"]["seconds"] / out["learned_fcg"]["seconds"]
+    out["speedup_multigrid_vs_jacobi"] = out["jacobi_fp64"]["seconds"] / out["multigrid_fcg"]["seconds"]
     out["iteration_ratio"] = out["learned_fcg"]["iterations"] / out["jacobi_fp64"]["iterations"]
+    out["network_gain_iterations"] = out["multigrid_fcg"]["iterations"] / out["learned_fcg"]["iterations"]
     out["compliance_rel_diff"] = abs(comp["learned_fcg"] - comp["jacobi_fp64"]) / abs(comp["jacobi_fp64"])
+    out["compliance_rel_diff_multigrid"] = abs(comp["multigrid_fcg"] - comp["jacobi_fp64"]) / abs(comp["jacobi_fp64"])
     ctx.close()
     return out
 
@@ -189,6 +197,15 @@ def main():
     torch.cuda.synchronize()
     wall0 = time.perf_counter() - t0
     ctx.set_pcg_form(pa.PF_PCG_SINGLE_REDUCTION)
+    # and with multigrid-preconditioned CG to the same 1e-10 (true residual)
+    ctx.mg_setup(1, 0.6)
+    ctx.simp_run(1, pa.simp_default_params(solver=pa.PF_OP_MG), s0=w["s"])  # allocations
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rm = ctx.simp_run(args.steps, pa.simp_default_params(solver=pa.PF_OP_MG), s0=w["s"])
+    torch.cuda.synchronize()
+    wallm = time.perf_counter() - t0
+    ctx.mg_setup(0, 0.0)
 
     # dominant kernel: live HIP-event timing of the PCG kernels on the final design
     ctx.set_density(r["rho"])
@@ -237,7 +254,11 @@ def main():
                    "mean_cg_iters": n_cg, "mean_bisection_steps": n_bis,
                    "scalingcg_order": {"value": args.steps / wall0, "ms_per_step": 1000.0 * wall0 / args.steps,
                                        "mean_cg_iters": float(np.mean(r0["cg_iters"])),
-                                       "compliance_rel_diff": float(np.max(np.abs(r0["f"] - r["f"]) / r["f"]))}},
+                                       "compliance_rel_diff": float(np.max(np.abs(r0["f"] - r["f"]) / r["f"]))},
+                   "multigrid_pcg": {"value": args.steps / wallm, "ms_per_step": 1000.0 * wallm / args.steps,
+                                     "mean_cg_iters": float(np.mean(rm["cg_iters"])),
+                                     "compliance_rel_diff": float(np.max(np.abs(rm["f"] - r["f"]) / r["f"])),
+                                     "density_max_abs_diff": float(np.max(np.abs(rm["rho"] - r["rho"])))}},
         "roofline": {"bound": "hbm", "kernel": name, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "avg_ms": ms, "algorithmic_bytes": byts,
                      "scalingcg_order": {"k1_ms": prof["k1_ms"], "k2_ms": prof["k2_ms"], "iteration_ms": iter_ms,
