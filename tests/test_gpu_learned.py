@@ -1,8 +1,8 @@
 """GPU tests of the learned preconditioner (no counterpart in the reference):
 the MFMA network kernels against the CPU restatement (oracle/convnet.py,
 itself pinned by finite differences), training on pairs produced by the plain
-solver, what the trained network gains over Jacobi-PCG, and the Jacobi
-fallback when the network is useless."""
+solver, what the trained network gains over Jacobi-PCG and over the
+multigrid V-cycle, and the Jacobi fallback when the network is useless."""
 import numpy as np
 import pytest
 
@@ -79,6 +79,45 @@ def test_training_and_gain_over_jacobi(pa):
     # the residual is left in, so both are held to the same bound
     xt, _ = ctx.pcg(b, op=pa.PF_OP_EBE64, eps=1e-12)
     assert relerr(xj, xt) < 2e-5 and relerr(xl, xt) < 2e-5, (relerr(xl, xt), relerr(xj, xt))
+
+
+def test_learned_over_multigrid(pa):
+    """The network trained as a correction over the multigrid V-cycle (the
+    bench's learned leg): the learned mixed-precision solve reaches the FP64
+    1e-8 true residual, its compliance matches plain FP64 Jacobi-PCG's to 1e-6
+    (the target's bar), it needs far fewer iterations than Jacobi-PCG, and
+    against the V-cycle alone (the method without the network) it is at most
+    a quarter worse in iterations: at this size and at 1024 x 512 the network
+    adds no gain over the V-cycle (measured 0.9x), so the test holds it to
+    not losing much."""
+    nx, ny = 96, 48
+    ctx = pa.cantilever2d(nx, ny)
+    ctx.precond_init(seed=11)
+    ctx.mg_setup(1, 0.6)
+    r = ctx.simp_run(20, pa.simp_default_params(collect=2, collect_max=16))
+    s = r["s"]
+    for k in range(3):
+        rr = ctx.simp_run(1, pa.simp_default_params(), s0=s)
+        s = rr["s"]
+        ctx.set_density(rr["rho"])
+        ctx.precond_collect_random(40, seed=100 + k)
+    # the reported losses are single pairs' (the first and the last step's);
+    # over the V-cycle they start near 0.2 and the pair-to-pair spread hides
+    # the trend, so only the solves below are asserted
+    ctx.precond_train(300, lr=2e-3, seed=17)
+    rr = ctx.simp_run(1, pa.simp_default_params(), s0=s)
+    ctx.set_density(rr["rho"])
+    b = ctx.load()
+    xj, stj = ctx.pcg(b, op=pa.PF_OP_EBE64, eps=1e-8)
+    xm, stm = ctx.pcg(b, op=pa.PF_OP_MG, eps=1e-8)
+    xl, stl = ctx.pcg(b, op=pa.PF_OP_LEARNED, eps=1e-8)
+    print(f"  jacobi {stj['iterations']}, multigrid {stm['iterations']}, learned over multigrid "
+          f"{stl['iterations']} iterations", flush=True)
+    assert stl["status"] == 1 and stl["true_residual"] <= 1.0000001e-8 and stl["fallbacks"] == 0
+    assert stm["status"] == 1 and stm["true_residual"] <= 1.0000001e-8
+    assert abs(xl @ b - xj @ b) / abs(xj @ b) < 1e-6
+    assert stl["iterations"] < 0.1 * stj["iterations"]
+    assert stl["iterations"] <= 1.25 * stm["iterations"] + 2
 
 
 def test_fallback_when_network_is_useless(pa):
