@@ -913,7 +913,7 @@ int mg_build(pf_ctx* ctx) {
     k_mg_diag0<<<gsize(l0.nn), kBlock, 0, ctx->stream>>>(l0.nx, l0.ny, ctx->d_E, ctx->d_K0, ctx->d_fix, l0.D);
     for (int l = 1; l <= c; l++) {
         MgLevel& a = S->L[l];
-        const long t = a.ne * 64;
+        const long t = a.ne * 8;
         k_mg_coarsen<<<(unsigned)((t + kBlock - 1) / kBlock), kBlock, 0, ctx->stream>>>(
             a.nx, a.ny, ctx->d_E, ctx->d_K0, ctx->d_fix, l == 1 ? nullptr : S->L[l - 1].Ke64, a.Ke64, a.Ke32);
         k_mg_diag_full<<<gsize(a.nn), kBlock, 0, ctx->stream>>>(a.nx, a.ny, a.Ke64, a.D, a.fix);
@@ -938,7 +938,7 @@ int mg_build(pf_ctx* ctx) {
     k_mg_dense<<<(unsigned)((nf2 + kBlock - 1) / kBlock), kBlock, 0, ctx->stream>>>(C.nx, C.ny, C.Ke64, S->dof,
                                                                                    S->nf, S->A);
     if (S->nf <= kMgLdsDense)
-        k_mg_invert_lds<<<1, kBlock, 0, ctx->stream>>>(S->nf, S->A);
+        k_mg_invert_lds<<<1, 1024, 0, ctx->stream>>>(S->nf, S->A);
     else
         k_mg_invert<<<1, kBlock, 0, ctx->stream>>>(S->nf, S->A, S->W);
     PF_TRY(check_launch(ctx));

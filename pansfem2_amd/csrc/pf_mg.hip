@@ -312,7 +312,8 @@ __global__ void __launch_bounds__(256) k_mg_restrict(int cnx, int cny, const flo
         mg_restrict_at(n, cnx, cny, res, cfix, rc);
 }
 
-// Galerkin coarse element matrices: one thread per (coarse element, entry).
+// Galerkin coarse element matrices: one thread per (coarse element, row);
+// row 2q + d of Kc is  sum_a sum_k w_a[k][q] (row 2k + d of K_a) P_a.
 // Children's matrices: level 0 (E, K0 and the fixity of the fine nodes,
 // whose rows and columns are zeroed) when Kf is null, else Kf (FP64).
 __global__ void __launch_bounds__(256) k_mg_coarsen(int cnx, int cny, const double* __restrict__ E,
@@ -322,25 +323,25 @@ __global__ void __launch_bounds__(256) k_mg_coarsen(int cnx, int cny, const doub
     const long nce = (long)cnx * cny;
     const int ny = 2 * cny;
     const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    if (t >= nce * 64) return;
-    const long ce = t >> 6;
-    const int ent = (int)(t & 63), row = ent >> 3, col = ent & 7;
-    const int q = row >> 1, d = row & 1, q2 = col >> 1, d2 = col & 1;
+    if (t >= nce * 8) return;
+    const long ce = t >> 3;
+    const int row = (int)(t & 7), q = row >> 1, d = row & 1;
     const int CI = (int)(ce / cny), CJ = (int)(ce - (long)CI * cny);
     // node k of an element at local offsets (0,0), (1,0), (1,1), (0,1)
     const int lx[4] = {0, 1, 1, 0}, ly[4] = {0, 0, 1, 1};
-    double s = 0.0;
+    double out[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (int a0 = 0; a0 < 2; a0++)
         for (int a1 = 0; a1 < 2; a1++) {
             const int fi = 2 * CI + a0, fj = 2 * CJ + a1;
             const long fe = (long)ny * fi + fj;
-            // w[k]: weight of coarse node q (resp. q2) at child node k
-            double wq[4], wq2[4];
+            // w[k][q']: weight of coarse node q' at child node k
+            double w[4][4];
             for (int k = 0; k < 4; k++) {
                 const double x = 0.5 * (a0 + lx[k]), y = 0.5 * (a1 + ly[k]);
-                const double w4[4] = {(1 - x) * (1 - y), x * (1 - y), x * y, (1 - x) * y};
-                wq[k] = w4[q];
-                wq2[k] = w4[q2];
+                w[k][0] = (1 - x) * (1 - y);
+                w[k][1] = x * (1 - y);
+                w[k][2] = x * y;
+                w[k][3] = (1 - x) * y;
             }
             uint8_t fk[4] = {0, 0, 0, 0};
             if (!Kf) {
@@ -350,18 +351,25 @@ __global__ void __launch_bounds__(256) k_mg_coarsen(int cnx, int cny, const doub
                 fk[2] = fix[nb + (ny + 1) + 1];
                 fk[3] = fix[nb + 1];
             }
+            const double e = Kf ? 0.0 : E[fe];
             for (int k = 0; k < 4; k++) {
-                if (wq[k] == 0.0 || ((fk[k] >> d) & 1)) continue;
-                for (int k2 = 0; k2 < 4; k2++) {
-                    if (wq2[k2] == 0.0 || ((fk[k2] >> d2) & 1)) continue;
-                    const int rr = 2 * k + d, cc = 2 * k2 + d2;
-                    const double kv = Kf ? Kf[fe * 64 + rr * 8 + cc] : E[fe] * K0[rr * 8 + cc];
-                    s += wq[k] * wq2[k2] * kv;
-                }
+                if (w[k][q] == 0.0 || ((fk[k] >> d) & 1)) continue;
+                const int rr = 2 * k + d;
+                // (row rr of K_a) P_a, entries (q2, d2)
+                for (int k2 = 0; k2 < 4; k2++)
+                    for (int d2 = 0; d2 < 2; d2++) {
+                        if ((fk[k2] >> d2) & 1) continue;
+                        const int cc = 2 * k2 + d2;
+                        const double kv = Kf ? Kf[fe * 64 + rr * 8 + cc] : e * K0[rr * 8 + cc];
+                        const double wk = w[k][q] * kv;
+                        for (int q2 = 0; q2 < 4; q2++) out[2 * q2 + d2] += wk * w[k2][q2];
+                    }
             }
         }
-    Kc[ce * 64 + ent] = s;
-    Kc32[ce * 64 + ent] = (float)s;
+    for (int c = 0; c < 8; c++) {
+        Kc[ce * 64 + row * 8 + c] = out[c];
+        Kc32[ce * 64 + row * 8 + c] = (float)out[c];
+    }
 }
 
 // Diagonal of a coarse level (level >= 1) and its fixity: a DOF whose
@@ -443,23 +451,34 @@ __global__ void __launch_bounds__(256) k_mg_dense(int nx, int ny, const double* 
 constexpr int kMgLdsDense = 90;
 
 // In-place inverse of an SPD n x n matrix (row-major, FP64, n <= 90) by
-// Gauss-Jordan without pivoting, in LDS, one workgroup.
-__global__ void __launch_bounds__(256) k_mg_invert_lds(int n, double* __restrict__ A) {
+// Gauss-Jordan without pivoting, in LDS, one workgroup of 1024 threads
+// (thread (i, j) of a 32 x 32 layout updates entries i + 32 a, j + 32 b).
+__global__ void __launch_bounds__(1024) k_mg_invert_lds(int n, double* __restrict__ A) {
     __shared__ double M[kMgLdsDense * kMgLdsDense];
     __shared__ double col[kMgLdsDense];
+    __shared__ double row[kMgLdsDense];
+    const int ti = threadIdx.x >> 5, tj = threadIdx.x & 31;
     for (int t = threadIdx.x; t < n * n; t += blockDim.x) M[t] = A[t];
     __syncthreads();
     for (int k = 0; k < n; k++) {
-        const double piv = M[k * n + k];
-        for (int i = threadIdx.x; i < n; i += blockDim.x) col[i] = M[i * n + k];
-        __syncthreads();
-        for (int j = threadIdx.x; j < n; j += blockDim.x) M[k * n + j] = (j == k) ? 1.0 / piv : M[k * n + j] / piv;
-        __syncthreads();
-        for (int t = threadIdx.x; t < n * n; t += blockDim.x) {
-            const int i = t / n, j = t - i * n;
-            if (i == k) continue;
-            M[t] = (j == k) ? -col[i] / piv : M[t] - col[i] * M[k * n + j];
+        const double inv = 1.0 / M[k * n + k];
+        // column k (old values) and row k (scaled) of this step
+        for (int i = threadIdx.x; i < n; i += blockDim.x) {
+            col[i] = M[i * n + k];
+            row[i] = (i == k) ? inv : M[k * n + i] * inv;
         }
+        __syncthreads();
+        for (int i = ti; i < n; i += 32)
+            for (int j = tj; j < n; j += 32) {
+                double v;
+                if (i == k)
+                    v = row[j];
+                else if (j == k)
+                    v = -col[i] * inv;
+                else
+                    v = M[i * n + j] - col[i] * row[j];
+                M[i * n + j] = v;
+            }
         __syncthreads();
     }
     for (int t = threadIdx.x; t < n * n; t += blockDim.x) A[t] = M[t];
