@@ -159,6 +159,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-preconditioner comparison")
+    ap.add_argument("--save-design", metavar="NPY",
+                    help="run --design-iters OC iterations of the Jacobi-PCG loop, save the densities, exit")
+    ap.add_argument("--design-iters", type=int, default=34)
+    ap.add_argument("--mg-solve", metavar="NPY",
+                    help="multigrid-preconditioned solves to 1e-8 on the densities saved by --save-design "
+                         "(the program the multigrid profile traces)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only the PCG/filter/OC kernels (pf_profile_kernels) on a seeded density field, then a "
                          "plain device copy of known size: the program the counter runs profile")
@@ -176,6 +182,22 @@ def main():
     N, ne = ctx.N, ctx.ne
     if args.kernels_only:
         kernels_only(ctx, pa, torch, nx, ny)
+        return
+    if args.save_design:
+        ctx.set_pcg_form(pa.PF_PCG_SINGLE_REDUCTION)
+        r = ctx.simp_run(args.design_iters)
+        np.save(args.save_design, r["rho"])
+        print(json.dumps({"saved_design": args.save_design, "oc_iterations": args.design_iters}))
+        return
+    if args.mg_solve:
+        ctx.set_density(np.load(args.mg_solve))
+        ctx.mg_setup(1, 0.6)
+        b = ctx.load()
+        out = []
+        for _ in range(2):
+            x, st = ctx.pcg(b, op=pa.PF_OP_MG, eps=1e-8)
+            out.append({"seconds": st["seconds"], "iterations": st["iterations"], "true_residual": st["true_residual"]})
+        print(json.dumps({"multigrid_solves": out}))
         return
     # warmup OC iterations (untimed), then K timed iterations continuing the same trajectory
     ctx.set_pcg_form(pa.PF_PCG_SINGLE_REDUCTION)
