@@ -566,6 +566,9 @@ struct MgState {
     double* W = nullptr;    // inversion scratch
     std::vector<void*> owned;
 };
+// dot products one batched reduction holds (multidot_batch)
+constexpr long kMpartDots = 128;
+
 struct LearnedNet {
     int H[4] = {0}, Wd[4] = {0};
     std::vector<NetLayer> L;  // c0a c0b c1a c1b c2a c2b c3a c3b d2 d1 d0 out
@@ -688,7 +691,7 @@ int net_build(pf_ctx* ctx, uint64_t seed) {
     N->wpart_n = ((p0 + ppb - 1) / ppb) * maxw;
     PF_TRY(net_alloc(ctx, N, &N->wpart, N->wpart_n));
     const long nv = ctx->nv;
-    PF_TRY(net_alloc(ctx, N, &N->mpart, 256L * 32));
+    PF_TRY(net_alloc(ctx, N, &N->mpart, 256L * kMpartDots));
     double** sc[] = {&N->w, &N->Kw, &N->gd, &N->rr, &N->ee, &N->dd};
     for (double** q : sc) PF_TRY(net_alloc(ctx, N, q, nv));
     return sync(ctx);
@@ -1138,12 +1141,11 @@ int optimal_pair(pf_ctx* ctx, const double* r, double* a, double* c);
 // (K e = r makes this computable without e).  d is never worse, in that
 // norm, than the best-scaled Jacobi step.  N->w holds y, N->Kw K y, N->dd t
 // and N->gd K t on return.
-int net_apply(pf_ctx* ctx, const double* rho, const double* r, double* d, double* rs_out, double rnorm = -1.0) {
+// The pieces of M(r) (stream-ordered, no host synchronisation): N->w = y,
+// N->dd = t, N->gd = K t, N->Kw = K y.
+int net_parts(pf_ctx* ctx, const double* rho, const double* r, double rn) {
     LearnedNet* N = ctx->net;
     const long nv = ctx->nv;
-    int rc = PF_OK;
-    const double rn = rnorm >= 0.0 ? rnorm : norm_nodal(ctx, r, &rc);
-    PF_TRY(rc);
     const double rs = rn > 0 ? rn / std::sqrt((double)std::max(1, ctx->N)) : 1.0;
     k_net_input<<<gsize(ctx->nn), kBlock, 0, ctx->stream>>>(ctx->nx, ctx->ny, rho, r, rs, N->x0);
     PF_TRY(net_forward(ctx));
@@ -1154,6 +1156,30 @@ int net_apply(pf_ctx* ctx, const double* rho, const double* r, double* d, double
         k_div_d<<<gsize(nv), kBlock, 0, ctx->stream>>>(nv, r, ctx->d_D, N->dd);   // t = r ./ D
     PF_TRY(apply_nodal(ctx, false, N->dd, N->gd, true));                     // K t
     PF_TRY(apply_nodal(ctx, false, N->w, N->Kw, true));                      // K y
+    return check_launch(ctx);
+}
+
+// (a, c) minimising ||a t + c y - e||_K from the five sums t.Kt, t.Ky, t.r,
+// y.Ky, y.r; the best-scaled t alone when the 2x2 Gram matrix is singular.
+void pair_from_sums(double tkt, double tky, double tr, double yky, double yr, double* a, double* c) {
+    const double det = tkt * yky - tky * tky;
+    if (std::isfinite(det) && det > 1e-12 * tkt * yky && yky > 0.0) {
+        *a = (yky * tr - tky * yr) / det;
+        *c = (tkt * yr - tky * tr) / det;
+    } else {
+        *a = tkt > 0.0 ? tr / tkt : 1.0;
+        *c = 0.0;
+    }
+}
+
+int net_apply(pf_ctx* ctx, const double* rho, const double* r, double* d, double* rs_out, double rnorm = -1.0) {
+    LearnedNet* N = ctx->net;
+    const long nv = ctx->nv;
+    int rc = PF_OK;
+    const double rn = rnorm >= 0.0 ? rnorm : norm_nodal(ctx, r, &rc);
+    PF_TRY(rc);
+    const double rs = rn > 0 ? rn / std::sqrt((double)std::max(1, ctx->N)) : 1.0;
+    PF_TRY(net_parts(ctx, rho, r, rn));
     double a = 1.0, c = 0.0;
     PF_TRY(optimal_pair(ctx, r, &a, &c));
     k_lincomb_d<<<gsize(nv), kBlock, 0, ctx->stream>>>(nv, a, N->dd, c, N->w, d);
@@ -1177,7 +1203,12 @@ int multidot(pf_ctx* ctx, const double* a, const double* const* b, int nb, doubl
     PtrPack P{};
     for (int j = 0; j < nb; j++) P.p[j] = b[j];
     const unsigned g = std::min<unsigned>(gsize(ctx->nv), 256u);
-    k_multidot<<<g, kBlock, 0, ctx->stream>>>(ctx->nv, nb, a, P, N->mpart);
+    if (nb <= 4)
+        k_multidot<4><<<g, kBlock, 0, ctx->stream>>>(ctx->nv, nb, a, P, N->mpart);
+    else if (nb <= 8)
+        k_multidot<8><<<g, kBlock, 0, ctx->stream>>>(ctx->nv, nb, a, P, N->mpart);
+    else
+        k_multidot<32><<<g, kBlock, 0, ctx->stream>>>(ctx->nv, nb, a, P, N->mpart);
     PF_TRY(check_launch(ctx));
     std::vector<double> h((size_t)g * nb);
     PF_HIP(hipMemcpyAsync(h.data(), N->mpart, sizeof(double) * g * nb, hipMemcpyDeviceToHost, ctx->stream));
@@ -1190,6 +1221,58 @@ int multidot(pf_ctx* ctx, const double* a, const double* const* b, int nb, doubl
     return PF_OK;
 }
 
+// Several sets of dot products (a_k . b_kj) in one batch: the launches
+// (at most 32 vectors each), one copy of the partials and one host
+// synchronisation; sums in a fixed order on the host.
+struct DotJob {
+    const double* a;
+    std::vector<const double*> b;
+    std::vector<double> out;
+};
+int multidot_batch(pf_ctx* ctx, std::vector<DotJob>& jobs) {
+    LearnedNet* N = ctx->net;
+    const unsigned g = std::min<unsigned>(gsize(ctx->nv), 256u);
+    struct Chunk {
+        int job, first, nb;
+        long off;
+    };
+    std::vector<Chunk> ch;
+    long off = 0;
+    for (int j = 0; j < (int)jobs.size(); j++) {
+        const int nbj = (int)jobs[j].b.size();
+        jobs[j].out.assign(nbj, 0.0);
+        for (int f = 0; f < nbj; f += 32) {
+            const int nb = std::min(32, nbj - f);
+            ch.push_back({j, f, nb, off});
+            off += (long)g * nb;
+        }
+    }
+    PF_CHECK(off <= 256L * kMpartDots, PF_ERR_ARG, "too many dot products in one batch");
+    for (const Chunk& c : ch) {
+        PtrPack P{};
+        for (int q = 0; q < c.nb; q++) P.p[q] = jobs[c.job].b[c.first + q];
+        const double* a = jobs[c.job].a;
+        double* part = N->mpart + c.off;
+        if (c.nb <= 4)
+            k_multidot<4><<<g, kBlock, 0, ctx->stream>>>(ctx->nv, c.nb, a, P, part);
+        else if (c.nb <= 8)
+            k_multidot<8><<<g, kBlock, 0, ctx->stream>>>(ctx->nv, c.nb, a, P, part);
+        else
+            k_multidot<32><<<g, kBlock, 0, ctx->stream>>>(ctx->nv, c.nb, a, P, part);
+    }
+    PF_TRY(check_launch(ctx));
+    std::vector<double> h(off);
+    PF_HIP(hipMemcpyAsync(h.data(), N->mpart, sizeof(double) * off, hipMemcpyDeviceToHost, ctx->stream));
+    PF_TRY(sync(ctx));
+    for (const Chunk& c : ch)
+        for (int q = 0; q < c.nb; q++) {
+            double s = 0.0;
+            for (unsigned b = 0; b < g; b++) s += h[c.off + (long)b * c.nb + q];
+            jobs[c.job].out[c.first + q] = s;
+        }
+    return PF_OK;
+}
+
 int optimal_pair(pf_ctx* ctx, const double* r, double* a, double* c) {
     LearnedNet* N = ctx->net;
     double d1[3], d2[2];
@@ -1197,15 +1280,7 @@ int optimal_pair(pf_ctx* ctx, const double* r, double* a, double* c) {
     PF_TRY(multidot(ctx, N->dd, b1, 3, d1));   // t.Kt, t.Ky, t.r
     const double* b2[2] = {N->Kw, r};
     PF_TRY(multidot(ctx, N->w, b2, 2, d2));    // y.Ky, y.r
-    const double tkt = d1[0], tky = d1[1], tr = d1[2], yky = d2[0], yr = d2[1];
-    const double det = tkt * yky - tky * tky;
-    if (std::isfinite(det) && det > 1e-12 * tkt * yky && yky > 0.0) {
-        *a = (yky * tr - tky * yr) / det;
-        *c = (tkt * yr - tky * tr) / det;
-    } else {
-        *a = tkt > 0.0 ? tr / tkt : 1.0;
-        *c = 0.0;
-    }
+    pair_from_sums(d1[0], d1[1], d1[2], d2[0], d2[1], a, c);
     return PF_OK;
 }
 
@@ -1343,31 +1418,29 @@ int net_collect_random(pf_ctx* ctx, int count, uint64_t seed, int* total) {
 
 // Stored directions of the flexible CG.  With the network over the Jacobi
 // step, m = 8 needs about 1.4x the iterations of m = 32 on the 96 x 48
-// cantilever; over the V-cycle the preconditioner varies far less and a few
-// directions suffice.  CoefPack/PtrPack hold 32.
+// cantilever; over the V-cycle the preconditioner varies far less: at
+// 1024 x 512, m = 4 needs the iterations of m = 8 (53) at less traffic per
+// iteration, m = 2 needs 62.  CoefPack/PtrPack hold 32.
 constexpr int kFcgDirections = 32;
-constexpr int kFcgDirectionsMg = 8;
-constexpr int kFcgDirectionsMgOnly = 2;
+#ifndef PF_FCG_DIRECTIONS_MG
+#define PF_FCG_DIRECTIONS_MG 4
+#endif
+constexpr int kFcgDirectionsMg = PF_FCG_DIRECTIONS_MG;
 
-// Flexible CG (Notay's FCG(m)): each new direction is made K-conjugate to
-// the last m stored ones, which keeps the Krylov-like acceleration that a
-// varying preconditioner would otherwise lose.  M is the network correction
-// over the base step (use_net), or the base step alone; the base step is the
-// multigrid V-cycle after pf_mg_setup, else the Jacobi step r ./ D.  FP64
+// Flexible CG (Notay's FCG(m)) with M = the network correction over the base
+// step (the multigrid V-cycle after pf_mg_setup, else the Jacobi step
+// r ./ D): each new direction is made K-conjugate to the last m stored ones,
+// which keeps the Krylov-like acceleration that a varying preconditioner
+// would otherwise lose.  Per iteration two batched reductions and no
+// operator application of its own (K p comes from K t and K y).  FP64
 // vectors; stops on the FP64 true residual; when M breaks down, diverges
 // (residual 10x above its best) or stagnates (no halving in `stall` steps)
 // Jacobi-PCG takes over from the current iterate.
-int pcg_learned(pf_ctx* ctx, const double* b, double* x, double eps, int itrmax, pf_solve_stats& st,
-                bool use_net = true) {
+int pcg_learned(pf_ctx* ctx, const double* b, double* x, double eps, int itrmax, pf_solve_stats& st) {
     LearnedNet* N = ctx->net;
-    if (use_net) PF_CHECK(N && N->trained, PF_ERR_STATE, "learned preconditioner has not been trained");
-    PF_CHECK(use_net || ctx->mg, PF_ERR_STATE, "multigrid not set up: call pf_mg_setup first");
-    if (!N) {
-        PF_TRY(net_build(ctx, 1));  // work buffers for the directions (the network itself stays untrained)
-        N = ctx->net;
-    }
+    PF_CHECK(N && N->trained, PF_ERR_STATE, "learned preconditioner has not been trained");
     const long nv = ctx->nv;
-    const int m = !ctx->mg ? kFcgDirections : (use_net ? kFcgDirectionsMg : kFcgDirectionsMgOnly);
+    const int m = !ctx->mg ? kFcgDirections : kFcgDirectionsMg;
     if (N->fcg_m != m) {
         N->fP.assign(m, nullptr);
         N->fQ.assign(m, nullptr);
@@ -1391,44 +1464,66 @@ int pcg_learned(pf_ctx* ctx, const double* b, double* x, double eps, int itrmax,
     const int stall = 2000;
     bool fallback = false;
     for (; k < itrmax; k++) {
-        if (use_net)
-            PF_TRY(net_apply(ctx, ctx->d_rho, r, z, nullptr, rn));
-        else
-            PF_TRY(mg_apply(ctx, r, z));
         const int nst = std::min(k, m);
         const int slot = k % m;
         double* p = N->fP[slot];
         double* q = N->fQ[slot];
-        // p = z - sum_j (z.q_j / p_j.q_j) p_j over the stored directions
         CoefPack cf{};
-        PtrPack Pp{};
-        if (nst > 0) {
-            std::vector<const double*> qs(nst);
-            std::vector<double> zq(nst);
-            for (int j = 0; j < nst; j++) qs[j] = N->fQ[(k - 1 - j + m) % m];
-            PF_TRY(multidot(ctx, z, qs.data(), nst, zq.data()));
-            for (int j = 0; j < nst; j++) {
-                const int sj = (k - 1 - j + m) % m;
-                cf.c[j] = zq[j] / pq[sj];
-                Pp.p[j] = N->fP[sj];
-            }
+        PtrPack Pp{}, Qp{};
+        std::vector<const double*> qs(nst);
+        for (int j = 0; j < nst; j++) {
+            const int sj = (k - 1 - j + m) % m;
+            qs[j] = N->fQ[sj];
+            Pp.p[j] = N->fP[sj];
+            Qp.p[j] = N->fQ[sj];
         }
-        k_fcg_direction<<<gsize(nv), kBlock, 0, ctx->stream>>>(nv, nst, z, Pp, cf, p);
-        PF_TRY(apply_nodal(ctx, false, p, q, true));
-        double d2[2];
-        const double* bq[2] = {q, r};
-        PF_TRY(multidot(ctx, p, bq, 2, d2));
-        const double pqv = d2[0], pr = d2[1];
+        {
+            // t, y and their images, then one batch for the pair (a, c) and for
+            // z.q_j = a t.q_j + c y.q_j (z = a t + c y is never formed)
+            PF_TRY(net_parts(ctx, ctx->d_rho, r, rn));
+            std::vector<DotJob> jobs(2);
+            jobs[0].a = N->dd;
+            jobs[0].b = {N->gd, N->Kw, r};
+            jobs[1].a = N->w;
+            jobs[1].b = {N->Kw, r};
+            for (const double* qj : qs) {
+                jobs[0].b.push_back(qj);
+                jobs[1].b.push_back(qj);
+            }
+            PF_TRY(multidot_batch(ctx, jobs));
+            const std::vector<double>& T = jobs[0].out;
+            const std::vector<double>& Y = jobs[1].out;
+            double a, c;
+            pair_from_sums(T[0], T[1], T[2], Y[0], Y[1], &a, &c);
+            for (int j = 0; j < nst; j++) cf.c[j] = (a * T[3 + j] + c * Y[2 + j]) / pq[(k - 1 - j + m) % m];
+            // p = z - sum_j c_j p_j and q = K p = K z - sum_j c_j q_j
+            k_fcg_direction2<<<gsize(nv), kBlock, 0, ctx->stream>>>(nv, nst, a, c, N->dd, N->w, N->gd, N->Kw, Pp,
+                                                                    Qp, cf, p, q);
+        }
+        // p.q, p.r and, for the new residual's norm, q.q, q.r
+        std::vector<DotJob> step(2);
+        step[0].a = p;
+        step[0].b = {q, r};
+        step[1].a = q;
+        step[1].b = {q, r};
+        PF_TRY(multidot_batch(ctx, step));
+        const double pqv = step[0].out[0], pr = step[0].out[1], qq = step[1].out[0], qr = step[1].out[1];
         if (!(pqv > 0.0) || !std::isfinite(pr)) {  // breakdown: M(r) is not a descent direction
             fallback = true;
             break;
         }
         pq[slot] = pqv;
         const double alpha = pr / pqv;
-        k_axpy_d<<<gsize(nv), kBlock, 0, ctx->stream>>>(nv, alpha, p, x);
-        k_axpy_d<<<gsize(nv), kBlock, 0, ctx->stream>>>(nv, -alpha, q, r);
-        rn = norm_nodal(ctx, r, &rc);
-        PF_TRY(rc);
+        k_step2<<<gsize(nv), kBlock, 0, ctx->stream>>>(nv, alpha, p, q, x, r);
+        // |r - alpha q|^2 = |r|^2 - 2 alpha q.r + alpha^2 q.q; computed directly
+        // when that cancels (a step that removes nearly all of r)
+        const double rr_new = rn * rn - 2.0 * alpha * qr + alpha * alpha * qq;
+        if (rr_new > 1e-4 * rn * rn) {
+            rn = std::sqrt(rr_new);
+        } else {
+            rn = norm_nodal(ctx, r, &rc);
+            PF_TRY(rc);
+        }
         if (!std::isfinite(rn)) {
             fallback = true;
             break;

@@ -557,7 +557,7 @@ __global__ void __launch_bounds__(256) k_cg1_q4_tile(int nx, int ny, Cg1Tiles G,
     constexpr int PER = ((TI + 2) * (TJ + 2) + 255) / 256;  // staged entries per thread
     constexpr int PE = ((TI + 1) * (TJ + 1) + 255) / 256;   // element moduli per thread
     constexpr int OWN = (TI * TJ + 255) / 256;              // own nodes per thread
-    __shared__ double lds[32];
+    __shared__ double lds[48];  // block_sum<3>: 16 per sum
     __shared__ Vec2<T> su[(TI + 2) * (TJ + 2)];  // u_m on the tile and its halo, row stride tj + 2
     __shared__ Vec2<T> uo[TI * TJ];              // u_{m-1} = r_{m-1} ./ D at the own nodes, stride tj
     __shared__ T se[(TI + 1) * (TJ + 1)];        // moduli of the tile's elements, stride tj + 1
@@ -597,6 +597,20 @@ __global__ void __launch_bounds__(256) k_cg1_q4_tile(int nx, int ny, Cg1Tiles G,
             const int a = h / (tj + 1), b = h - a * (tj + 1);
             const int ei = i0 - 1 + a, ej = j0 - 1 + b;
             ev[u] = (h < es && ei >= 0 && ei < nx && ej >= 0 && ej < ny) ? E[(long)ny * ei + ej] : (T)0;
+        }
+    };
+    auto load_px = [&](Vec2<T>(&pv)[OWN], Vec2<T>(&xv)[OWN]) {
+#pragma unroll
+        for (int o = 0; o < OWN; o++) {
+            const int q = threadIdx.x + o * 256;
+            const int r = q / tj, c = q - r * tj;
+            const int i = i0 + r, j = j0 + c;
+            pv[o] = xv[o] = Vec2<T>{0, 0};
+            if (prev && has && q < own && i <= nx && j <= ny) {
+                const long n = (long)(ny + 1) * i + j;
+                pv[o] = load2(V.p, n);
+                xv[o] = load2(V.x, n);
+            }
         }
     };
     // the first tile's loads, the previous launch's partials and the done
@@ -648,20 +662,10 @@ __global__ void __launch_bounds__(256) k_cg1_q4_tile(int nx, int ny, Cg1Tiles G,
             const int h = threadIdx.x + u * 256;
             if (h < es) se[h] = ev[u];
         }
-        // p_{m-2} and x_{m-1} of this thread's own nodes: in flight during the operator
+        // p_{m-2} and x_{m-1} of this thread's own nodes: in flight during the
+        // operator (issued with the other loads they would cost occupancy)
         Vec2<T> pv[OWN], xv[OWN];
-#pragma unroll
-        for (int o = 0; o < OWN; o++) {
-            const int q = threadIdx.x + o * 256;
-            const int r = q / tj, c = q - r * tj;
-            const int i = i0 + r, j = j0 + c;
-            pv[o] = xv[o] = Vec2<T>{0, 0};
-            if (prev && q < own && i <= nx && j <= ny) {
-                const long n = (long)(ny + 1) * i + j;
-                pv[o] = load2(V.p, n);
-                xv[o] = load2(V.x, n);
-            }
-        }
+        load_px(pv, xv);
         __syncthreads();
 #pragma unroll
         for (int o = 0; o < OWN; o++) {
@@ -721,7 +725,7 @@ __global__ void k_cg1_start(PcgDev* __restrict__ S) {
 // After the host's last launch (M launches: iterate M-1 is the last one):
 // the convergence test of that iterate and the itrmax exit (CG.h:451-452).
 __global__ void k_cg1_final(int M, const double* __restrict__ partIn, int nbIn, PcgDev* __restrict__ S, double eps) {
-    __shared__ double lds[32];
+    __shared__ double lds[48];  // block_sum<3>: 16 per sum
     if (S->done) return;
     double v[3];
     reduce_partials<3>(partIn, nbIn, lds, v);

@@ -261,29 +261,51 @@ struct CoefPack {
     double c[32];
 };
 
+// NB: the accumulator count (>= nb), so that they stay in registers.
+template <int NB>
 __global__ void __launch_bounds__(256) k_multidot(long n, int nb, const double* __restrict__ a, PtrPack B,
                                                   double* __restrict__ part) {
-    __shared__ double lds[16];
-    double acc[32];
-    for (int j = 0; j < nb; j++) acc[j] = 0.0;
+    __shared__ double lds[16 * NB];
+    double acc[NB];
+#pragma unroll
+    for (int j = 0; j < NB; j++) acc[j] = 0.0;
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
         const double ai = a[i];
-        for (int j = 0; j < nb; j++) acc[j] += ai * B.p[j][i];
+#pragma unroll
+        for (int j = 0; j < NB; j++)
+            if (j < nb) acc[j] += ai * B.p[j][i];
     }
-    for (int j = 0; j < nb; j++) {
-        double v[1] = {acc[j]};
-        block_sum<1>(v, lds);
-        if (threadIdx.x == 0) part[blockIdx.x * nb + j] = v[0];
+    block_sum<NB>(acc, lds);
+    if (threadIdx.x == 0)
+        for (int j = 0; j < nb; j++) part[blockIdx.x * nb + j] = acc[j];
+}
+
+// The direction and its image without an operator application: with
+// z = a t + c y and K z = a Kt + c Ky,
+//   p = z - sum_j c_j P_j,  q = K p = K z - sum_j c_j Q_j.
+__global__ void __launch_bounds__(256) k_fcg_direction2(long n, int m, double a, double c, const double* __restrict__ t,
+                                                        const double* __restrict__ y, const double* __restrict__ Kt,
+                                                        const double* __restrict__ Ky, PtrPack P, PtrPack Q,
+                                                        CoefPack cf, double* __restrict__ p, double* __restrict__ q) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        double v = a * t[i] + c * y[i];
+        double w = a * Kt[i] + c * Ky[i];
+        for (int j = 0; j < m; j++) {
+            v -= cf.c[j] * P.p[j][i];
+            w -= cf.c[j] * Q.p[j][i];
+        }
+        p[i] = v;
+        q[i] = w;
     }
 }
 
-// p = z - sum_j c_j P_j
-__global__ void __launch_bounds__(256) k_fcg_direction(long n, int m, const double* __restrict__ z, PtrPack P,
-                                                       CoefPack c, double* __restrict__ p) {
+// x += alpha p, r -= alpha q
+__global__ void __launch_bounds__(256) k_step2(long n, double alpha, const double* __restrict__ p,
+                                               const double* __restrict__ q, double* __restrict__ x,
+                                               double* __restrict__ r) {
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-        double v = z[i];
-        for (int j = 0; j < m; j++) v -= c.c[j] * P.p[j][i];
-        p[i] = v;
+        x[i] += alpha * p[i];
+        r[i] -= alpha * q[i];
     }
 }
 
