@@ -261,29 +261,34 @@ int pf_simp_run(pf_ctx* ctx, const pf_simp_params* prm, int iters, double* s, do
  * 0.7, 1.2, 1e-6), density filter; Heaviside continuation 0.5, 2, 40). */
 void pf_simp_default_params(pf_simp_params* prm);
 
-/* ---- multigrid base preconditioner (quad4) ------------------------------ */
+/* ---- multigrid base preconditioner (quad4 and hex8) --------------------- */
 /* No counterpart in the reference.  pf_mg_setup makes a geometric multigrid
  * V-cycle the base step of the flexible CG: pf_pcg with PF_OP_MG (FCG with
  * the V-cycle alone) and PF_OP_LEARNED (the network correction over the
  * V-cycle instead of over the Jacobi step; train after pf_mg_setup so the
  * network learns against the same base).  Coarse levels halve both element
- * counts down to at most 512 nodal DOFs (or an odd count); a coarse
+ * counts down to at most 90 nodal DOFs (or an odd count); a coarse
  * element's matrix is the Galerkin product of its four children's
  * (bilinear prolongation, fixed DOFs removed at the finest level), built in
  * FP64 and applied in FP32; nu damped-Jacobi sweeps (weight omega) before
  * and after the coarse correction; a dense solve on the coarsest level.  The
  * hierarchy is rebuilt from the current moduli at the start of every solve.
- * nu = 0 switches the multigrid off. */
+ * nu = 0 switches the multigrid off; omega <= 0 takes the element's default
+ * (0.6 quad4, 0.4 hex8: damped Jacobi smooths only below 2 / lambda_max of
+ * D^-1 K, about 0.67 and 0.48). */
 int pf_mg_setup(pf_ctx* ctx, int nu, double omega);
 /* Build the hierarchy for the current moduli now (solves do it themselves). */
 int pf_mg_build(pf_ctx* ctx);
 /* One V-cycle z = V(r) on free-DOF host vectors, for the current moduli. */
 int pf_mg_apply(pf_ctx* ctx, const double* r_free, double* z_free);
-/* Level `level` of the built hierarchy: its element counts, and (when the
- * pointers are not null) its FP64 element matrices (ne x 64, row-major per
- * element, nodes BL, BR, TR, TL, x before y; level >= 1), FP32 diagonal and
- * nodal fixity bits (bit 0 x, bit 1 y).  *levels receives the level count. */
-int pf_mg_level(pf_ctx* ctx, int level, int* levels, int* nx, int* ny, double* Ke, float* D, uint8_t* fix);
+/* Level `level` of the built hierarchy: its element counts (nz = 0 for
+ * quad4), and (when the pointers are not null) its FP64 element matrices
+ * (level >= 1; row-major per element, element-local nodes in the shape
+ * function's order, DOFs of a node together: ne x 64 for quad4, ne x 576 for
+ * hex8), FP32 diagonal and nodal fixity bits (bit d: DOF d fixed).  *levels
+ * receives the level count. */
+int pf_mg_level(pf_ctx* ctx, int level, int* levels, int* nx, int* ny, int* nz, double* Ke, float* D,
+                uint8_t* fix);
 
 /* ---- measurement -------------------------------------------------------- */
 /* Average device duration (HIP events on the context's stream, `reps`

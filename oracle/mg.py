@@ -171,3 +171,106 @@ def pcg(apply_A, b, M, eps=1e-8, itmax=100000):
         p = z + rz1 / rz * p
         rz = rz1
     return x, itmax
+
+
+# ------------------------------------------------------------------- hex8 ----
+# Nodes (i, j, k) -> ((ny+1)(nz+1)) i + (nz+1) j + k, element (i, j, k) ->
+# ny nz i + nz j + k, element-local node q at offsets (0,0,0), (1,0,0),
+# (1,1,0), (0,1,0), then the same at dk = 1; 3 DOFs per node.
+LX3 = np.array([0, 1, 1, 0, 0, 1, 1, 0])
+LY3 = np.array([0, 0, 1, 1, 0, 0, 1, 1])
+LZ3 = np.array([0, 0, 0, 0, 1, 1, 1, 1])
+
+
+def element_dofs3(nx, ny, nz):
+    I, J, K = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    I, J, K = I.ravel(), J.ravel(), K.ravel()
+    pl = (ny + 1) * (nz + 1)
+    nodes = np.stack([pl * (I + LX3[q]) + (nz + 1) * (J + LY3[q]) + K + LZ3[q] for q in range(8)], 1)
+    return np.stack([3 * nodes, 3 * nodes + 1, 3 * nodes + 2], 2).reshape(-1, 24)
+
+
+def assemble3(nx, ny, nz, Ke):
+    dofs = element_dofs3(nx, ny, nz)
+    r = np.repeat(dofs, 24, 1).ravel()
+    c = np.tile(dofs, (1, 24)).ravel()
+    n = 3 * (nx + 1) * (ny + 1) * (nz + 1)
+    return sp.csr_matrix((Ke.reshape(-1), (r, c)), shape=(n, n))
+
+
+def fix_dofs3(fix):
+    f = np.asarray(fix, np.uint8)
+    return np.stack([(f & 1) > 0, (f & 2) > 0, (f & 4) > 0], 1).reshape(-1)
+
+
+def level0_matrices3(nx, ny, nz, E, K0, fix):
+    Ke = np.asarray(E, float)[:, None, None] * np.asarray(K0, float).reshape(24, 24)[None]
+    keep = (~fix_dofs3(fix)[element_dofs3(nx, ny, nz)]).astype(float)
+    return Ke * keep[:, :, None] * keep[:, None, :]
+
+
+def child_prolongation3(a0, a1, a2):
+    """24 x 24 map from a coarse hex8 element's DOFs to child (a0, a1, a2)'s."""
+    P = np.zeros((24, 24))
+    for k in range(8):
+        x, y, z = 0.5 * (a0 + LX3[k]), 0.5 * (a1 + LY3[k]), 0.5 * (a2 + LZ3[k])
+        for q in range(8):
+            w = (x if LX3[q] else 1 - x) * (y if LY3[q] else 1 - y) * (z if LZ3[q] else 1 - z)
+            for d in range(3):
+                P[3 * k + d, 3 * q + d] = w
+    return P
+
+
+def coarsen3(nx, ny, nz, Ke):
+    cnx, cny, cnz = nx // 2, ny // 2, nz // 2
+    Kf = Ke.reshape(nx, ny, nz, 24, 24)
+    Kc = np.zeros((cnx, cny, cnz, 24, 24))
+    for a0 in range(2):
+        for a1 in range(2):
+            for a2 in range(2):
+                P = child_prolongation3(a0, a1, a2)
+                Kc += np.einsum("ki,IJLkl,lj->IJLij", P, Kf[a0::2, a1::2, a2::2], P)
+    return Kc.reshape(-1, 24, 24)
+
+
+def prolongation3(nx, ny, nz):
+    """Trilinear P from the (nx/2, ny/2, nz/2) grid's nodes to the fine grid's."""
+    cny, cnz = ny // 2, nz // 2
+    rows, cols, vals = [], [], []
+    for i in range(nx + 1):
+        for j in range(ny + 1):
+            for k in range(nz + 1):
+                f = (ny + 1) * (nz + 1) * i + (nz + 1) * j + k
+                for a in range(i % 2 + 1):
+                    for b in range(j % 2 + 1):
+                        for c in range(k % 2 + 1):
+                            w = (0.5 if i % 2 else 1.0) * (0.5 if j % 2 else 1.0) * (0.5 if k % 2 else 1.0)
+                            cn = (cny + 1) * (cnz + 1) * (i // 2 + a) + (cnz + 1) * (j // 2 + b) + k // 2 + c
+                            for d in range(3):
+                                rows.append(3 * f + d)
+                                cols.append(3 * cn + d)
+                                vals.append(w)
+    n_f = 3 * (nx + 1) * (ny + 1) * (nz + 1)
+    n_c = 3 * (nx // 2 + 1) * (ny // 2 + 1) * (nz // 2 + 1)
+    return sp.csr_matrix((vals, (rows, cols)), shape=(n_f, n_c))
+
+
+def build3(nx, ny, nz, E, K0, fix, coarse_dofs=90):
+    """Hex8 levels as the device builds them."""
+    Ke = level0_matrices3(nx, ny, nz, E, K0, fix)
+    free0 = ~fix_dofs3(fix)
+    levels = []
+    while True:
+        A = assemble3(nx, ny, nz, Ke)
+        d = A.diagonal().copy()
+        free = free0 if not levels else d > 0
+        levels.append(dict(nx=nx, ny=ny, nz=nz, Ke=Ke, A=A, free=free, D=np.where(free, d, 1.0)))
+        if 3 * (nx + 1) * (ny + 1) * (nz + 1) <= coarse_dofs or nx % 2 or ny % 2 or nz % 2:
+            break
+        levels[-1]["P"] = prolongation3(nx, ny, nz)
+        Ke = coarsen3(nx, ny, nz, Ke)
+        nx, ny, nz = nx // 2, ny // 2, nz // 2
+    c = levels[-1]
+    f = c["free"]
+    c["inv"] = np.linalg.inv(c["A"][f][:, f].toarray())
+    return levels

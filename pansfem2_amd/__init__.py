@@ -105,7 +105,8 @@ def lib():
             "pf_mg_setup": ([c_void_p, c_int, c_double], c_int),
             "pf_mg_build": ([c_void_p], c_int),
             "pf_mg_apply": ([c_void_p, _f64p, _f64p], c_int),
-            "pf_mg_level": ([c_void_p, c_int, P(c_int), P(c_int), P(c_int), c_void_p, c_void_p, c_void_p], c_int),
+            "pf_mg_level": ([c_void_p, c_int, P(c_int), P(c_int), P(c_int), P(c_int), c_void_p, c_void_p, c_void_p],
+                            c_int),
             "pf_compliance_sens": ([c_void_p, _f64p, c_double, P(c_double), _f64p], c_int),
             "pf_filter_setup": ([c_void_p, c_double], c_int),
             "pf_filter": ([c_void_p, c_int, _f64p, _f64p], c_int),
@@ -283,9 +284,10 @@ class Context:
         self._ck(lib().pf_apply_K(self._h, op, _f64(x), y))
         return y
 
-    def mg_setup(self, nu=1, omega=0.6):
+    def mg_setup(self, nu=1, omega=0.0):
         """Geometric multigrid V-cycle as the flexible CG's base step
-        (PF_OP_MG, and PF_OP_LEARNED's base); nu = 0 switches it off."""
+        (PF_OP_MG, and PF_OP_LEARNED's base); nu = 0 switches it off;
+        omega <= 0: the element's default Jacobi weight (0.6 quad4, 0.4 hex8)."""
         self._ck(lib().pf_mg_setup(self._h, nu, omega))
 
     def mg_build(self):
@@ -297,17 +299,20 @@ class Context:
         return z
 
     def mg_level(self, level, want_ke=True):
-        """(nx, ny, Ke (ne x 8 x 8) or None, D, fix) of a built level."""
-        nl, nx, ny = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(0)
-        self._ck(lib().pf_mg_level(self._h, level, ctypes.byref(nl), ctypes.byref(nx), ctypes.byref(ny),
-                                   None, None, None))
-        nn = (nx.value + 1) * (ny.value + 1)
-        ke = np.zeros((nx.value * ny.value, 8, 8)) if (want_ke and level > 0) else None
-        D = np.zeros(2 * nn, np.float32)
+        """(nx, ny, nz, Ke (ne x 8 x 8, or ne x 24 x 24 for hex8) or None, D, fix) of a built level."""
+        nl, nx, ny, nz = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(0)
+        refs = (ctypes.byref(nl), ctypes.byref(nx), ctypes.byref(ny), ctypes.byref(nz))
+        self._ck(lib().pf_mg_level(self._h, level, *refs, None, None, None))
+        h8 = nz.value > 0
+        nn = (nx.value + 1) * (ny.value + 1) * (nz.value + 1 if h8 else 1)
+        ne = nx.value * ny.value * (nz.value if h8 else 1)
+        nd = 24 if h8 else 8
+        ke = np.zeros((ne, nd, nd)) if (want_ke and level > 0) else None
+        D = np.zeros((3 if h8 else 2) * nn, np.float32)
         fix = np.zeros(nn, np.uint8)
-        self._ck(lib().pf_mg_level(self._h, level, ctypes.byref(nl), ctypes.byref(nx), ctypes.byref(ny),
-                                   ke.ctypes.data if ke is not None else None, D.ctypes.data, fix.ctypes.data))
-        return dict(levels=nl.value, nx=nx.value, ny=ny.value, Ke=ke, D=D, fix=fix)
+        self._ck(lib().pf_mg_level(self._h, level, *refs, ke.ctypes.data if ke is not None else None,
+                                   D.ctypes.data, fix.ctypes.data))
+        return dict(levels=nl.value, nx=nx.value, ny=ny.value, nz=nz.value, Ke=ke, D=D, fix=fix)
 
     def set_pcg_form(self, form):
         """PF_PCG_SCALINGCG (ScalingCG's order, the default) or

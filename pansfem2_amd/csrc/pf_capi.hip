@@ -6,6 +6,7 @@
 #include "pf_hex8.hip"
 #include "pf_learned.hip"
 #include "pf_mg.hip"
+#include "pf_mg3.hip"
 
 #include <algorithm>
 #include <chrono>
@@ -546,9 +547,9 @@ struct NetLayer {
 // Levels of the V-cycle (k_mg_* kernels): level 0 is the grid (E, K0, the
 // context's fixity); coarser levels hold FP64 and FP32 element matrices.
 struct MgLevel {
-    int nx = 0, ny = 0;
+    int nx = 0, ny = 0, nz = 0;  // nz > 0: hex8
     long nn = 0, nv = 0, ne = 0;
-    double* Ke64 = nullptr;  // levels >= 1, and level 0 when it is the coarsest
+    double* Ke64 = nullptr;  // levels >= 1, and level 0 when it is the coarsest (64 or 576 per element)
     float* Ke32 = nullptr;   // levels >= 1
     float* D = nullptr;
     uint8_t* fix = nullptr;  // levels >= 1 (level 0 uses the context's)
@@ -867,35 +868,39 @@ int mg_plan(pf_ctx* ctx, int nu, float omega) {
     MgState* S = ctx->mg;
     S->nu = nu;
     S->omega = omega;
-    int nx = ctx->nx, ny = ctx->ny;
+    const bool h8 = !is2d(ctx);
+    int nx = ctx->nx, ny = ctx->ny, nz = h8 ? ctx->nz : 0;
     for (;;) {
         MgLevel a;
         a.nx = nx;
         a.ny = ny;
-        a.nn = (long)(nx + 1) * (ny + 1);
-        a.nv = 2 * a.nn;
-        a.ne = (long)nx * ny;
+        a.nz = nz;
+        a.nn = (long)(nx + 1) * (ny + 1) * (h8 ? nz + 1 : 1);
+        a.nv = (h8 ? 3 : 2) * a.nn;
+        a.ne = (long)nx * ny * (h8 ? nz : 1);
         S->L.push_back(a);
-        if (a.nv <= kMgCoarse || nx % 2 || ny % 2) break;
+        if (a.nv <= kMgCoarse || nx % 2 || ny % 2 || (h8 && nz % 2)) break;
         nx /= 2;
         ny /= 2;
+        nz /= 2;
     }
     const int c = (int)S->L.size() - 1;
     PF_CHECK(S->L[c].nv <= kMgDenseMax, PF_ERR_UNSUPPORTED,
              "multigrid: the grid does not coarsen to a small enough level (element counts must be divisible by "
              "powers of two)");
+    const long kes = h8 ? 576 : 64;
     for (int l = 0; l <= c; l++) {
         MgLevel& a = S->L[l];
-        if (l > 0 || l == c) PF_TRY(mg_alloc(ctx, &a.Ke64, a.ne * 64));
+        if (l > 0 || l == c) PF_TRY(mg_alloc(ctx, &a.Ke64, a.ne * kes));
         if (l > 0) {
-            PF_TRY(mg_alloc(ctx, &a.Ke32, a.ne * 64));
+            PF_TRY(mg_alloc(ctx, &a.Ke32, a.ne * kes));
             PF_TRY(mg_alloc(ctx, &a.fix, a.nn));
         }
         float** v[] = {&a.D, &a.r, &a.x, &a.x2, &a.res, &a.res2};
         for (float** q : v) PF_TRY(mg_alloc(ctx, q, a.nv));
     }
     // the small levels (and the coarsest solve) run in one workgroup
-    if (S->L[c].nv <= kMgLdsDense) {
+    if (!h8 && S->L[c].nv <= kMgLdsDense) {
         int t = c;
         while (t - 1 >= 1 && S->L[t - 1].nn <= kMgTailNodes && c - (t - 1) + 1 <= kMgTailMax) t--;
         if (S->L[t].nn <= kMgTailNodes && t >= 1) S->tail = t;
@@ -912,19 +917,39 @@ int mg_build(pf_ctx* ctx) {
     MgState* S = ctx->mg;
     PF_CHECK(S, PF_ERR_STATE, "multigrid not set up: call pf_mg_setup first");
     const int c = (int)S->L.size() - 1;
+    const bool h8 = S->L[0].nz > 0;
+    const int ndpn = h8 ? 3 : 2;
     MgLevel& l0 = S->L[0];
-    k_mg_diag0<<<gsize(l0.nn), kBlock, 0, ctx->stream>>>(l0.nx, l0.ny, ctx->d_E, ctx->d_K0, ctx->d_fix, l0.D);
+    if (h8)
+        k_mg3_diag0<<<gsize(l0.nn), kBlock, 0, ctx->stream>>>(l0.nx, l0.ny, l0.nz, ctx->d_E, ctx->d_K0, ctx->d_fix,
+                                                             l0.D);
+    else
+        k_mg_diag0<<<gsize(l0.nn), kBlock, 0, ctx->stream>>>(l0.nx, l0.ny, ctx->d_E, ctx->d_K0, ctx->d_fix, l0.D);
     for (int l = 1; l <= c; l++) {
         MgLevel& a = S->L[l];
-        const long t = a.ne * 8;
-        k_mg_coarsen<<<(unsigned)((t + kBlock - 1) / kBlock), kBlock, 0, ctx->stream>>>(
-            a.nx, a.ny, ctx->d_E, ctx->d_K0, ctx->d_fix, l == 1 ? nullptr : S->L[l - 1].Ke64, a.Ke64, a.Ke32);
-        k_mg_diag_full<<<gsize(a.nn), kBlock, 0, ctx->stream>>>(a.nx, a.ny, a.Ke64, a.D, a.fix);
+        const double* Kf = l == 1 ? nullptr : S->L[l - 1].Ke64;
+        if (h8) {
+            const long t = a.ne * 24;
+            k_mg3_coarsen<<<(unsigned)((t + kBlock - 1) / kBlock), kBlock, 0, ctx->stream>>>(
+                a.nx, a.ny, a.nz, ctx->d_E, ctx->d_K0, ctx->d_fix, Kf, a.Ke64, a.Ke32);
+            k_mg3_diag_full<<<gsize(a.nn), kBlock, 0, ctx->stream>>>(a.nx, a.ny, a.nz, a.Ke64, a.D, a.fix);
+        } else {
+            const long t = a.ne * 8;
+            k_mg_coarsen<<<(unsigned)((t + kBlock - 1) / kBlock), kBlock, 0, ctx->stream>>>(
+                a.nx, a.ny, ctx->d_E, ctx->d_K0, ctx->d_fix, Kf, a.Ke64, a.Ke32);
+            k_mg_diag_full<<<gsize(a.nn), kBlock, 0, ctx->stream>>>(a.nx, a.ny, a.Ke64, a.D, a.fix);
+        }
     }
     MgLevel& C = S->L[c];
-    if (c == 0)
-        k_mg_ke0_full<<<(unsigned)((C.ne * 64 + kBlock - 1) / kBlock), kBlock, 0, ctx->stream>>>(C.ne, ctx->d_E,
-                                                                                             ctx->d_K0, C.Ke64);
+    if (c == 0) {
+        const long t = C.ne * (h8 ? 576 : 64);
+        if (h8)
+            k_mg3_ke0_full<<<(unsigned)((t + kBlock - 1) / kBlock), kBlock, 0, ctx->stream>>>(C.ne, ctx->d_E,
+                                                                                          ctx->d_K0, C.Ke64);
+        else
+            k_mg_ke0_full<<<(unsigned)((t + kBlock - 1) / kBlock), kBlock, 0, ctx->stream>>>(C.ne, ctx->d_E,
+                                                                                         ctx->d_K0, C.Ke64);
+    }
     PF_TRY(check_launch(ctx));
     // free DOFs of the coarsest level (its fixity is known only now)
     std::vector<uint8_t> hf(C.nn);
@@ -932,14 +957,18 @@ int mg_build(pf_ctx* ctx) {
     PF_TRY(sync(ctx));
     std::vector<int> dof;
     for (long n = 0; n < C.nn; n++)
-        for (int d = 0; d < 2; d++)
-            if (!((hf[n] >> d) & 1)) dof.push_back((int)(2 * n + d));
+        for (int d = 0; d < ndpn; d++)
+            if (!((hf[n] >> d) & 1)) dof.push_back((int)(ndpn * n + d));
     S->nf = (int)dof.size();
     PF_CHECK(S->nf > 0, PF_ERR_STATE, "multigrid: no free DOFs on the coarsest level");
     PF_HIP(hipMemcpyAsync(S->dof, dof.data(), sizeof(int) * dof.size(), hipMemcpyHostToDevice, ctx->stream));
     const long nf2 = (long)S->nf * S->nf;
-    k_mg_dense<<<(unsigned)((nf2 + kBlock - 1) / kBlock), kBlock, 0, ctx->stream>>>(C.nx, C.ny, C.Ke64, S->dof,
-                                                                                   S->nf, S->A);
+    if (h8)
+        k_mg3_dense<<<(unsigned)((nf2 + kBlock - 1) / kBlock), kBlock, 0, ctx->stream>>>(C.nx, C.ny, C.nz, C.Ke64,
+                                                                                        S->dof, S->nf, S->A);
+    else
+        k_mg_dense<<<(unsigned)((nf2 + kBlock - 1) / kBlock), kBlock, 0, ctx->stream>>>(C.nx, C.ny, C.Ke64,
+                                                                                       S->dof, S->nf, S->A);
     if (S->nf <= kMgLdsDense)
         k_mg_invert_lds<<<1, 1024, 0, ctx->stream>>>(S->nf, S->A);
     else
@@ -974,6 +1003,37 @@ float* mg_cycle(pf_ctx* ctx, int l) {
         k_mg_coarse_solve<<<(unsigned)((S->nf + 63) / 64), 64, 0, ctx->stream>>>(S->nf, S->A, S->dof, a.r, a.x,
                                                                                 S->done);
         return a.x;
+    }
+    if (a.nz > 0) {  // hex8: two-pass sweeps
+        const uint8_t* fx3 = l == 0 ? ctx->d_fix : a.fix;
+        float *x = a.x, *res = a.res, *xo = a.x2;
+        auto run3 = [&](auto ke) {
+            const unsigned gv = gsize(a.nv), gn = gsize(a.nn);
+            k_mg3_update<<<gv, kBlock, 0, ctx->stream>>>(a.nv, S->omega, nullptr, nullptr, a.r, a.D, fx3, x, S->done);
+            k_mg3_residual<<<gn, kBlock, 0, ctx->stream>>>(a.nx, a.ny, a.nz, ke, a.D, fx3, S->omega, a.r, x, res,
+                                                           nullptr, S->done);
+            for (int k = 1; k < S->nu; k++) {
+                k_mg3_update<<<gv, kBlock, 0, ctx->stream>>>(a.nv, S->omega, x, res, a.r, a.D, fx3, xo, S->done);
+                std::swap(x, xo);
+                k_mg3_residual<<<gn, kBlock, 0, ctx->stream>>>(a.nx, a.ny, a.nz, ke, a.D, fx3, S->omega, a.r, x, res,
+                                                               nullptr, S->done);
+            }
+            MgLevel& b = S->L[l + 1];
+            k_mg3_restrict<<<gsize(b.nn), kBlock, 0, ctx->stream>>>(b.nx, b.ny, b.nz, res, b.fix, b.r, S->done);
+            const float* xc = mg_cycle(ctx, l + 1);
+            k_mg3_prolong<<<gn, kBlock, 0, ctx->stream>>>(a.nx, a.ny, a.nz, x, xc, fx3, xo, S->done);
+            std::swap(x, xo);
+            for (int k = 0; k < S->nu; k++) {
+                k_mg3_residual<<<gn, kBlock, 0, ctx->stream>>>(a.nx, a.ny, a.nz, ke, a.D, fx3, S->omega, a.r, x,
+                                                               nullptr, xo, S->done);
+                std::swap(x, xo);
+            }
+        };
+        if (l == 0)
+            run3(Mg3KeShared{ctx->d_Ef, ctx->d_K0f});
+        else
+            run3(Mg3KeFull{a.Ke32});
+        return x;
     }
     const uint8_t* fx = l == 0 ? ctx->d_fix : a.fix;
     const unsigned g = gsize(a.nn);
@@ -1026,8 +1086,9 @@ int pcg_mg_device(pf_ctx* ctx, const double* b, double* x, double eps, int itrma
     PcgDev* S = (PcgDev*)ctx->d_pcg;
     double* partA = ctx->red.partials;
     double* partB = ctx->red.partials + 2 * kMaxPartials;
-    const K1Shape sh = k1_shape(ctx->nx, ctx->ny);
-    const unsigned g1 = sh.grid;
+    const bool h8 = !is2d(ctx);
+    const K1Shape sh = h8 ? K1Shape{0, 0, 0u} : k1_shape(ctx->nx, ctx->ny);
+    const unsigned g1 = h8 ? gsize(ctx->nn) : sh.grid;
     const unsigned g2 = std::min<unsigned>(gsize(n), (unsigned)k2_cap());
     PF_CHECK(g1 <= (unsigned)kMaxPartials, PF_ERR_ARG, "grid too large for the reduction scratch");
     double *r = ctx->d_r, *z = ctx->d_z, *p0 = ctx->d_p0, *p1 = ctx->d_p1, *q = ctx->d_q;
@@ -1053,9 +1114,17 @@ int pcg_mg_device(pf_ctx* ctx, const double* b, double* x, double eps, int itrma
         for (; k < kend; k++) {
             double* pold = (k & 1) ? p1 : p0;
             double* pnew = (k & 1) ? p0 : p1;
-            k_pcg_k1_q4_tile<double, kTileI, kTileJ><<<g1, kBlock, 0, ctx->stream>>>(
-                ctx->nx, ctx->ny, sh.tilesJ, sh.ntiles, k, pold, z, pnew, q, ctx->d_E, ctx->d_K0, ctx->d_fix, partB,
-                (int)g2, partA, S, eps);
+            if (h8) {
+                // p = beta p + z in place (head and convergence test), q = K p
+                k_pcg_k3<double><<<g2, kBlock, 0, ctx->stream>>>(n, k, p0, z, partB, (int)g2, S, eps);
+                k_pcg_k1_h8<double><<<g1, kBlock, 0, ctx->stream>>>(g3(ctx), p0, q, ctx->d_E, ctx->d_K0, ctx->d_fix,
+                                                                     partA, S);
+                pnew = p0;
+            } else {
+                k_pcg_k1_q4_tile<double, kTileI, kTileJ><<<g1, kBlock, 0, ctx->stream>>>(
+                    ctx->nx, ctx->ny, sh.tilesJ, sh.ntiles, k, pold, z, pnew, q, ctx->d_E, ctx->d_K0, ctx->d_fix,
+                    partB, (int)g2, partA, S, eps);
+            }
             k_mg_pcg_update<<<g2, kBlock, 0, ctx->stream>>>(n, k, x, r, pnew, q, r32, partA, (int)g1, S);
             k_mg_pcg_z<<<g2, kBlock, 0, ctx->stream>>>(n, mg_cycle(ctx, 0), r, z, partB, &S->done);
         }
@@ -2314,9 +2383,11 @@ int pf_mg_setup(pf_ctx* ctx, int nu, double omega) {
         mg_free(ctx);
         return PF_OK;
     }
-    PF_CHECK(is2d(ctx), PF_ERR_UNSUPPORTED, "pf_mg_setup: quad4 grids only");
     PF_CHECK(nu >= 1 && nu <= 8, PF_ERR_ARG, "pf_mg_setup: nu must be 1..8 (0 to switch off)");
-    PF_CHECK(omega > 0.0 && omega <= 1.0, PF_ERR_ARG, "pf_mg_setup: omega must be in (0, 1]");
+    // damped Jacobi smooths only for omega < 2 / lambda_max(D^-1 K): about 0.67
+    // for quad4 and 0.48 for hex8 (lambda_max about 3 and 4.2)
+    if (omega <= 0.0) omega = is2d(ctx) ? 0.6 : 0.4;
+    PF_CHECK(omega <= 1.0, PF_ERR_ARG, "pf_mg_setup: omega must be at most 1 (or <= 0 for the default)");
     const int rc = mg_plan(ctx, nu, (float)omega);
     if (rc < 0) mg_free(ctx);
     return rc;
@@ -2342,7 +2413,8 @@ int pf_mg_apply(pf_ctx* ctx, const double* r_free, double* z_free) {
     return sync(ctx);
 }
 
-int pf_mg_level(pf_ctx* ctx, int level, int* levels, int* nx, int* ny, double* Ke, float* D, uint8_t* fix) {
+int pf_mg_level(pf_ctx* ctx, int level, int* levels, int* nx, int* ny, int* nz, double* Ke, float* D,
+                uint8_t* fix) {
     if (!ctx) return PF_ERR_ARG;
     PF_CHECK(ctx->mg, PF_ERR_STATE, "multigrid not set up: call pf_mg_setup first");
     PF_TRY(set_device(ctx));
@@ -2352,9 +2424,11 @@ int pf_mg_level(pf_ctx* ctx, int level, int* levels, int* nx, int* ny, double* K
     const MgLevel& a = S->L[level];
     if (nx) *nx = a.nx;
     if (ny) *ny = a.ny;
+    if (nz) *nz = a.nz;
     if (Ke) {
         PF_CHECK(level >= 1, PF_ERR_ARG, "pf_mg_level: level 0's element matrices are E_e K0");
-        PF_HIP(hipMemcpyAsync(Ke, a.Ke64, sizeof(double) * a.ne * 64, hipMemcpyDeviceToHost, ctx->stream));
+        const long kes = a.nz > 0 ? 576 : 64;
+        PF_HIP(hipMemcpyAsync(Ke, a.Ke64, sizeof(double) * a.ne * kes, hipMemcpyDeviceToHost, ctx->stream));
     }
     if (D) PF_HIP(hipMemcpyAsync(D, a.D, sizeof(float) * a.nv, hipMemcpyDeviceToHost, ctx->stream));
     if (fix)

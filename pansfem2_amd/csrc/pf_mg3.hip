@@ -1,0 +1,330 @@
+// Geometric multigrid V-cycle for the hex8 matrix-free operator (the 3D
+// counterpart of pf_mg.hip; no counterpart in the reference).
+//
+// Level l+1 halves the three element counts; a coarse element's 24 x 24
+// matrix is the Galerkin product sum_a P_a^T K_a P_a of its eight children's
+// (trilinear P_a, the same weights for the three DOF directions; fixed DOFs
+// removed at level 0 by zeroing their rows and columns).  Nodes (i, j, k) ->
+// ((ny+1)(nz+1)) i + (nz+1) j + k; element-local node q at offsets
+// (0,0,0), (1,0,0), (1,1,0), (0,1,0), then the same four at dk = 1
+// (ShapeFunction8Cubic order).  Vectors are nodal, 3 entries per node.
+// Smoothing is damped Jacobi in two passes per sweep (an elementwise update,
+// then an operator application that gathers the updated vector), since a
+// 3D node's operator row touches 27 nodes through 8 elements.
+#pragma once
+
+#include "pf_device.h"
+
+namespace pf {
+
+__device__ inline void mg3_lxyz(int q, int& x, int& y, int& z) {
+    z = q >> 2;
+    const int m = q & 3;
+    x = (m == 1 || m == 2) ? 1 : 0;
+    y = (m >= 2) ? 1 : 0;
+}
+
+// element-matrix sources (FP32): level 0 E_e K0, coarse levels one 24 x 24
+// matrix per element
+struct Mg3KeShared {
+    const float* E;
+    const float* K0;
+    __device__ float operator()(long e, int r, int c) const { return E[e] * K0[r * 24 + c]; }
+};
+struct Mg3KeFull {
+    const float* Ke;
+    __device__ float operator()(long e, int r, int c) const { return Ke[e * 576 + r * 24 + c]; }
+};
+
+// (A x) at node (i, j, k) over its (up to) eight elements
+template <class KE>
+__device__ inline void mg3_node_apply(int nx, int ny, int nz, int i, int j, int k, const KE& ke,
+                                      const float* __restrict__ x, float (&ax)[3]) {
+#pragma clang fp contract(fast)  // preconditioner arithmetic: FMA allowed
+    ax[0] = ax[1] = ax[2] = 0.f;
+    const long pl = (long)(ny + 1) * (nz + 1);
+    for (int ea = 0; ea < 2; ea++) {
+        const int ei = i - 1 + ea;
+        if (ei < 0 || ei >= nx) continue;
+        for (int eb = 0; eb < 2; eb++) {
+            const int ej = j - 1 + eb;
+            if (ej < 0 || ej >= ny) continue;
+            for (int ec = 0; ec < 2; ec++) {
+                const int ek = k - 1 + ec;
+                if (ek < 0 || ek >= nz) continue;
+                const long e = ((long)ny * nz) * ei + (long)nz * ej + ek;
+                const int di = i - ei, dj = j - ej, dk = k - ek;
+                const int me = 4 * dk + (dj == 0 ? di : 3 - di);
+#pragma unroll
+                for (int q = 0; q < 8; q++) {
+                    int qx, qy, qz;
+                    mg3_lxyz(q, qx, qy, qz);
+                    const long m = pl * (ei + qx) + (long)(nz + 1) * (ej + qy) + (ek + qz);
+                    const float u0 = x[3 * m], u1 = x[3 * m + 1], u2 = x[3 * m + 2];
+#pragma unroll
+                    for (int r = 0; r < 3; r++)
+                        ax[r] += ke(e, 3 * me + r, 3 * q) * u0 + ke(e, 3 * me + r, 3 * q + 1) * u1 +
+                                 ke(e, 3 * me + r, 3 * q + 2) * u2;
+                }
+            }
+        }
+    }
+}
+
+// out = x + omega D^-1 res (x null: 0; res null: r), masked
+__global__ void __launch_bounds__(256) k_mg3_update(long nv, float omega, const float* __restrict__ x,
+                                                    const float* __restrict__ res, const float* __restrict__ r,
+                                                    const float* __restrict__ D, const uint8_t* __restrict__ fix,
+                                                    float* __restrict__ out, const int* __restrict__ done) {
+    if (done && *done) return;
+    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < nv; t += (long)gridDim.x * blockDim.x) {
+        const long n = t / 3;
+        const int d = (int)(t - 3 * n);
+        const float rs = res ? res[t] : r[t];
+        float v = (x ? x[t] : 0.f) + omega * rs / D[t];
+        if ((fix[n] >> d) & 1) v = 0.f;
+        out[t] = v;
+    }
+}
+
+// res = r - A x (masked); with jac, x_out = x + omega D^-1 res instead
+template <class KE>
+__global__ void __launch_bounds__(256) k_mg3_residual(int nx, int ny, int nz, KE ke, const float* __restrict__ D,
+                                                      const uint8_t* __restrict__ fix, float omega,
+                                                      const float* __restrict__ r, const float* __restrict__ x,
+                                                      float* __restrict__ res, float* __restrict__ x_out,
+                                                      const int* __restrict__ done) {
+#pragma clang fp contract(fast)
+    if (done && *done) return;
+    const long nn = (long)(nx + 1) * (ny + 1) * (nz + 1);
+    const long pl = (long)(ny + 1) * (nz + 1);
+    for (long n = blockIdx.x * (long)blockDim.x + threadIdx.x; n < nn; n += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(n / pl);
+        const long rem = n - (long)i * pl;
+        const int j = (int)(rem / (nz + 1)), k = (int)(rem - (long)j * (nz + 1));
+        float ax[3];
+        mg3_node_apply(nx, ny, nz, i, j, k, ke, x, ax);
+        const uint8_t f = fix[n];
+        for (int d = 0; d < 3; d++) {
+            float v = r[3 * n + d] - ax[d];
+            if (x_out) v = x[3 * n + d] + omega * v / D[3 * n + d];
+            if ((f >> d) & 1) v = 0.f;
+            if (x_out)
+                x_out[3 * n + d] = v;
+            else
+                res[3 * n + d] = v;
+        }
+    }
+}
+
+// r_c = P^T res: coarse node (I, J, K) gathers the 27 fine nodes
+// (2I + a, 2J + b, 2K + c), weights (1 - |a|/2)(1 - |b|/2)(1 - |c|/2)
+__global__ void __launch_bounds__(256) k_mg3_restrict(int cnx, int cny, int cnz, const float* __restrict__ res,
+                                                      const uint8_t* __restrict__ cfix, float* __restrict__ rc,
+                                                      const int* __restrict__ done) {
+#pragma clang fp contract(fast)
+    if (done && *done) return;
+    const int nx = 2 * cnx, ny = 2 * cny, nz = 2 * cnz;
+    const long cnn = (long)(cnx + 1) * (cny + 1) * (cnz + 1);
+    const long cpl = (long)(cny + 1) * (cnz + 1), pl = (long)(ny + 1) * (nz + 1);
+    for (long n = blockIdx.x * (long)blockDim.x + threadIdx.x; n < cnn; n += (long)gridDim.x * blockDim.x) {
+        const int I = (int)(n / cpl);
+        const long rem = n - (long)I * cpl;
+        const int J = (int)(rem / (cnz + 1)), K = (int)(rem - (long)J * (cnz + 1));
+        float s[3] = {0.f, 0.f, 0.f};
+        for (int a = -1; a <= 1; a++)
+            for (int b = -1; b <= 1; b++)
+                for (int c = -1; c <= 1; c++) {
+                    const int i = 2 * I + a, j = 2 * J + b, k = 2 * K + c;
+                    if (i < 0 || i > nx || j < 0 || j > ny || k < 0 || k > nz) continue;
+                    const float w = (a ? 0.5f : 1.f) * (b ? 0.5f : 1.f) * (c ? 0.5f : 1.f);
+                    const long m = pl * i + (long)(nz + 1) * j + k;
+                    s[0] += w * res[3 * m];
+                    s[1] += w * res[3 * m + 1];
+                    s[2] += w * res[3 * m + 2];
+                }
+        const uint8_t f = cfix[n];
+        for (int d = 0; d < 3; d++) rc[3 * n + d] = ((f >> d) & 1) ? 0.f : s[d];
+    }
+}
+
+// out = x + P xc (trilinear prolongation of the coarse correction), masked
+__global__ void __launch_bounds__(256) k_mg3_prolong(int nx, int ny, int nz, const float* __restrict__ x,
+                                                     const float* __restrict__ xc, const uint8_t* __restrict__ fix,
+                                                     float* __restrict__ out, const int* __restrict__ done) {
+#pragma clang fp contract(fast)
+    if (done && *done) return;
+    const long nn = (long)(nx + 1) * (ny + 1) * (nz + 1);
+    const long pl = (long)(ny + 1) * (nz + 1);
+    const int cny = ny / 2, cnz = nz / 2;
+    const long cpl = (long)(cny + 1) * (cnz + 1);
+    for (long n = blockIdx.x * (long)blockDim.x + threadIdx.x; n < nn; n += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(n / pl);
+        const long rem = n - (long)i * pl;
+        const int j = (int)(rem / (nz + 1)), k = (int)(rem - (long)j * (nz + 1));
+        const int I0 = i >> 1, J0 = j >> 1, K0c = k >> 1, oi = i & 1, oj = j & 1, ok = k & 1;
+        float s[3] = {0.f, 0.f, 0.f};
+        for (int a = 0; a <= oi; a++)
+            for (int b = 0; b <= oj; b++)
+                for (int c = 0; c <= ok; c++) {
+                    const float w = (oi ? 0.5f : 1.f) * (oj ? 0.5f : 1.f) * (ok ? 0.5f : 1.f);
+                    const long m = cpl * (I0 + a) + (long)(cnz + 1) * (J0 + b) + (K0c + c);
+                    s[0] += w * xc[3 * m];
+                    s[1] += w * xc[3 * m + 1];
+                    s[2] += w * xc[3 * m + 2];
+                }
+        const uint8_t f = fix[n];
+        for (int d = 0; d < 3; d++) out[3 * n + d] = ((f >> d) & 1) ? 0.f : x[3 * n + d] + s[d];
+    }
+}
+
+// Galerkin coarse element matrices, one thread per (coarse element, row):
+// row 3q + d of Kc is sum_a sum_k w_a[k][q] (row 3k + d of K_a) P_a.
+// Children: level 0 (E, K0, fixity of the fine nodes) when Kf is null.
+__global__ void __launch_bounds__(256) k_mg3_coarsen(int cnx, int cny, int cnz, const double* __restrict__ E,
+                                                     const double* __restrict__ K0, const uint8_t* __restrict__ fix,
+                                                     const double* __restrict__ Kf, double* __restrict__ Kc,
+                                                     float* __restrict__ Kc32) {
+    const long nce = (long)cnx * cny * cnz;
+    const int ny = 2 * cny, nz = 2 * cnz;
+    const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (t >= nce * 24) return;
+    const long ce = t / 24;
+    const int row = (int)(t - ce * 24), q = row / 3, d = row - 3 * (row / 3);
+    const long cpl = (long)cny * cnz;
+    const int CI = (int)(ce / cpl);
+    const long rem = ce - (long)CI * cpl;
+    const int CJ = (int)(rem / cnz), CK = (int)(rem - (long)CJ * cnz);
+    double out[24];
+    for (int c = 0; c < 24; c++) out[c] = 0.0;
+    const long pl = (long)(ny + 1) * (nz + 1);
+    for (int ch = 0; ch < 8; ch++) {
+        int a0, a1, a2;
+        mg3_lxyz(ch, a0, a1, a2);
+        const int fi = 2 * CI + a0, fj = 2 * CJ + a1, fk = 2 * CK + a2;
+        const long fe = ((long)ny * nz) * fi + (long)nz * fj + fk;
+        // w[k][q']: weight of coarse node q' at child node k
+        double w[8][8];
+        uint8_t fx[8];
+        for (int k = 0; k < 8; k++) {
+            int lx, ly, lz;
+            mg3_lxyz(k, lx, ly, lz);
+            const double x = 0.5 * (a0 + lx), y = 0.5 * (a1 + ly), z = 0.5 * (a2 + lz);
+            for (int q2 = 0; q2 < 8; q2++) {
+                int qx, qy, qz;
+                mg3_lxyz(q2, qx, qy, qz);
+                w[k][q2] = (qx ? x : 1 - x) * (qy ? y : 1 - y) * (qz ? z : 1 - z);
+            }
+            fx[k] = Kf ? 0 : fix[pl * (fi + lx) + (long)(nz + 1) * (fj + ly) + (fk + lz)];
+        }
+        const double e = Kf ? 0.0 : E[fe];
+        for (int k = 0; k < 8; k++) {
+            if (w[k][q] == 0.0 || ((fx[k] >> d) & 1)) continue;
+            const int rr = 3 * k + d;
+            for (int k2 = 0; k2 < 8; k2++)
+                for (int d2 = 0; d2 < 3; d2++) {
+                    if ((fx[k2] >> d2) & 1) continue;
+                    const int cc = 3 * k2 + d2;
+                    const double kv = Kf ? Kf[fe * 576 + rr * 24 + cc] : e * K0[rr * 24 + cc];
+                    const double wk = w[k][q] * kv;
+                    for (int q2 = 0; q2 < 8; q2++) out[3 * q2 + d2] += wk * w[k2][q2];
+                }
+        }
+    }
+    for (int c = 0; c < 24; c++) {
+        Kc[ce * 576 + row * 24 + c] = out[c];
+        Kc32[ce * 576 + row * 24 + c] = (float)out[c];
+    }
+}
+
+// diagonal and fixity of a coarse level (a DOF without stiffness is fixed)
+__global__ void __launch_bounds__(256) k_mg3_diag_full(int nx, int ny, int nz, const double* __restrict__ Ke,
+                                                       float* __restrict__ D, uint8_t* __restrict__ fix) {
+    const long nn = (long)(nx + 1) * (ny + 1) * (nz + 1);
+    const long pl = (long)(ny + 1) * (nz + 1);
+    for (long n = blockIdx.x * (long)blockDim.x + threadIdx.x; n < nn; n += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(n / pl);
+        const long rem = n - (long)i * pl;
+        const int j = (int)(rem / (nz + 1)), k = (int)(rem - (long)j * (nz + 1));
+        double dg[3] = {0.0, 0.0, 0.0};
+        for (int ea = 0; ea < 2; ea++)
+            for (int eb = 0; eb < 2; eb++)
+                for (int ec = 0; ec < 2; ec++) {
+                    const int ei = i - 1 + ea, ej = j - 1 + eb, ek = k - 1 + ec;
+                    if (ei < 0 || ei >= nx || ej < 0 || ej >= ny || ek < 0 || ek >= nz) continue;
+                    const long e = ((long)ny * nz) * ei + (long)nz * ej + ek;
+                    const int di = i - ei, dj = j - ej, dk = k - ek;
+                    const int me = 4 * dk + (dj == 0 ? di : 3 - di);
+                    for (int d = 0; d < 3; d++) dg[d] += Ke[e * 576 + (3 * me + d) * 24 + 3 * me + d];
+                }
+        uint8_t f = 0;
+        for (int d = 0; d < 3; d++) {
+            if (!(dg[d] > 0.0)) f |= (uint8_t)(1u << d);
+            D[3 * n + d] = ((f >> d) & 1) ? 1.f : (float)dg[d];
+        }
+        fix[n] = f;
+    }
+}
+
+// level-0 diagonal in FP32 from E and K0 (fixed DOFs: 1)
+__global__ void __launch_bounds__(256) k_mg3_diag0(int nx, int ny, int nz, const double* __restrict__ E,
+                                                   const double* __restrict__ K0, const uint8_t* __restrict__ fix,
+                                                   float* __restrict__ D) {
+    const long nn = (long)(nx + 1) * (ny + 1) * (nz + 1);
+    const long pl = (long)(ny + 1) * (nz + 1);
+    for (long n = blockIdx.x * (long)blockDim.x + threadIdx.x; n < nn; n += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(n / pl);
+        const long rem = n - (long)i * pl;
+        const int j = (int)(rem / (nz + 1)), k = (int)(rem - (long)j * (nz + 1));
+        double dg[3] = {0.0, 0.0, 0.0};
+        for (int ea = 0; ea < 2; ea++)
+            for (int eb = 0; eb < 2; eb++)
+                for (int ec = 0; ec < 2; ec++) {
+                    const int ei = i - 1 + ea, ej = j - 1 + eb, ek = k - 1 + ec;
+                    if (ei < 0 || ei >= nx || ej < 0 || ej >= ny || ek < 0 || ek >= nz) continue;
+                    const double ee = E[((long)ny * nz) * ei + (long)nz * ej + ek];
+                    const int di = i - ei, dj = j - ej, dk = k - ek;
+                    const int me = 4 * dk + (dj == 0 ? di : 3 - di);
+                    for (int d = 0; d < 3; d++) dg[d] += ee * K0[(3 * me + d) * 24 + 3 * me + d];
+                }
+        const uint8_t f = fix[n];
+        for (int d = 0; d < 3; d++) D[3 * n + d] = ((f >> d) & 1) ? 1.f : (float)dg[d];
+    }
+}
+
+// dense free-DOF block of the coarsest level (3 DOFs per node)
+__global__ void __launch_bounds__(256) k_mg3_dense(int nx, int ny, int nz, const double* __restrict__ Ke,
+                                                   const int* __restrict__ dof, int nf, double* __restrict__ A) {
+    const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (t >= (long)nf * nf) return;
+    const int a = (int)(t / nf), b = (int)(t - (long)a * nf);
+    const int na = dof[a] / 3, da = dof[a] - 3 * na, nb = dof[b] / 3, db = dof[b] - 3 * nb;
+    const int pl = (ny + 1) * (nz + 1);
+    const int ia = na / pl, ja = (na - ia * pl) / (nz + 1), ka = na - ia * pl - ja * (nz + 1);
+    const int ib = nb / pl, jb = (nb - ib * pl) / (nz + 1), kb = nb - ib * pl - jb * (nz + 1);
+    double s = 0.0;
+    for (int ei = ia - 1; ei <= ia; ei++)
+        for (int ej = ja - 1; ej <= ja; ej++)
+            for (int ek = ka - 1; ek <= ka; ek++) {
+                if (ei < 0 || ei >= nx || ej < 0 || ej >= ny || ek < 0 || ek >= nz) continue;
+                int qa = -1, qb = -1;
+                for (int q = 0; q < 8; q++) {
+                    int qx, qy, qz;
+                    mg3_lxyz(q, qx, qy, qz);
+                    if (ei + qx == ia && ej + qy == ja && ek + qz == ka) qa = q;
+                    if (ei + qx == ib && ej + qy == jb && ek + qz == kb) qb = q;
+                }
+                if (qa < 0 || qb < 0) continue;
+                s += Ke[(((long)ny * nz) * ei + (long)nz * ej + ek) * 576 + (3 * qa + da) * 24 + 3 * qb + db];
+            }
+    A[t] = s;
+}
+
+// level-0 element matrices in FP64 (E_e K0), for a grid that is its own coarsest level
+__global__ void __launch_bounds__(256) k_mg3_ke0_full(long ne, const double* __restrict__ E,
+                                                      const double* __restrict__ K0, double* __restrict__ Ke) {
+    const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (t < ne * 576) Ke[t] = E[t / 576] * K0[t % 576];
+}
+
+}  // namespace pf

@@ -119,3 +119,20 @@ def test_hex8_simp_loop_against_oracle(pa):
     assert np.max(np.abs(r["f"] - f_ref) / f_ref) < 1e-9
     assert np.max(np.abs(r["rho"] - ref["rho"])) < 1e-6
     assert abs(int(r["cg_iters"][0]) - int(ref["hist"][0][2])) <= 2
+
+
+def test_hex8_simp_loop_multigrid_against_oracle(pa):
+    """The same 5 OC iterations with every solve multigrid-preconditioned
+    (PF_OP_MG, to the true residual 1e-10): the same compliance history and
+    densities as the oracle's Jacobi-PCG loop."""
+    from oracle import pyoracle as po
+    nx, ny, nz = 8, 4, 4
+    nodes, dofs, vals = pa.line_load_nodes3d(nx, ny, nz)
+    ref = po.simp3d(nx, ny, nz, 5, list(zip(nodes.tolist(), dofs.tolist(), vals.tolist())), vf=0.3)
+    ctx = pa.cantilever3d(nx, ny, nz)
+    ctx.mg_setup(1)
+    r = ctx.simp_run(5, pa.simp_default_params(vf=0.3, solver=pa.PF_OP_MG), s0=0.3)
+    f_ref = np.array([h[0] for h in ref["hist"]])
+    assert np.max(np.abs(r["f"] - f_ref) / f_ref) < 1e-9
+    assert np.max(np.abs(r["rho"] - ref["rho"])) < 1e-6
+    assert np.max(r["cg_iters"]) < 0.5 * int(ref["hist"][0][2])

@@ -69,3 +69,31 @@ def test_mg_pcg_converges_where_jacobi_crawls():
     xm, km = mg.pcg(lambda v: mg.apply_level(l0, v), b, lambda r: mg.vcycle(lv, r))
     assert kj > 500 and km < 100, (kj, km)
     assert np.linalg.norm(xm - xj) / np.linalg.norm(xj) < 1e-6
+
+
+def unit_k0_h8():
+    xy, conn = po.grid3d(1, 1, 1)
+    return po.ke_h8(xy[conn[0]], 1.0, 0.3)
+
+
+def test_galerkin_identity_hex8():
+    nx, ny, nz = 4, 2, 2
+    rng = np.random.default_rng(21)
+    E = 1e-9 + rng.uniform(0, 1, nx * ny * nz) ** 3
+    K0 = unit_k0_h8()
+    fix = np.zeros((nx + 1) * (ny + 1) * (nz + 1), np.uint8)
+    fix[: (ny + 1) * (nz + 1)] = 7  # x = 0 face
+    lv = mg.build3(nx, ny, nz, E, K0, fix, coarse_dofs=10)
+    assert len(lv) == 2
+    A_raw = mg.assemble3(nx, ny, nz, E[:, None, None] * K0[None])
+    M = sp.diags((~mg.fix_dofs3(fix)).astype(float))
+    P = mg.prolongation3(nx, ny, nz)
+    want = (P.T @ M @ A_raw @ M @ P).toarray()
+    got = lv[1]["A"].toarray()
+    assert np.max(np.abs(got - want)) <= 1e-13 * np.max(np.abs(want))
+    # and the V-cycle over it is symmetric
+    free = lv[0]["free"]
+    a = rng.standard_normal(free.size) * free
+    b = rng.standard_normal(free.size) * free
+    Va, Vb = mg.vcycle(lv, a), mg.vcycle(lv, b)
+    assert abs(Va @ b - a @ Vb) <= 1e-12 * np.linalg.norm(Va) * np.linalg.norm(b)
