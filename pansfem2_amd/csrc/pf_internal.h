@@ -24,23 +24,10 @@ namespace pf {
 constexpr int kBlock = 256;        // threads per workgroup for the 1D kernels
 constexpr int kMaxPartials = 8192;  // upper bound on workgroups of a reducing launch
 
-// Scalars of one Jacobi-PCG solve, resident on the device.  The kernels of
-// iteration k read what the previous kernel's last workgroup wrote.
-struct PcgScalars {
-    double zr;      // z.r of the current iterate (CG.h:428,440)
-    double pq;      // p.Ap
-    double alpha, beta;
-    double rr;      // r.r
-    double bnorm, eps;
-    int k, itrmax, done, status;  // status: 0 running, 1 converged, 2 itrmax reached, 3 breakdown
-};
-
-// OC bisection state (OC.h:80-99), resident on the device.
-struct OcScalars {
-    double l0, l1, lambda, leps;
-    double g;        // constraint value at the last tested lambda
-    int steps, done, max_steps, pad;
-};
+// Device-resident scalars of a PCG solve (defined with the kernels that
+// write them, pf_kernels.hip) and of the OC bisection.
+struct PcgDev;
+struct OcDev;
 
 // Reduction scratch: per-workgroup partials and the arrival counter of the
 // "last workgroup reduces" pattern.  One set per context; launches that reduce
@@ -119,11 +106,11 @@ struct pf_ctx {
     double* d_fw = nullptr;
 
     // device scalars
-    pf::PcgScalars* d_pcg = nullptr;
-    pf::OcScalars* d_oc = nullptr;
+    pf::PcgDev* d_pcg = nullptr;
+    pf::OcDev* d_oc = nullptr;
     pf::ReduceScratch red{};
-    pf::PcgScalars* h_pcg = nullptr;  // pinned mirror
-    pf::OcScalars* h_oc = nullptr;    // pinned mirror
+    pf::PcgDev* h_pcg = nullptr;  // pinned mirror
+    pf::OcDev* h_oc = nullptr;    // pinned mirror
     double* h_red = nullptr;          // pinned mirror of red.out
 
     // MMA state
@@ -159,31 +146,6 @@ struct pf_ctx {
         if (rc_ < 0) return rc_;                                                              \
     } while (0)
 
-// ------------------------------------------------- launchers (per .hip) ----
 namespace pf {
 inline unsigned nblocks(long n, int bs = kBlock) { return (unsigned)((n + bs - 1) / bs); }
-
-// pf_kernels.hip
-int launch_E_from_rho(pf_ctx* ctx);
-int launch_diag(pf_ctx* ctx, bool f32);
-int launch_apply(pf_ctx* ctx, bool f32, const void* x, void* y, bool mask_fixed);
-int launch_pcg_init(pf_ctx* ctx, bool f32, const void* b, double eps, int itrmax, bool csr);
-int launch_pcg_iter(pf_ctx* ctx, bool f32, int parity, bool csr);
-int launch_free_to_nodal(pf_ctx* ctx, const double* xf, double* xn);
-int launch_nodal_to_free(pf_ctx* ctx, const double* xn, double* xf);
-int launch_filter(pf_ctx* ctx, int mode, const double* in, double* out);
-int launch_volume(pf_ctx* ctx, const double* rho, double vf, double scale1, double* g_dev);
-int launch_comp_sens(pf_ctx* ctx, const double* u, double scale0, double* dfdrho, double* f_dev);
-int launch_oc_init(pf_ctx* ctx, const pf_oc_params* prm, double* xplus);
-int launch_oc_step(pf_ctx* ctx, const pf_oc_params* prm, const double* x, const double* dfdx, const double* dgdx,
-                   double* xplus);
-int launch_fill(pf_ctx* ctx, double* p, long n, double v);
-int launch_ke_all(pf_ctx* ctx, const double* E);
-int launch_csr_count(pf_ctx* ctx, int32_t* rowlen);
-int launch_csr_fill(pf_ctx* ctx);
-int launch_csr_spmv(pf_ctx* ctx, const double* xf, double* yf);
-int launch_csr_diag(pf_ctx* ctx, double* Df);
-int launch_scan(pf_ctx* ctx, const int32_t* rowlen, int32_t* indptr, int n);
-int launch_residual(pf_ctx* ctx, const double* b, const double* x, double* r, double* rnorm2_dev);
-int launch_dot(pf_ctx* ctx, const double* a, const double* b, long n, double* out_dev);
 }  // namespace pf
