@@ -557,6 +557,11 @@ struct MgLevel {
 };
 struct MgState {
     const int* done = nullptr;  // while set, the V-cycle's kernels return at once when *done
+    // while set (quad4, nu = 1), the finest level's last sweep also writes z in
+    // FP64 with the sums z.r, r.r (r64) into zpart, one per tile
+    const double* r64 = nullptr;
+    double* z64 = nullptr;
+    double* zpart = nullptr;
     int tail = -1;              // first level of the one-workgroup tail (k_mg_tail), -1: none
     int nu = 1;
     float omega = 0.6f;
@@ -1041,8 +1046,10 @@ float* mg_cycle(pf_ctx* ctx, int l) {
     auto sweep = [&](auto ke, const float* xi, const float* ri, const float* xc, float* xout, float* rout) {
         if (a.nn >= kMgTileNodes) {  // large levels: LDS-tiled
             const int tilesJ = (a.ny + 1 + 63) / 64, tiles = tilesJ * ((a.nx + 1 + 7) / 8);
+            const bool zf = l == 0 && xc && S->z64 && S->nu == 1;  // the cycle's last sweep
             k_mg_sweep_tile<decltype(ke), 8, 64><<<tiles, kBlock, 0, ctx->stream>>>(
-                a.nx, a.ny, tilesJ, ke, a.D, fx, S->omega, a.r, xi, ri, xc, xout, rout, S->done);
+                a.nx, a.ny, tilesJ, ke, a.D, fx, S->omega, a.r, xi, ri, xc, xout, rout, S->done,
+                zf ? S->r64 : nullptr, zf ? S->z64 : nullptr, zf ? S->zpart : nullptr);
         } else {
             k_mg_sweep<<<g, kBlock, 0, ctx->stream>>>(a.nx, a.ny, ke, a.D, fx, S->omega, a.r, xi, ri, xc, xout,
                                                       rout, S->done);
@@ -1093,11 +1100,29 @@ int pcg_mg_device(pf_ctx* ctx, const double* b, double* x, double eps, int itrma
     PF_CHECK(g1 <= (unsigned)kMaxPartials, PF_ERR_ARG, "grid too large for the reduction scratch");
     double *r = ctx->d_r, *z = ctx->d_z, *p0 = ctx->d_p0, *p1 = ctx->d_p1, *q = ctx->d_q;
     float* r32 = M->L[0].r;
+    // the finest level's last V-cycle sweep writes z and its sums itself when
+    // it is the LDS-tiled quad4 sweep (one sweep per side), else k_mg_pcg_z does
+    const bool zfused = !h8 && M->nu == 1 && M->L[0].nn >= kMgTileNodes && (int)M->L.size() > 1;
+    const unsigned gz = zfused ? (unsigned)(((ctx->ny + 1 + 63) / 64) * ((ctx->nx + 1 + 7) / 8)) : g2;
+    PF_CHECK(gz <= (unsigned)kMaxPartials, PF_ERR_ARG, "grid too large for the reduction scratch");
+    auto cycle_z = [&](const int* done_flag) {
+        if (zfused) {
+            M->r64 = r;
+            M->z64 = z;
+            M->zpart = partB;
+            mg_cycle(ctx, 0);
+            M->r64 = nullptr;
+            M->z64 = nullptr;
+            M->zpart = nullptr;
+        } else {
+            k_mg_pcg_z<<<g2, kBlock, 0, ctx->stream>>>(n, mg_cycle(ctx, 0), r, z, partB, done_flag);
+        }
+    };
     // x = 0, r = b (k_pcg_init's z = b ./ D and its sums are replaced by V(b)'s)
     k_pcg_init<double><<<g2, kBlock, 0, ctx->stream>>>(n, b, ctx->d_D, x, r, z, p0, partB, S);
     k_d2f<<<gsize(n), kBlock, 0, ctx->stream>>>(n, b, r32);
     M->done = &S->done;
-    k_mg_pcg_z<<<g2, kBlock, 0, ctx->stream>>>(n, mg_cycle(ctx, 0), r, z, partB, nullptr);
+    cycle_z(nullptr);
     PF_TRY(check_launch(ctx));
     const int chunk = 8;
     int k = 0;
@@ -1123,10 +1148,10 @@ int pcg_mg_device(pf_ctx* ctx, const double* b, double* x, double eps, int itrma
             } else {
                 k_pcg_k1_q4_tile<double, kTileI, kTileJ><<<g1, kBlock, 0, ctx->stream>>>(
                     ctx->nx, ctx->ny, sh.tilesJ, sh.ntiles, k, pold, z, pnew, q, ctx->d_E, ctx->d_K0, ctx->d_fix,
-                    partB, (int)g2, partA, S, eps);
+                    partB, (int)gz, partA, S, eps);
             }
             k_mg_pcg_update<<<g2, kBlock, 0, ctx->stream>>>(n, k, x, r, pnew, q, r32, partA, (int)g1, S);
-            k_mg_pcg_z<<<g2, kBlock, 0, ctx->stream>>>(n, mg_cycle(ctx, 0), r, z, partB, &S->done);
+            cycle_z(&S->done);
         }
         if ((rc = check_launch(ctx)) < 0) break;
         if (hipMemcpyAsync((void*)&hflag[slot], &S->done, sizeof(int), hipMemcpyDeviceToHost, ctx->stream) !=
@@ -1147,7 +1172,7 @@ int pcg_mg_device(pf_ctx* ctx, const double* b, double* x, double eps, int itrma
     hipEventDestroy(ev[0]);
     hipEventDestroy(ev[1]);
     PF_TRY(rc);
-    k_pcg_final<<<1, kBlock, 0, ctx->stream>>>(k, partB, (int)g2, S, eps);
+    k_pcg_final<<<1, kBlock, 0, ctx->stream>>>(k, partB, (int)gz, S, eps);
     PF_TRY(check_launch(ctx));
     PF_HIP(hipMemcpyAsync(ctx->h_pcg, S, sizeof(PcgDev), hipMemcpyDeviceToHost, ctx->stream));
     PF_TRY(sync(ctx));

@@ -165,11 +165,17 @@ __global__ void __launch_bounds__(256) k_mg_sweep_tile(int nx, int ny, int tiles
                                                        const float* __restrict__ r, const float* __restrict__ x,
                                                        const float* __restrict__ res, const float* __restrict__ xc,
                                                        float* __restrict__ x_out, float* __restrict__ res_out,
-                                                       const int* __restrict__ done) {
+                                                       const int* __restrict__ done, const double* __restrict__ r64,
+                                                       double* __restrict__ z64, double* __restrict__ zpart) {
 #pragma clang fp contract(fast)
+    // z64 (post-smoothing form only): the V-cycle's result in FP64 as well,
+    // with the sums z.r64, r64.r64 per workgroup into zpart (the finest
+    // level's last sweep inside the multigrid PCG)
     if (done && *done) return;
     constexpr int HALO = (TI + 2) * (TJ + 2), PER = (HALO + 255) / 256;
     __shared__ Vec2<float> sx[TI + 2][TJ + 2];
+    __shared__ double lds[32];
+    double acc[2] = {0.0, 0.0};
     const int t = blockIdx.x;
     const int j0 = (t % tilesJ) * TJ, i0 = (t / tilesJ) * TI;
     const int cny = ny / 2;
@@ -228,12 +234,22 @@ __global__ void __launch_bounds__(256) k_mg_sweep_tile(int nx, int ny, int tiles
         if (xc) {
             const Vec2<float> d = load2(D, n);
             Vec2<float> xo{xv.x + omega * (rn.x - ax) / d.x, xv.y + omega * (rn.y - ay) / d.y};
-            store2(x_out, n, mg_masked(xo, f));
+            xo = mg_masked(xo, f);
+            store2(x_out, n, xo);
+            if (z64) {
+                const double zx = (double)xo.x, zy = (double)xo.y;
+                const double rx = r64[2 * n], ry = r64[2 * n + 1];
+                z64[2 * n] = zx;
+                z64[2 * n + 1] = zy;
+                acc[0] += zx * rx + zy * ry;
+                acc[1] += rx * rx + ry * ry;
+            }
         } else {
             store2(x_out, n, xv);
             if (res_out) store2(res_out, n, mg_masked(Vec2<float>{rn.x - ax, rn.y - ay}, f));
         }
     }
+    if (z64) store_partials<2>(acc, lds, zpart);
 }
 
 // plain Jacobi sweep x' = x + omega D^-1 (r - A x) at node n
