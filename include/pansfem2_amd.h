@@ -179,13 +179,27 @@ int pf_pcg(pf_ctx* ctx, int op, const double* b_free, double* x_free, double eps
 /* Form of the Jacobi-PCG recurrences for the quad4 matrix-free solves of
  * this context (pf_pcg and pf_pcg_dev with PF_OP_EBE64 / PF_OP_EBE32, and
  * pf_simp_run), until changed.  PF_PCG_SCALINGCG (the default) is
- * ScalingCG's own order (CG.h:419-453), two launches per iteration.
- * PF_PCG_SINGLE_REDUCTION is Chronopoulos and Gear's form (s = K p carried
- * as a vector, p.Kp from u.Ku): one launch and one reduction per iteration,
- * the same iterates in exact arithmetic, the same stopping test on the
- * recursive residual.  Hex8, CSR and learned solves keep ScalingCG's order.
- * Has no counterpart in the reference. */
-enum { PF_PCG_SCALINGCG = 0, PF_PCG_SINGLE_REDUCTION = 1 };
+ * ScalingCG's own order (CG.h:419-453), two launches per iteration.  The
+ * single-reduction forms follow Chronopoulos and Gear (p.Kp from u.Ku): one
+ * launch and one reduction per iteration, the same iterates in exact
+ * arithmetic, the same stopping test on the recursive residual, the Jacobi
+ * step as a product with 1/D.
+ *   PF_PCG_SINGLE_REDUCTION_CARRIED: s = K p and w = K u carried as vectors
+ *     (88 bytes per free DOF and iteration);
+ *   PF_PCG_SINGLE_REDUCTION_RECOMPUTED: s = K p recomputed from p on a
+ *     two-node tile halo, w = K u kept only for its sum (56 bytes);
+ *   PF_PCG_SINGLE_REDUCTION: the carried kernel while its working set (nine
+ *     nodal vectors and the moduli) fits in 200 MB, so in the 256 MB last-level
+ *     cache, else the recomputing one.
+ * Only FP64 quad4 solves use them: FP32 inner solves (PF_OP_EBE32), hex8,
+ * CSR and learned solves keep ScalingCG's order.  Has no counterpart in the
+ * reference. */
+enum {
+    PF_PCG_SCALINGCG = 0,
+    PF_PCG_SINGLE_REDUCTION = 1,
+    PF_PCG_SINGLE_REDUCTION_CARRIED = 2,
+    PF_PCG_SINGLE_REDUCTION_RECOMPUTED = 3
+};
 int pf_set_pcg_form(pf_ctx* ctx, int form);
 /* Same on device-resident free-DOF vectors. */
 int pf_pcg_dev(pf_ctx* ctx, int op, const double* b_free_dev, double* x_free_dev, double eps, int itrmax,

@@ -17,6 +17,7 @@ PF_OK, PF_NOT_CONVERGED = 0, 1
 PF_OP_CSR64, PF_OP_EBE64, PF_OP_EBE32, PF_OP_LEARNED, PF_OP_MG = 0, 1, 2, 3, 4
 PF_FILTER_VARIABLES, PF_FILTER_SENSITIVITIES = 0, 1
 PF_PCG_SCALINGCG, PF_PCG_SINGLE_REDUCTION = 0, 1
+PF_PCG_SINGLE_REDUCTION_CARRIED, PF_PCG_SINGLE_REDUCTION_RECOMPUTED = 2, 3
 
 _i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
 _f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
@@ -315,9 +316,12 @@ class Context:
         return dict(levels=nl.value, nx=nx.value, ny=ny.value, nz=nz.value, Ke=ke, D=D, fix=fix)
 
     def set_pcg_form(self, form):
-        """PF_PCG_SCALINGCG (ScalingCG's order, the default) or
-        PF_PCG_SINGLE_REDUCTION (one launch per iteration) for the quad4
-        matrix-free solves of this context."""
+        """PF_PCG_SCALINGCG (ScalingCG's order, the default) or a
+        single-reduction form (one launch per iteration) for the FP64 quad4
+        matrix-free solves of this context: PF_PCG_SINGLE_REDUCTION_CARRIED
+        (s = K p carried), PF_PCG_SINGLE_REDUCTION_RECOMPUTED (s = K p
+        recomputed on a two-node halo) or PF_PCG_SINGLE_REDUCTION (the
+        carried kernel while its working set fits in 200 MB)."""
         self._ck(lib().pf_set_pcg_form(self._h, form))
 
     def pcg(self, b, op=PF_OP_EBE64, eps=1e-10, itrmax=100000):

@@ -39,6 +39,17 @@ constexpr int kTileJ = PF_K1_TJ, kTileI = PF_K1_TI;
 #endif
 
 constexpr int kCg1TI = PF_CG1_TI, kCg1TJ = PF_CG1_TJ;
+// the single-reduction form's carried kernel runs while its working set
+// stays below this many bytes (inside the 256 MB last-level cache)
+constexpr double kCg1CacheBytes = 200e6;
+// tile of the recomputing single-reduction kernel (k_cg2_q4_tile)
+#ifndef PF_CG2_TI
+#define PF_CG2_TI 16
+#endif
+#ifndef PF_CG2_TJ
+#define PF_CG2_TJ 32
+#endif
+constexpr int kCg2TI = PF_CG2_TI, kCg2TJ = PF_CG2_TJ;
 
 // Launch shape of the quad4 operator kernel: tiles, and workgroups (a
 // multiple of 8, one group per XCD), one tile each (several tiles per
@@ -113,7 +124,7 @@ void free_grid(pf_ctx* ctx) {
     dfree(ctx->d_x); dfree(ctx->d_r); dfree(ctx->d_z); dfree(ctx->d_q); dfree(ctx->d_p0); dfree(ctx->d_p1);
     dfree(ctx->d_D); dfree(ctx->d_b); dfree(ctx->d_t0); dfree(ctx->d_t1);
     dfree(ctx->f_x); dfree(ctx->f_r); dfree(ctx->f_z); dfree(ctx->f_q); dfree(ctx->f_p0); dfree(ctx->f_p1);
-    dfree(ctx->d_s0); dfree(ctx->d_s1); dfree(ctx->f_s0); dfree(ctx->f_s1);
+    dfree(ctx->d_s0); dfree(ctx->d_s1); dfree(ctx->f_s0); dfree(ctx->f_s1); dfree(ctx->d_Dinv);
     dfree(ctx->f_D); dfree(ctx->f_b);
     dfree(ctx->d_e0); dfree(ctx->d_e1); dfree(ctx->d_e2); dfree(ctx->d_e3); dfree(ctx->d_e4); dfree(ctx->d_e5);
     dfree(ctx->d_e6);
@@ -136,7 +147,7 @@ int alloc_grid(pf_ctx* ctx) {
     PF_TRY(dalloc(ctx, &ctx->d_E, ne));
     PF_TRY(dalloc(ctx, &ctx->d_Ef, ne));
     double** dv[] = {&ctx->d_x, &ctx->d_r, &ctx->d_z, &ctx->d_q, &ctx->d_p0, &ctx->d_p1, &ctx->d_D, &ctx->d_b,
-                     &ctx->d_t0, &ctx->d_t1, &ctx->d_load, &ctx->d_s0, &ctx->d_s1};
+                     &ctx->d_t0, &ctx->d_t1, &ctx->d_load, &ctx->d_s0, &ctx->d_s1, &ctx->d_Dinv};
     for (double** p : dv) PF_TRY(dalloc(ctx, p, nv));
     float** fv[] = {&ctx->f_x, &ctx->f_r, &ctx->f_z, &ctx->f_q, &ctx->f_p0, &ctx->f_p1, &ctx->f_D, &ctx->f_b,
                     &ctx->f_s0, &ctx->f_s1};
@@ -258,12 +269,12 @@ struct SnapHook {
 };
 
 // Workgroups of k_cg1_q4_tile: full tiles, then edge tiles (launched first).
-Cg1Tiles cg1_tiles(const pf_ctx* ctx, unsigned* grid) {
+Cg1Tiles cg1_tiles(const pf_ctx* ctx, unsigned* grid, int ti = kCg1TI, int tj = kCg1TJ) {
     Cg1Tiles G;
-    G.fullI = (ctx->nx + 1) / kCg1TI;
-    G.fullJ = (ctx->ny + 1) / kCg1TJ;
-    G.tilesI = (ctx->nx + 1 + kCg1TI - 1) / kCg1TI;
-    G.tilesJ = (ctx->ny + 1 + kCg1TJ - 1) / kCg1TJ;
+    G.fullI = (ctx->nx + 1) / ti;
+    G.fullJ = (ctx->ny + 1) / tj;
+    G.tilesI = (ctx->nx + 1 + ti - 1) / ti;
+    G.tilesJ = (ctx->ny + 1 + tj - 1) / tj;
     G.perF = (G.fullI * G.fullJ + 7) / 8;
     G.nEdgePad = 8 * ((G.tilesI * G.tilesJ - G.fullI * G.fullJ + 7) / 8);
     G.edgeFirst = 1;
@@ -271,16 +282,22 @@ Cg1Tiles cg1_tiles(const pf_ctx* ctx, unsigned* grid) {
     return G;
 }
 
-// Single-reduction form on quad4 grids (k_cg1_q4_tile, one launch per
-// iteration).  Work vectors: x and p in place; r, w = K u and s = K p each in
+// Single-reduction forms on quad4 grids, one launch per iteration.
+// Carried (k_cg1_q4_tile): x and p in place; r, w = K u and s = K p each in
 // two buffers that alternate between launches (r|z, p1|q, s0|s1).
+// Recomputed (k_cg2_q4_tile): x in place; r and p alternate (r|z, p1|q).
+// Both scale by 1/D (d_Dinv, formed at the start of each solve).
 template <class T>
 int pcg_device_cg1(pf_ctx* ctx, const T* b, T* x, const T* D, T* r, T* z, T* p0, T* p1, T* q, double eps,
                    int itrmax, PcgRun& out) {
     PcgDev* S = (PcgDev*)ctx->d_pcg;
     double* part[2] = {ctx->red.partials, ctx->red.partials + 3 * kMaxPartials};
     unsigned g1;
-    const Cg1Tiles G = cg1_tiles(ctx, &g1);
+    // carried kernel: r, w, s (two buffers each), p, x, 1/D and the moduli
+    const double carried = 8.0 * (9.0 * ctx->nv + ctx->ne);
+    const bool kp = ctx->pcg_form == PF_PCG_SINGLE_REDUCTION_RECOMPUTED ||
+                    (ctx->pcg_form == PF_PCG_SINGLE_REDUCTION && carried > kCg1CacheBytes);
+    const Cg1Tiles G = kp ? cg1_tiles(ctx, &g1, kCg2TI, kCg2TJ) : cg1_tiles(ctx, &g1);
     PF_CHECK(g1 <= (unsigned)kMaxPartials, PF_ERR_ARG, "grid too large for the reduction scratch");
     const bool f32 = std::is_same<T, float>::value;
     const T* Ev = f32 ? (const T*)ctx->d_Ef : (const T*)ctx->d_E;
@@ -290,7 +307,9 @@ int pcg_device_cg1(pf_ctx* ctx, const T* b, T* x, const T* D, T* r, T* z, T* p0,
     T* R[2] = {r, z};
     T* W[2] = {p1, q};
     T* Sv[2] = {s0, s1};
+    T* Dinv = (T*)ctx->d_Dinv;
     k_cg1_start<<<1, 1, 0, ctx->stream>>>(S);
+    k_recip<T><<<gsize(ctx->nv), kBlock, 0, ctx->stream>>>(ctx->nv, D, Dinv);
     PF_TRY(check_launch(ctx));
     const int chunk = 32;
     const int mmax = itrmax + 1;  // launch m finishes iterate m; iterate itrmax is the last
@@ -306,7 +325,13 @@ int pcg_device_cg1(pf_ctx* ctx, const T* b, T* x, const T* D, T* r, T* z, T* p0,
         const int mend = std::min(mmax, m + chunk);
         for (; m < mend; m++) {
             const int c = m & 1, o = c ^ 1;
-            Cg1Vecs<T> V{m == 0 ? b : R[o], W[o], Sv[o], R[c], W[c], Sv[c], p0, x, D};
+            if (kp) {
+                Cg2Vecs<T> V{m == 0 ? b : R[o], W[o], R[c], W[c], x, Dinv};
+                k_cg2_q4_tile<T, kCg2TI, kCg2TJ><<<g1, kBlock, 0, ctx->stream>>>(
+                    ctx->nx, ctx->ny, G, m, V, Ev, Kv, ctx->d_fix, part[o], (int)g1, part[c], S, eps);
+                continue;
+            }
+            Cg1Vecs<T> V{m == 0 ? b : R[o], W[o], Sv[o], R[c], W[c], Sv[c], p0, x, Dinv};
             k_cg1_q4_tile<T, kCg1TI, kCg1TJ><<<g1, kBlock, 0, ctx->stream>>>(
                 ctx->nx, ctx->ny, G, m, V, Ev, Kv, ctx->d_fix, part[o], (int)g1, part[c], S, eps);
         }
@@ -340,7 +365,7 @@ int pcg_device(pf_ctx* ctx, bool csr, const T* b, T* x, const T* D, T* r, T* z, 
                int itrmax, PcgRun& out, SnapHook* snap = nullptr) {
     // FP32 solves keep ScalingCG's order: with FP32 vectors the carried s = K p
     // drifts and the single-reduction form needs about a third more iterations
-    if (!csr && is2d(ctx) && ctx->pcg_form == PF_PCG_SINGLE_REDUCTION && !snap && std::is_same<T, double>::value)
+    if (!csr && is2d(ctx) && ctx->pcg_form != PF_PCG_SCALINGCG && !snap && std::is_same<T, double>::value)
         return pcg_device_cg1<T>(ctx, b, x, D, r, z, p0, p1, q, eps, itrmax, out);
     const long n = csr ? ctx->N : ctx->nv;
     PcgDev* S = (PcgDev*)ctx->d_pcg;
@@ -2463,7 +2488,7 @@ int pf_mg_level(pf_ctx* ctx, int level, int* levels, int* nx, int* ny, int* nz, 
 
 int pf_set_pcg_form(pf_ctx* ctx, int form) {
     if (!ctx) return PF_ERR_ARG;
-    PF_CHECK(form == PF_PCG_SCALINGCG || form == PF_PCG_SINGLE_REDUCTION, PF_ERR_ARG, "pf_set_pcg_form: unknown form");
+    PF_CHECK(form >= PF_PCG_SCALINGCG && form <= PF_PCG_SINGLE_REDUCTION_RECOMPUTED, PF_ERR_ARG, "pf_set_pcg_form: unknown form");
     ctx->pcg_form = form;
     return PF_OK;
 }
@@ -2878,12 +2903,13 @@ int pf_profile_kernels(pf_ctx* ctx, int op, int reps, double* ms) {
         double* pc0 = ctx->red.partials;
         double* pc1 = ctx->red.partials + 3 * kMaxPartials;
         k_cg1_start<<<1, 1, 0, ctx->stream>>>(S);
+        k_recip<double><<<gsize(ctx->nv), kBlock, 0, ctx->stream>>>(ctx->nv, ctx->d_D, ctx->d_Dinv);
         Cg1Vecs<double> V0{ctx->d_load, ctx->d_p1, ctx->d_s1, ctx->d_r, ctx->d_p1, ctx->d_s0, ctx->d_p0, ctx->d_x,
-                           ctx->d_D};
+                           ctx->d_Dinv};
         k_cg1_q4_tile<double, kCg1TI, kCg1TJ><<<gc, kBlock, 0, ctx->stream>>>(
             ctx->nx, ctx->ny, G, 0, V0, ctx->d_E, ctx->d_K0, ctx->d_fix, pc1, (int)gc, pc0, S, 0.0);
         Cg1Vecs<double> V1{ctx->d_r, ctx->d_p1, ctx->d_s0, ctx->d_z, ctx->d_q, ctx->d_s1, ctx->d_p0, ctx->d_x,
-                           ctx->d_D};
+                           ctx->d_Dinv};
         ms[4] = timed([&] {
             k_cg1_q4_tile<double, kCg1TI, kCg1TJ><<<gc, kBlock, 0, ctx->stream>>>(
                 ctx->nx, ctx->ny, G, 1, V1, ctx->d_E, ctx->d_K0, ctx->d_fix, pc0, (int)gc, pc1, S, 0.0);

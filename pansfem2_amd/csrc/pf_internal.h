@@ -80,6 +80,7 @@ struct pf_ctx {
     // second buffers of the single-reduction solver's s = K p
     double *d_s0 = nullptr, *d_s1 = nullptr;
     float *f_s0 = nullptr, *f_s1 = nullptr;
+    double* d_Dinv = nullptr;  // 1 / D of the single-reduction solves
     float *f_D = nullptr, *f_b = nullptr;
 
     // element work vectors (ne)

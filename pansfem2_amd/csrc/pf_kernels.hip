@@ -451,7 +451,7 @@ struct Cg1Vecs {
     T* sout;       // s_{m-1}
     T* p;          // p_{m-2} -> p_{m-1}
     T* x;          // x_{m-1} -> x_m
-    const T* D;
+    const T* Dinv; // 1 / D
 };
 
 // the head of launch m >= 1 from the sums v = (gamma, delta, r.r) of iterate m-1
@@ -584,7 +584,7 @@ __global__ void __launch_bounds__(256) k_cg1_q4_tile(int nx, int ny, Cg1Tiles G,
             if (h < halo && ii >= 0 && ii <= nx && jj >= 0 && jj <= ny) {
                 const long n = (long)(ny + 1) * ii + jj;
                 rv[u] = load2(V.rin, n);
-                dv[u] = load2(V.D, n);
+                dv[u] = load2(V.Dinv, n);
                 if (prev) {
                     wv[u] = load2(V.win, n);
                     sv[u] = load2(V.sin, n);
@@ -642,12 +642,12 @@ __global__ void __launch_bounds__(256) k_cg1_q4_tile(int nx, int ny, Cg1Tiles G,
                 sn.y = wv[u].y + tb * sv[u].y;
                 rn.x = rv[u].x - ta * sn.x;     // r -= alpha s
                 rn.y = rv[u].y - ta * sn.y;
-                un.x = rn.x / dv[u].x;          // Scaling (CG.h:412)
-                un.y = rn.y / dv[u].y;
+                un.x = rn.x * dv[u].x;          // Scaling (CG.h:412), by 1/D
+                un.y = rn.y * dv[u].y;
                 su[h] = un;
                 if (a >= 1 && a <= ti && b >= 1 && b <= tj && ii <= nx && jj <= ny) {
                     const long n = (long)(ny + 1) * ii + jj;
-                    uo[(a - 1) * tj + (b - 1)] = Vec2<T>{rv[u].x / dv[u].x, rv[u].y / dv[u].y};
+                    uo[(a - 1) * tj + (b - 1)] = Vec2<T>{rv[u].x * dv[u].x, rv[u].y * dv[u].y};
                     store2(V.sout, n, sn);
                     store2(V.rout, n, rn);
                     acc[0] += (double)rn.x * (double)un.x;
@@ -714,6 +714,253 @@ __global__ void __launch_bounds__(256) k_cg1_q4_tile(int nx, int ny, Cg1Tiles G,
         }
     }
     store_partials<3>(acc, lds, partOut);
+}
+
+// The same recurrences with s = K p recomputed instead of carried, and w = K u
+// used only for its sum.  Launch m forms p_{m-1} = u_{m-1} + beta p_{m-2}
+// (u_{m-1} = r_{m-1} ./ D) on the tile and a two-node halo, s_{m-1} = K p_{m-1},
+// r_m = r_{m-1} - alpha s_{m-1} and u_m = r_m ./ D on the tile and a one-node
+// halo, then w_m = K u_m on the tile for delta_m = u_m.w_m.  x_m = x_{m-1} +
+// alpha p_{m-1} at the own nodes.  r and p alternate between two buffers
+// (neighbouring tiles read the previous launch's values on their halos); x is
+// updated in place.  The Jacobi step multiplies by 1/D (precomputed per
+// solve) where ScalingCG divides by D.  A launch reads r, 1/D, p, x and writes
+// r, p, x: 56 bytes per free DOF in FP64, against 88 when s and w are carried.
+// w_m is not masked at fixed DOFs: u_m is exactly 0 there, so delta_m gets
+// nothing from them either way.
+template <class T>
+struct Cg2Vecs {
+    const T* rin;  // r_{m-1} (b at m = 0)
+    const T* pin;  // p_{m-2}
+    T* rout;       // r_m
+    T* pout;       // p_{m-1}
+    T* x;          // x_{m-1} -> x_m
+    const T* Dinv; // 1 / D
+};
+
+template <class T, int TI, int TJ>
+__global__ void __launch_bounds__(256) k_cg2_q4_tile(int nx, int ny, Cg1Tiles G, int m,
+                                                     Cg2Vecs<T> V, const T* __restrict__ E,
+                                                     const T* __restrict__ K0, const uint8_t* __restrict__ fix,
+                                                     const double* __restrict__ partIn, int nbIn,
+                                                     double* __restrict__ partOut, PcgDev* __restrict__ S,
+                                                     double eps) {
+#pragma clang fp contract(fast)  // not a check-path kernel: FMA allowed
+    constexpr int w1 = TJ + 2, w2 = TJ + 4, we = TJ + 3;  // row strides of R1, R2 and the elements
+    constexpr int n1 = (TI + 2) * w1;   // R1: the tile and a one-node halo
+    constexpr int n2 = (TI + 4) * w2;   // R2: the tile and a two-node halo
+    constexpr int nel = (TI + 3) * we;  // the elements that touch R1
+    constexpr int own = TI * TJ;
+    constexpr int P1 = (n1 + 255) / 256, P2 = (n2 + 255) / 256;
+    constexpr int PE = (nel + 255) / 256, OWN = (own + 255) / 256;
+    __shared__ double lds[48];   // block_sum<3>: 16 per sum
+    __shared__ Vec2<T> sp[n2];   // p_{m-1} on R2 (stride w2), then u_m on R1 (stride w1)
+    __shared__ T se[nel];        // element (i0 - 2 + a, j0 - 2 + b) at a we + b
+    int TIr, TJr;
+    const bool has = cg1_tile(G, blockIdx.x, TIr, TJr);
+    const int i0 = TIr * TI, j0 = TJr * TJ;
+    const bool prev = m > 0;  // launch 0 reads only r = b
+    // entry h of R2 -> its R2 coordinates (node (i0 - 2 + a2, j0 - 2 + b2)):
+    // the R1 nodes first, row by row, so that a thread's R1 entries are among
+    // its R2 entries; then the ring: top row, bottom row, left and right columns
+    auto entry = [&](int h, int& a2, int& b2) {
+        if (h < n1) {
+            const int a = h / w1;
+            a2 = a + 1;
+            b2 = h - a * w1 + 1;
+            return;
+        }
+        int e = h - n1;
+        if (e < w2) { a2 = 0; b2 = e; return; }
+        e -= w2;
+        if (e < w2) { a2 = TI + 3; b2 = e; return; }
+        e -= w2;
+        if (e < TI + 2) { a2 = 1 + e; b2 = 0; return; }
+        e -= TI + 2;
+        a2 = 1 + e;
+        b2 = w2 - 1;
+    };
+    Vec2<T> rv[P2], dv[P2], pv[P2];
+    T ev[PE];
+    if (has) {
+#pragma unroll
+        for (int u = 0; u < P2; u++) {
+            const int h = threadIdx.x + u * 256;
+            rv[u] = pv[u] = Vec2<T>{0, 0};
+            dv[u] = Vec2<T>{1, 1};
+            if (h < n2) {
+                int a2, b2;
+                entry(h, a2, b2);
+                const int ii = i0 - 2 + a2, jj = j0 - 2 + b2;
+                if (ii >= 0 && ii <= nx && jj >= 0 && jj <= ny) {
+                    const long n = (long)(ny + 1) * ii + jj;
+                    rv[u] = load2(V.rin, n);
+                    dv[u] = load2(V.Dinv, n);
+                    if (prev) pv[u] = load2(V.pin, n);
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < PE; u++) {
+            const int h = threadIdx.x + u * 256;
+            const int a = h / we, b = h - a * we;
+            const int ei = i0 - 2 + a, ej = j0 - 2 + b;
+            ev[u] = (h < nel && ei >= 0 && ei < nx && ej >= 0 && ej < ny) ? E[(long)ny * ei + ej] : (T)0;
+        }
+    }
+    PartRegs<3, 5> pr;
+    if (prev) load_partials(partIn, nbIn, pr);
+    if (S->done) return;
+    double alpha = 0.0, beta = 0.0;
+    if (prev) {
+        double v[3];
+        bool stop;
+        sum_partials(pr, partIn, nbIn, lds, v);
+        cg1_head(m, v, S, eps, alpha, beta, stop);
+        if (stop) return;
+    }
+    const T ta = (T)alpha, tb = (T)beta;
+    double acc[3] = {0.0, 0.0, 0.0};  // r.u, u.w, r.r of iterate m
+    if (has) {
+#pragma unroll
+        for (int u = 0; u < P2; u++) {
+            const int h = threadIdx.x + u * 256;
+            if (h < n2) {
+                int a2, b2;
+                entry(h, a2, b2);
+                Vec2<T> pn;
+                pn.x = rv[u].x * dv[u].x + tb * pv[u].x;  // p = u + beta p (CG.h:412, 440)
+                pn.y = rv[u].y * dv[u].y + tb * pv[u].y;
+                sp[a2 * w2 + b2] = pn;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < PE; u++) {
+            const int h = threadIdx.x + u * 256;
+            if (h < nel) se[h] = ev[u];
+        }
+        // x and the fixed-DOF flags of this thread's R1 entries, in flight
+        // across the barrier (issued with the first loads they cost more)
+        Vec2<T> xv[P1];
+        int fv[P1];
+#pragma unroll
+        for (int u = 0; u < P1; u++) {
+            const int h = threadIdx.x + u * 256;
+            const int a = h / w1, b = h - a * w1;
+            const int ii = i0 - 1 + a, jj = j0 - 1 + b;
+            xv[u] = Vec2<T>{0, 0};
+            fv[u] = 0;
+            if (h < n1 && ii >= 0 && ii <= nx && jj >= 0 && jj <= ny) {
+                const long n = (long)(ny + 1) * ii + jj;
+                fv[u] = fix[n];
+                if (prev && a >= 1 && a <= TI && b >= 1 && b <= TJ) xv[u] = load2(V.x, n);
+            }
+        }
+        __syncthreads();
+        Vec2<T> un[P1];
+#pragma unroll
+        for (int u = 0; u < P1; u++) {
+            const int h = threadIdx.x + u * 256;
+            un[u] = Vec2<T>{0, 0};
+            if (h >= n1) continue;
+            const int a = h / w1, b = h - a * w1;  // R1 coordinates
+            const int ii = i0 - 1 + a, jj = j0 - 1 + b;
+            if (ii < 0 || ii > nx || jj < 0 || jj > ny) continue;
+            const long n = (long)(ny + 1) * ii + jj;
+            const bool mine = a >= 1 && a <= TI && b >= 1 && b <= TJ;
+            const Vec2<T>* s0 = sp + a * w2 + b;  // R2 rows a, a+1, a+2 from column b
+            T ax = 0, ay = 0, ex, ey;
+            if (prev) {  // s = K p (alpha = 0 at launch 0)
+                if (ii >= 1 && jj >= 1) {
+                    elem_rows_q4(K0, 2, s0[0], s0[w2], s0[w2 + 1], s0[1], ex, ey);
+                    const T Ee = se[a * we + b];
+                    ax += Ee * ex; ay += Ee * ey;
+                }
+                if (ii >= 1 && jj < ny) {
+                    elem_rows_q4(K0, 1, s0[1], s0[w2 + 1], s0[w2 + 2], s0[2], ex, ey);
+                    const T Ee = se[a * we + b + 1];
+                    ax += Ee * ex; ay += Ee * ey;
+                }
+                if (ii < nx && jj >= 1) {
+                    elem_rows_q4(K0, 3, s0[w2], s0[2 * w2], s0[2 * w2 + 1], s0[w2 + 1], ex, ey);
+                    const T Ee = se[(a + 1) * we + b];
+                    ax += Ee * ex; ay += Ee * ey;
+                }
+                if (ii < nx && jj < ny) {
+                    elem_rows_q4(K0, 0, s0[w2 + 1], s0[2 * w2 + 1], s0[2 * w2 + 2], s0[w2 + 2], ex, ey);
+                    const T Ee = se[(a + 1) * we + b + 1];
+                    ax += Ee * ex; ay += Ee * ey;
+                }
+                if (fv[u] & 1) ax = 0;
+                if (fv[u] & 2) ay = 0;
+            }
+            Vec2<T> rn;
+            rn.x = rv[u].x - ta * ax;  // r -= alpha s
+            rn.y = rv[u].y - ta * ay;
+            un[u].x = rn.x * dv[u].x;
+            un[u].y = rn.y * dv[u].y;
+            if (mine) {
+                const Vec2<T> pc = s0[w2 + 1];
+                Vec2<T> xn;
+                xn.x = xv[u].x + ta * pc.x;  // x += alpha p
+                xn.y = xv[u].y + ta * pc.y;
+                store2(V.rout, n, rn);
+                store2(V.pout, n, pc);
+                store2(V.x, n, xn);
+                acc[0] += (double)rn.x * (double)un[u].x;
+                acc[0] += (double)rn.y * (double)un[u].y;
+                acc[2] += (double)rn.x * (double)rn.x;
+                acc[2] += (double)rn.y * (double)rn.y;
+            }
+        }
+        __syncthreads();  // every read of p_{m-1} is done: u_m takes its place
+        Vec2<T>* su = sp;
+#pragma unroll
+        for (int u = 0; u < P1; u++) {
+            const int h = threadIdx.x + u * 256;
+            if (h < n1) su[h] = un[u];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int o = 0; o < OWN; o++) {
+            const int q = threadIdx.x + o * 256;
+            const int r = q / TJ, c = q - r * TJ;
+            const int i = i0 + r, j = j0 + c;
+            if (q >= own || i > nx || j > ny) continue;
+            const Vec2<T>* s0 = su + r * w1 + c;  // R1 rows r, r+1, r+2 from column c
+            T ax = 0, ay = 0, ex, ey;
+            if (i >= 1 && j >= 1) {
+                elem_rows_q4(K0, 2, s0[0], s0[w1], s0[w1 + 1], s0[1], ex, ey);
+                const T Ee = se[(r + 1) * we + c + 1];
+                ax += Ee * ex; ay += Ee * ey;
+            }
+            if (i >= 1 && j < ny) {
+                elem_rows_q4(K0, 1, s0[1], s0[w1 + 1], s0[w1 + 2], s0[2], ex, ey);
+                const T Ee = se[(r + 1) * we + c + 2];
+                ax += Ee * ex; ay += Ee * ey;
+            }
+            if (i < nx && j >= 1) {
+                elem_rows_q4(K0, 3, s0[w1], s0[2 * w1], s0[2 * w1 + 1], s0[w1 + 1], ex, ey);
+                const T Ee = se[(r + 2) * we + c + 1];
+                ax += Ee * ex; ay += Ee * ey;
+            }
+            if (i < nx && j < ny) {
+                elem_rows_q4(K0, 0, s0[w1 + 1], s0[2 * w1 + 1], s0[2 * w1 + 2], s0[w1 + 2], ex, ey);
+                const T Ee = se[(r + 2) * we + c + 2];
+                ax += Ee * ex; ay += Ee * ey;
+            }
+            const Vec2<T> uc = s0[w1 + 1];
+            acc[1] += (double)uc.x * (double)ax;
+            acc[1] += (double)uc.y * (double)ay;
+        }
+    }
+    store_partials<3>(acc, lds, partOut);
+}
+
+template <class T>
+__global__ void k_recip(long n, const T* __restrict__ D, T* __restrict__ out) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        out[i] = (T)1 / D[i];
 }
 
 __global__ void k_cg1_start(PcgDev* __restrict__ S) {

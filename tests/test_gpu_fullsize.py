@@ -7,9 +7,11 @@ drawn i.i.d. uniform in [1e-3, 1] and on a SIMP-like field (density 1
 inside seeded random ellipses, 1e-3 elsewhere, smoothed once by the R = 1.5
 filter): the FP64 solve reaches a 1e-8 recursive residual (ScalingCG's
 criterion), the mixed one a 1e-8 FP64 true residual, and the two agree to
-what those residuals allow.  (With SIMP moduli E = rho^3 the i.i.d.
-field puts contrasts up to 1e9 between neighbouring elements, beyond what
-FP32 CG resolves: there the refinement falls back to FP64 corrections.)
+what those residuals allow.  (With SIMP moduli E = rho^3 the i.i.d. field
+puts contrasts up to 1e9 between neighbouring elements, beyond what FP32 CG
+resolves: there the refinement falls back to FP64 corrections.)  The
+single-reduction form at this size (its recomputing kernel) against
+ScalingCG's order.
 
 128 x 64 x 64 hex8 (1.6M DOFs): the matrix-free operator against the
 assembled one, its symmetry, and the FP64 solve's true residual.
@@ -84,6 +86,36 @@ def test_c3_mixed_precision_2048x1024(pa, field):
     assert rel(x32, x64) < 1e-4
     # compliance f = u.b agrees far more tightly than the displacement
     assert abs(x32 @ b - x64 @ b) / abs(x64 @ b) < 1e-7
+    ctx.close()
+
+
+def test_c3_single_reduction_recomputed_2048x1024(pa):
+    """At 2048x1024 the carried single-reduction kernel's working set (about
+    320 MB) exceeds the last-level cache, so PF_PCG_SINGLE_REDUCTION runs the
+    recomputing kernel: the same bits as PF_PCG_SINGLE_REDUCTION_RECOMPUTED,
+    and the same solution as ScalingCG's order to what the residuals allow."""
+    nx, ny = 2048, 1024
+    print("c3 single-reduction forms, i.i.d. field", flush=True)
+    ctx = pa.cantilever2d(nx, ny)
+    ctx.set_density(np.random.default_rng(20261204).uniform(1e-3, 1.0, nx * ny))
+    b = ctx.load()
+    ctx.set_pcg_form(pa.PF_PCG_SCALINGCG)
+    x0, s0 = ctx.pcg(b, op=pa.PF_OP_EBE64, eps=1e-8)
+    ctx.set_pcg_form(pa.PF_PCG_SINGLE_REDUCTION)
+    x1, s1 = ctx.pcg(b, op=pa.PF_OP_EBE64, eps=1e-8)
+    ctx.set_pcg_form(pa.PF_PCG_SINGLE_REDUCTION_RECOMPUTED)
+    x3, s3 = ctx.pcg(b, op=pa.PF_OP_EBE64, eps=1e-8, itrmax=500)
+    print(f"  ScalingCG order {s0['iterations']} it {s0['seconds']:.2f} s; single reduction "
+          f"{s1['iterations']} it {s1['seconds']:.2f} s; |x1-x0|/|x0| {rel(x1, x0):.2e}", flush=True)
+    assert s0["status"] == 1 and s1["status"] == 1 and s1["rel_residual"] < 1e-8
+    assert abs(s1["iterations"] - s0["iterations"]) <= max(3, 0.005 * s0["iterations"])
+    assert rel(x1, x0) < 1e-6
+    assert abs(x1 @ b - x0 @ b) / abs(x0 @ b) < 1e-9
+    # the automatic form ran the recomputing kernel: its first 500 iterates
+    # carry the same bits
+    ctx.set_pcg_form(pa.PF_PCG_SINGLE_REDUCTION)
+    x1s, s1s = ctx.pcg(b, op=pa.PF_OP_EBE64, eps=1e-8, itrmax=500)
+    assert s3["iterations"] == s1s["iterations"] == 500 and np.array_equal(x3, x1s)
     ctx.close()
 
 

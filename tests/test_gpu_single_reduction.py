@@ -1,14 +1,15 @@
-"""GPU parity of the single-reduction Jacobi-PCG form (PF_PCG_SINGLE_REDUCTION:
-Chronopoulos and Gear's recurrences, one launch per iteration) against the
-reference's ScalingCG results and against the device's ScalingCG-order form.
+"""GPU parity of the single-reduction Jacobi-PCG forms (Chronopoulos and
+Gear's recurrences, one launch per iteration; s = K p carried as a vector or
+recomputed on a two-node tile halo) against the reference's ScalingCG results
+and against the device's ScalingCG-order form.
 
 Bars as for ScalingCG itself: displacement within 1e-8 of the reference's at
 eps = 1e-10; the 50-iteration SIMP loop within 1e-9 (compliance) and 1e-6
 (densities); CG iteration counts within max(3, 0.5%).  The two forms generate
 the same iterates in exact arithmetic, so at grids beyond the oracle's reach
 they must agree with each other to what their residuals allow.  Grid shapes
-cover every tile layout of the kernel: only edge tiles (4x2, 120x40), full
-tiles only (127x63), and both (1024x512, 135x70).
+cover the tile layouts of both kernels (8x64 and 16x32 nodes): only edge
+tiles (4x2, 120x40 for 8x64), full tiles only (127x63 for 8x64), and both.
 """
 import numpy as np
 import pytest
@@ -32,15 +33,19 @@ def close_counts(a, b):
     return abs(a - b) <= max(3, 0.005 * b)
 
 
+FORMS = [2, 3]  # PF_PCG_SINGLE_REDUCTION_CARRIED, PF_PCG_SINGLE_REDUCTION_RECOMPUTED
+
+
+@pytest.mark.parametrize("form", FORMS)
 @pytest.mark.parametrize("case", ["simp_4x2_oc", "simp_8x4_oc", "simp_120x40_oc"])
-def test_solve_against_reference(pa, case):
+def test_solve_against_reference(pa, case, form):
     from oracle import pyoracle as po
     g = gi.load(case)
     m = gi.meta(case)
-    print(f"single-reduction solve {case}", flush=True)
+    print(f"single-reduction solve {case} form {form}", flush=True)
     ctx = pa.cantilever2d(m["nx"], m["ny"])
     ctx.set_density(g["rho0"])
-    ctx.set_pcg_form(pa.PF_PCG_SINGLE_REDUCTION)
+    ctx.set_pcg_form(form)
     u, st = ctx.pcg(g["F"], op=pa.PF_OP_EBE64, eps=1e-10)
     assert st["status"] == 1
     assert rel(u, g["u0_free"]) < 1e-8
@@ -50,14 +55,15 @@ def test_solve_against_reference(pa, case):
     ctx.close()
 
 
-def test_simp_loop_c1_against_reference(pa):
+@pytest.mark.parametrize("form", FORMS)
+def test_simp_loop_c1_against_reference(pa, form):
     """50 OC iterations at 120x40 with the single-reduction solver."""
     from oracle import pyoracle as po
     case = "simp_120x40_oc"
     g = gi.load(case)
-    print("single-reduction SIMP loop 120x40", flush=True)
+    print(f"single-reduction SIMP loop 120x40 form {form}", flush=True)
     ctx = pa.cantilever2d(120, 40)
-    ctx.set_pcg_form(pa.PF_PCG_SINGLE_REDUCTION)
+    ctx.set_pcg_form(form)
     r = ctx.simp_run(50)
     ref = po.simp2d(120, 40, 50)
     assert gi.sha(ref["f"]) == gi.digest(case, "hist_f")
@@ -68,17 +74,18 @@ def test_simp_loop_c1_against_reference(pa):
     ctx.close()
 
 
+@pytest.mark.parametrize("form", FORMS)
 @pytest.mark.parametrize("nx,ny", [(127, 63), (135, 70), (1024, 512)])
-def test_agrees_with_scalingcg_order(pa, nx, ny):
+def test_agrees_with_scalingcg_order(pa, nx, ny, form):
     """Beyond the oracle's sizes: both forms on an i.i.d. density field."""
-    print(f"single-reduction vs ScalingCG order {nx}x{ny}", flush=True)
+    print(f"single-reduction vs ScalingCG order {nx}x{ny} form {form}", flush=True)
     ctx = pa.cantilever2d(nx, ny)
     rng = np.random.default_rng(20261206)
     ctx.set_density(rng.uniform(1e-3, 1.0, nx * ny))
     b = ctx.load()
     ctx.set_pcg_form(pa.PF_PCG_SCALINGCG)
     u0, s0 = ctx.pcg(b, op=pa.PF_OP_EBE64, eps=1e-8)
-    ctx.set_pcg_form(pa.PF_PCG_SINGLE_REDUCTION)
+    ctx.set_pcg_form(form)
     u1, s1 = ctx.pcg(b, op=pa.PF_OP_EBE64, eps=1e-8)
     print(f"  iterations {s0['iterations']} vs {s1['iterations']}, |u1-u0|/|u0| {rel(u1, u0):.2e}, "
           f"true residuals {s0['true_residual']:.2e} {s1['true_residual']:.2e}", flush=True)
@@ -91,12 +98,13 @@ def test_agrees_with_scalingcg_order(pa, nx, ny):
     ctx.close()
 
 
-def test_zero_load_and_itrmax_exit(pa):
+@pytest.mark.parametrize("form", FORMS)
+def test_zero_load_and_itrmax_exit(pa, form):
     ctx = pa.cantilever2d(64, 32)
     rng = np.random.default_rng(5)
     ctx.set_density(rng.uniform(0.1, 1.0, 64 * 32))
     b = ctx.load()
-    ctx.set_pcg_form(pa.PF_PCG_SINGLE_REDUCTION)
+    ctx.set_pcg_form(form)
     # b = 0: x = 0 without iterating
     x, st = ctx.pcg(np.zeros_like(b), op=pa.PF_OP_EBE64, eps=1e-10)
     assert st["status"] == 1 and st["iterations"] == 0 and not np.any(x)
@@ -108,6 +116,21 @@ def test_zero_load_and_itrmax_exit(pa):
     x0, st0 = ctx.pcg(b, op=pa.PF_OP_EBE64, eps=1e-10, itrmax=7)
     assert st0["status"] == 2 and st0["iterations"] == 7
     assert rel(x1, x0) < 1e-12
+    ctx.close()
+
+
+def test_automatic_form_takes_the_carried_kernel_below_200_mb(pa):
+    """PF_PCG_SINGLE_REDUCTION at 1024x512 (a 93 MB working set) runs the
+    carried kernel: the same bits as PF_PCG_SINGLE_REDUCTION_CARRIED."""
+    ctx = pa.cantilever2d(1024, 512)
+    rng = np.random.default_rng(11)
+    ctx.set_density(rng.uniform(1e-3, 1.0, ctx.ne))
+    b = ctx.load()
+    ctx.set_pcg_form(pa.PF_PCG_SINGLE_REDUCTION)
+    x1, s1 = ctx.pcg(b, op=pa.PF_OP_EBE64, eps=1e-6)
+    ctx.set_pcg_form(pa.PF_PCG_SINGLE_REDUCTION_CARRIED)
+    x2, s2 = ctx.pcg(b, op=pa.PF_OP_EBE64, eps=1e-6)
+    assert s1["iterations"] == s2["iterations"] and np.array_equal(x1, x2)
     ctx.close()
 
 
