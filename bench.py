@@ -83,6 +83,28 @@ def kernels_only(ctx, pa, torch, nx, ny, reps=20, copy_mib=256):
                       "N": ctx.N, "ne": ctx.ne, "kernels_ms": prof}))
 
 
+def form_sweep(pa, sizes=((1024, 512), (1280, 640), (1536, 768), (1792, 896), (2048, 1024)), iters=2000):
+    """Device time per FP64 Jacobi-PCG iteration of ScalingCG's order and of
+    the two single-reduction kernels, on quad4 cantilevers of several sizes
+    (density i.i.d. uniform in [1e-3, 1], seed 20261202; best of three solves
+    of `iters` iterations each): where the carried kernel's working set
+    leaves the last-level cache, the recomputing kernel takes over."""
+    forms = (("scalingcg", pa.PF_PCG_SCALINGCG), ("carried", pa.PF_PCG_SINGLE_REDUCTION_CARRIED),
+             ("recomputed", pa.PF_PCG_SINGLE_REDUCTION_RECOMPUTED))
+    for nx, ny in sizes:
+        ctx = pa.cantilever2d(nx, ny)
+        ctx.set_density(np.random.default_rng(20261202).uniform(1e-3, 1.0, ctx.ne))
+        b = ctx.load()
+        nv = 2 * (nx + 1) * (ny + 1)
+        row = {"nx": nx, "ny": ny, "carried_working_set_MB": round(8 * (9 * nv + ctx.ne) / 1e6, 1)}
+        for name, form in forms:
+            ctx.set_pcg_form(form)
+            best = min(ctx.pcg(b, op=pa.PF_OP_EBE64, eps=1e-30, itrmax=iters)[1]["seconds"] for _ in range(3))
+            row[name + "_us_per_iteration"] = round(best / iters * 1e6, 2)
+        print(json.dumps(row), flush=True)
+        ctx.close()
+
+
 def stream_copy_gbs(torch, mib=256, reps=20):
     """STREAM-like device-to-device copy of `mib` MiB on torch's stream, timed
     with events: bytes read + written per second, the bandwidth a plain
@@ -168,6 +190,8 @@ def main():
     ap.add_argument("--kernels-only", action="store_true",
                     help="only the PCG/filter/OC kernels (pf_profile_kernels) on a seeded density field, then a "
                          "plain device copy of known size: the program the counter runs profile")
+    ap.add_argument("--form-sweep", action="store_true",
+                    help="only the per-iteration time of the three FP64 Jacobi-PCG forms at five grid sizes")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); a multi-GPU decomposition of "
@@ -177,6 +201,9 @@ def main():
     import torch
     import pansfem2_amd as pa
 
+    if args.form_sweep:
+        form_sweep(pa)
+        return
     nx, ny = args.nx, args.ny
     ctx = pa.cantilever2d(nx, ny)
     N, ne = ctx.N, ctx.ne

@@ -189,8 +189,8 @@ int pf_pcg(pf_ctx* ctx, int op, const double* b_free, double* x_free, double eps
  *   PF_PCG_SINGLE_REDUCTION_RECOMPUTED: s = K p recomputed from p on a
  *     two-node tile halo, w = K u kept only for its sum (56 bytes);
  *   PF_PCG_SINGLE_REDUCTION: the carried kernel while its working set (nine
- *     nodal vectors and the moduli) fits in 200 MB, so in the 256 MB last-level
- *     cache, else the recomputing one.
+ *     nodal vectors and the moduli) stays below 150 MB, else the recomputing
+ *     one (the two meet near 125 MB; the last-level cache holds 256 MB).
  * Only FP64 quad4 solves use them: FP32 inner solves (PF_OP_EBE32), hex8,
  * CSR and learned solves keep ScalingCG's order.  Has no counterpart in the
  * reference. */

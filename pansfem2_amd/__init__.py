@@ -321,7 +321,7 @@ class Context:
         matrix-free solves of this context: PF_PCG_SINGLE_REDUCTION_CARRIED
         (s = K p carried), PF_PCG_SINGLE_REDUCTION_RECOMPUTED (s = K p
         recomputed on a two-node halo) or PF_PCG_SINGLE_REDUCTION (the
-        carried kernel while its working set fits in 200 MB)."""
+        carried kernel while its working set stays below 150 MB)."""
         self._ck(lib().pf_set_pcg_form(self._h, form))
 
     def pcg(self, b, op=PF_OP_EBE64, eps=1e-10, itrmax=100000):

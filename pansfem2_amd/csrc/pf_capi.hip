@@ -40,8 +40,10 @@ constexpr int kTileJ = PF_K1_TJ, kTileI = PF_K1_TI;
 
 constexpr int kCg1TI = PF_CG1_TI, kCg1TJ = PF_CG1_TJ;
 // the single-reduction form's carried kernel runs while its working set
-// stays below this many bytes (inside the 256 MB last-level cache)
-constexpr double kCg1CacheBytes = 200e6;
+// stays below this many bytes; beyond, the recomputing kernel's smaller
+// per-iteration traffic wins (measured crossover near 125 MB; the last-level
+// cache holds 256 MB)
+constexpr double kCg1CacheBytes = 150e6;
 // tile of the recomputing single-reduction kernel (k_cg2_q4_tile)
 #ifndef PF_CG2_TI
 #define PF_CG2_TI 16

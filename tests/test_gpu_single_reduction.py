@@ -119,8 +119,8 @@ def test_zero_load_and_itrmax_exit(pa, form):
     ctx.close()
 
 
-def test_automatic_form_takes_the_carried_kernel_below_200_mb(pa):
-    """PF_PCG_SINGLE_REDUCTION at 1024x512 (a 93 MB working set) runs the
+def test_automatic_form_takes_the_carried_kernel_below_150_mb(pa):
+    """PF_PCG_SINGLE_REDUCTION at 1024x512 (an 80 MB working set) runs the
     carried kernel: the same bits as PF_PCG_SINGLE_REDUCTION_CARRIED."""
     ctx = pa.cantilever2d(1024, 512)
     rng = np.random.default_rng(11)
