@@ -599,8 +599,9 @@ struct MgState {
     double* W = nullptr;    // inversion scratch
     std::vector<void*> owned;
 };
-// dot products one batched reduction holds (multidot_batch)
-constexpr long kMpartDots = 128;
+// dot products one batched reduction holds (multidot_batch), and its grid
+constexpr int kMpartDots = 128;
+constexpr int kMpartGrid = 1024;
 
 struct LearnedNet {
     int H[4] = {0}, Wd[4] = {0};
@@ -624,7 +625,7 @@ struct LearnedNet {
     // flexible CG: stored directions and their operator images
     int fcg_m = 0;
     std::vector<double*> fP, fQ;
-    double* mpart = nullptr;  // multi-dot partials (grid x 32)
+    double* mpart = nullptr;  // multi-dot partials (grid x dots), then the sums
 };
 
 namespace {
@@ -724,7 +725,7 @@ int net_build(pf_ctx* ctx, uint64_t seed) {
     N->wpart_n = ((p0 + ppb - 1) / ppb) * maxw;
     PF_TRY(net_alloc(ctx, N, &N->wpart, N->wpart_n));
     const long nv = ctx->nv;
-    PF_TRY(net_alloc(ctx, N, &N->mpart, 256L * kMpartDots));
+    PF_TRY(net_alloc(ctx, N, &N->mpart, (long)kMpartGrid * kMpartDots + kMpartDots));
     double** sc[] = {&N->w, &N->Kw, &N->gd, &N->rr, &N->ee, &N->dd};
     for (double** q : sc) PF_TRY(net_alloc(ctx, N, q, nv));
     return sync(ctx);
@@ -732,7 +733,12 @@ int net_build(pf_ctx* ctx, uint64_t seed) {
 
 template <int CI, int CO>
 void conv_fwd_t(pf_ctx* ctx, int H, int W, const bf16* X, const bf16* Wt, bf16* Y, float* Yf, int relu) {
-    dim3 grid(H, (W + 63) / 64);
+    if (CI == 64 || CO == 64) {
+        dim3 grid(H, (W + 63) / 64);
+        k_conv_fwd_row<CI, CO><<<grid, 256, 0, ctx->stream>>>(H, W, X, Wt, Y, Yf, relu);
+        return;
+    }
+    dim3 grid((H + kConvTR - 1) / kConvTR, (W + kConvTW - 1) / kConvTW);
     k_conv_fwd<CI, CO><<<grid, 256, 0, ctx->stream>>>(H, W, X, Wt, Y, Yf, relu);
 }
 
@@ -776,11 +782,11 @@ int net_forward(pf_ctx* ctx) {
         return conv_fwd(ctx, l.cin, l.cout, N->H[l.level], N->Wd[l.level], X, l.Wb, Y, Yf, l.relu);
     };
     auto pool = [&](int lv, int C, const bf16* in, bf16* out) {
-        long n = lvl_px(N, lv + 1) * C;
+        long n = lvl_px(N, lv + 1) * C / 8;  // 8 channels per thread
         k_pool2<<<gsize(n), kBlock, 0, ctx->stream>>>(N->H[lv], N->Wd[lv], C, in, out);
     };
     auto upadd = [&](int lv, int C, const bf16* coarse, const bf16* skip, bf16* out) {
-        long n = lvl_px(N, lv) * C;
+        long n = lvl_px(N, lv) * C / 8;
         k_up2_add<<<gsize(n), kBlock, 0, ctx->stream>>>(N->H[lv], N->Wd[lv], C, coarse, skip, out);
     };
     PF_TRY(lay(0, N->x0, N->a0, nullptr));
@@ -1317,34 +1323,10 @@ double dot_nodal(pf_ctx* ctx, const double* a, const double* b, int* rc) {
 // (a, c) minimising ||a t + c y - e||_K with t = N->dd, y = N->w, K t = N->gd,
 // K y = N->Kw and K e = r; falls back to the best-scaled t alone when the
 // 2x2 Gram matrix is singular.
-// dots of `a` with up to 32 vectors in one pass, summed on the host in a
-// fixed order
-int multidot(pf_ctx* ctx, const double* a, const double* const* b, int nb, double* out) {
-    LearnedNet* N = ctx->net;
-    PtrPack P{};
-    for (int j = 0; j < nb; j++) P.p[j] = b[j];
-    const unsigned g = std::min<unsigned>(gsize(ctx->nv), 256u);
-    if (nb <= 4)
-        k_multidot<4><<<g, kBlock, 0, ctx->stream>>>(ctx->nv, nb, a, P, N->mpart);
-    else if (nb <= 8)
-        k_multidot<8><<<g, kBlock, 0, ctx->stream>>>(ctx->nv, nb, a, P, N->mpart);
-    else
-        k_multidot<32><<<g, kBlock, 0, ctx->stream>>>(ctx->nv, nb, a, P, N->mpart);
-    PF_TRY(check_launch(ctx));
-    std::vector<double> h((size_t)g * nb);
-    PF_HIP(hipMemcpyAsync(h.data(), N->mpart, sizeof(double) * g * nb, hipMemcpyDeviceToHost, ctx->stream));
-    PF_TRY(sync(ctx));
-    for (int j = 0; j < nb; j++) {
-        double s = 0.0;
-        for (unsigned q = 0; q < g; q++) s += h[(size_t)q * nb + j];
-        out[j] = s;
-    }
-    return PF_OK;
-}
-
-// Several sets of dot products (a_k . b_kj) in one batch: the launches
-// (at most 32 vectors each), one copy of the partials and one host
-// synchronisation; sums in a fixed order on the host.
+// Several sets of dot products (a_k . b_kj) in one batch: the partial-sum
+// launches (at most 32 vectors each), one launch that reduces every dot's
+// partials on the device in a fixed order, one copy of the sums and one host
+// synchronisation.
 struct DotJob {
     const double* a;
     std::vector<const double*> b;
@@ -1352,23 +1334,31 @@ struct DotJob {
 };
 int multidot_batch(pf_ctx* ctx, std::vector<DotJob>& jobs) {
     LearnedNet* N = ctx->net;
-    const unsigned g = std::min<unsigned>(gsize(ctx->nv), 256u);
+    const unsigned g = std::min<unsigned>(gsize(ctx->nv / 2), (unsigned)kMpartGrid);
     struct Chunk {
         int job, first, nb;
         long off;
     };
     std::vector<Chunk> ch;
+    DotIdx idx{};
     long off = 0;
+    int nd = 0;
     for (int j = 0; j < (int)jobs.size(); j++) {
         const int nbj = (int)jobs[j].b.size();
         jobs[j].out.assign(nbj, 0.0);
         for (int f = 0; f < nbj; f += 32) {
             const int nb = std::min(32, nbj - f);
+            PF_CHECK(nd + nb <= kMpartDots, PF_ERR_ARG, "too many dot products in one batch");
             ch.push_back({j, f, nb, off});
+            for (int q = 0; q < nb; q++, nd++) {
+                idx.off[nd] = (int)off;
+                idx.nb[nd] = (short)nb;
+                idx.q[nd] = (short)q;
+            }
             off += (long)g * nb;
         }
     }
-    PF_CHECK(off <= 256L * kMpartDots, PF_ERR_ARG, "too many dot products in one batch");
+    if (nd == 0) return PF_OK;
     for (const Chunk& c : ch) {
         PtrPack P{};
         for (int q = 0; q < c.nb; q++) P.p[q] = jobs[c.job].b[c.first + q];
@@ -1381,16 +1371,25 @@ int multidot_batch(pf_ctx* ctx, std::vector<DotJob>& jobs) {
         else
             k_multidot<32><<<g, kBlock, 0, ctx->stream>>>(ctx->nv, c.nb, a, P, part);
     }
+    double* sums = N->mpart + (long)kMpartGrid * kMpartDots;
+    k_multidot_final<<<nd, kBlock, 0, ctx->stream>>>(N->mpart, (int)g, idx, sums);
     PF_TRY(check_launch(ctx));
-    std::vector<double> h(off);
-    PF_HIP(hipMemcpyAsync(h.data(), N->mpart, sizeof(double) * off, hipMemcpyDeviceToHost, ctx->stream));
+    std::vector<double> h(nd);
+    PF_HIP(hipMemcpyAsync(h.data(), sums, sizeof(double) * nd, hipMemcpyDeviceToHost, ctx->stream));
     PF_TRY(sync(ctx));
+    int d = 0;
     for (const Chunk& c : ch)
-        for (int q = 0; q < c.nb; q++) {
-            double s = 0.0;
-            for (unsigned b = 0; b < g; b++) s += h[c.off + (long)b * c.nb + q];
-            jobs[c.job].out[c.first + q] = s;
-        }
+        for (int q = 0; q < c.nb; q++) jobs[c.job].out[c.first + q] = h[d++];
+    return PF_OK;
+}
+
+// dots of `a` with up to 32 vectors in one pass
+int multidot(pf_ctx* ctx, const double* a, const double* const* b, int nb, double* out) {
+    std::vector<DotJob> jobs(1);
+    jobs[0].a = a;
+    jobs[0].b.assign(b, b + nb);
+    PF_TRY(multidot_batch(ctx, jobs));
+    for (int j = 0; j < nb; j++) out[j] = jobs[0].out[j];
     return PF_OK;
 }
 

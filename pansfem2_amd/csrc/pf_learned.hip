@@ -45,11 +45,14 @@ __device__ __host__ inline float bf2f(bf16 h) {
 __device__ inline bf16x4 ld4(const bf16* p) { return *reinterpret_cast<const bf16x4*>(p); }
 
 // ------------------------------------------------------------- forward ---
-// One workgroup = one image row, 4 wavefronts x 16 pixels.  mode: 0 linear,
-// 1 ReLU.  If Yf is non-null the FP32 result is written there instead.
+// Row form: one workgroup = one image row, 4 wavefronts x 16 pixels, inputs
+// and weights read from global memory per tap.  Used where a channel count
+// is 64 (the coarse levels): there the tiled form below needs 50 KB of LDS
+// or 76 accumulator registers and runs at 3 waves per SIMD.
 template <int CIN, int COUT>
-__global__ void __launch_bounds__(256) k_conv_fwd(int H, int W, const bf16* __restrict__ X, const bf16* __restrict__ Wt,
-                                                  bf16* __restrict__ Y, float* __restrict__ Yf, int relu) {
+__global__ void __launch_bounds__(256) k_conv_fwd_row(int H, int W, const bf16* __restrict__ X,
+                                                      const bf16* __restrict__ Wt, bf16* __restrict__ Y,
+                                                      float* __restrict__ Yf, int relu) {
     const int i = blockIdx.x;
     const int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
     const int j0 = blockIdx.y * 64 + wave * 16;
@@ -94,6 +97,84 @@ __global__ void __launch_bounds__(256) k_conv_fwd(int H, int W, const bf16* __re
         } else {
             bf16x4 pk = {(short)f2bf(v[0]), (short)f2bf(v[1]), (short)f2bf(v[2]), (short)f2bf(v[3])};
             *reinterpret_cast<bf16x4*>(Y + o + co) = pk;
+        }
+    }
+}
+
+// Tiled form: one workgroup = a tile of 4 image rows x 64 pixels.  The tile's input and
+// its one-pixel halo (6 x 66 pixels, channels last) are staged once in LDS;
+// wave w computes row w of the tile as four 16-pixel groups, so each weight
+// fragment (tap, output block, input block) feeds four MFMAs.  Per output
+// the sum runs over taps, then input blocks of 16 channels.  mode: 0 linear,
+// 1 ReLU.  If Yf is non-null the FP32 result is written there instead.
+constexpr int kConvTR = 4, kConvTW = 64;
+template <int CIN, int COUT>
+__global__ void __launch_bounds__(256) k_conv_fwd(int H, int W, const bf16* __restrict__ X, const bf16* __restrict__ Wt,
+                                                  bf16* __restrict__ Y, float* __restrict__ Yf, int relu) {
+    constexpr int SW = kConvTW + 2, SR = kConvTR + 2, CH = CIN / 8;  // 16-byte chunks per pixel
+    constexpr int CT = COUT / 16, KT = CIN / 16;
+    __shared__ __align__(16) bf16 sx[SR * SW * CIN];
+    const int i0 = blockIdx.x * kConvTR, j0 = blockIdx.y * kConvTW;
+    for (int e = threadIdx.x; e < SR * SW * CH; e += 256) {
+        const int pix = e / CH, ch = e - pix * CH;
+        const int r = pix / SW, cc = pix - r * SW;
+        const int ii = i0 - 1 + r, jj = j0 - 1 + cc;
+        uint4 v = {0u, 0u, 0u, 0u};
+        if (ii >= 0 && ii < H && jj >= 0 && jj < W)
+            v = *reinterpret_cast<const uint4*>(X + ((long)ii * W + jj) * CIN + ch * 8);
+        *reinterpret_cast<uint4*>(sx + pix * CIN + ch * 8) = v;
+    }
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const int g = l >> 4, pl = l & 15;
+    const int i = i0 + wave;
+    if (i >= H) return;  // no barrier follows
+    f32x4 acc[4][CT];
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+#pragma unroll
+        for (int c = 0; c < CT; c++) acc[q][c] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int tap = 0; tap < 9; tap++) {
+        const int dr = tap / 3, dc = tap % 3;
+#pragma unroll
+        for (int kc = 0; kc < KT; kc++) {
+            const int ci = kc * 16 + 4 * g;
+            bf16x4 b[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+                b[q] = *reinterpret_cast<const bf16x4*>(sx + ((wave + dr) * SW + q * 16 + pl + dc) * CIN + ci);
+#pragma unroll
+            for (int c = 0; c < CT; c++) {
+                const int co = c * 16 + pl;
+                const bf16x4 a = ld4(Wt + ((long)(tap * COUT + co) * CIN + ci));
+#pragma unroll
+                for (int q = 0; q < 4; q++)
+                    acc[q][c] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b[q], acc[q][c], 0, 0, 0);
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int px = j0 + q * 16 + pl;
+        if (px >= W) continue;
+        const long o = ((long)i * W + px) * COUT;
+#pragma unroll
+        for (int c = 0; c < CT; c++) {
+            float v[4];
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                v[t] = acc[q][c][t];
+                if (relu && v[t] < 0.f) v[t] = 0.f;
+            }
+            const int co = c * 16 + 4 * g;
+            if (Yf) {
+#pragma unroll
+                for (int t = 0; t < 4; t++) Yf[o + co + t] = v[t];
+            } else {
+                bf16x4 pk = {(short)f2bf(v[0]), (short)f2bf(v[1]), (short)f2bf(v[2]), (short)f2bf(v[3])};
+                *reinterpret_cast<bf16x4*>(Y + o + co) = pk;
+            }
         }
     }
 }
@@ -156,24 +237,42 @@ __global__ void k_relu_bwd(long n, const bf16* __restrict__ Y, bf16* __restrict_
 }
 
 // average pooling 2x2 (ceil), coarse (Hc, Wc) from fine (H, W)
+// Eight channels (16 bytes) per access: C is a multiple of 16.
+__device__ inline void unpack8(uint4 v, float (&f)[8]) {
+    const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int t = 0; t < 8; t++) f[t] = bf2f((bf16)((t & 1) ? (w[t >> 1] >> 16) : (w[t >> 1] & 0xFFFFu)));
+}
+__device__ inline uint4 pack8(const float (&f)[8]) {
+    uint32_t w[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) w[k] = (uint32_t)f2bf(f[2 * k]) | ((uint32_t)f2bf(f[2 * k + 1]) << 16);
+    return uint4{w[0], w[1], w[2], w[3]};
+}
+
 __global__ void k_pool2(int H, int W, int C, const bf16* __restrict__ in, bf16* __restrict__ out) {
-    const int Hc = (H + 1) / 2, Wc = (W + 1) / 2;
-    const long n = (long)Hc * Wc * C;
-    for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
-        const int c = (int)(q % C);
-        const long pc = q / C;
+    const int Hc = (H + 1) / 2, Wc = (W + 1) / 2, C8 = C / 8;
+    const long n8 = (long)Hc * Wc * C8;
+    for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n8; q += (long)gridDim.x * blockDim.x) {
+        const int c8 = (int)(q % C8);
+        const long pc = q / C8;
         const int I = (int)(pc / Wc), J = (int)(pc - (long)I * Wc);
-        float s = 0.f;
+        float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
         int cnt = 0;
         for (int a = 0; a < 2; a++)
             for (int b = 0; b < 2; b++) {
                 int i = 2 * I + a, j = 2 * J + b;
                 if (i < H && j < W) {
-                    s += bf2f(in[((long)i * W + j) * C + c]);
+                    float f[8];
+                    unpack8(*reinterpret_cast<const uint4*>(in + ((long)i * W + j) * C + 8 * c8), f);
+#pragma unroll
+                    for (int t = 0; t < 8; t++) s[t] += f[t];
                     cnt++;
                 }
             }
-        out[q] = f2bf(s / (float)cnt);
+#pragma unroll
+        for (int t = 0; t < 8; t++) s[t] = s[t] / (float)cnt;
+        *reinterpret_cast<uint4*>(out + 8 * q) = pack8(s);
     }
 }
 
@@ -194,13 +293,18 @@ __global__ void k_pool2_bwd_add(int H, int W, int C, const bf16* __restrict__ do
 // out (fine) = nearest-upsampled coarse + skip
 __global__ void k_up2_add(int H, int W, int C, const bf16* __restrict__ coarse, const bf16* __restrict__ skip,
                           bf16* __restrict__ out) {
-    const int Wc = (W + 1) / 2;
-    const long n = (long)H * W * C;
-    for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
-        const int c = (int)(q % C);
-        const long p = q / C;
+    const int Wc = (W + 1) / 2, C8 = C / 8;
+    const long n8 = (long)H * W * C8;
+    for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n8; q += (long)gridDim.x * blockDim.x) {
+        const int c8 = (int)(q % C8);
+        const long p = q / C8;
         const int i = (int)(p / W), j = (int)(p - (long)i * W);
-        out[q] = f2bf(bf2f(coarse[((long)(i / 2) * Wc + j / 2) * C + c]) + bf2f(skip[q]));
+        float fc[8], fs[8];
+        unpack8(*reinterpret_cast<const uint4*>(coarse + ((long)(i / 2) * Wc + j / 2) * C + 8 * c8), fc);
+        unpack8(*reinterpret_cast<const uint4*>(skip + 8 * q), fs);
+#pragma unroll
+        for (int t = 0; t < 8; t++) fc[t] = fc[t] + fs[t];
+        *reinterpret_cast<uint4*>(out + 8 * q) = pack8(fc);
     }
 }
 
@@ -261,7 +365,8 @@ struct CoefPack {
     double c[32];
 };
 
-// NB: the accumulator count (>= nb), so that they stay in registers.
+// NB: the accumulator count (>= nb), so that they stay in registers.  Pairs
+// of entries per load; an odd last entry is added by thread 0 of block 0.
 template <int NB>
 __global__ void __launch_bounds__(256) k_multidot(long n, int nb, const double* __restrict__ a, PtrPack B,
                                                   double* __restrict__ part) {
@@ -269,15 +374,47 @@ __global__ void __launch_bounds__(256) k_multidot(long n, int nb, const double* 
     double acc[NB];
 #pragma unroll
     for (int j = 0; j < NB; j++) acc[j] = 0.0;
-    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-        const double ai = a[i];
+    const long n2 = n >> 1;
+    const double2* a2 = reinterpret_cast<const double2*>(a);
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n2; i += (long)gridDim.x * blockDim.x) {
+        const double2 ai = a2[i];
 #pragma unroll
         for (int j = 0; j < NB; j++)
-            if (j < nb) acc[j] += ai * B.p[j][i];
+            if (j < nb) {
+                const double2 bi = reinterpret_cast<const double2*>(B.p[j])[i];
+                acc[j] += ai.x * bi.x;
+                acc[j] += ai.y * bi.y;
+            }
+    }
+    if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) {
+#pragma unroll
+        for (int j = 0; j < NB; j++)
+            if (j < nb) acc[j] += a[n - 1] * B.p[j][n - 1];
     }
     block_sum<NB>(acc, lds);
     if (threadIdx.x == 0)
         for (int j = 0; j < nb; j++) part[blockIdx.x * nb + j] = acc[j];
+}
+
+// Which partials make up dot d of a batch: part[off[d] + b * nb[d] + q[d]]
+// for b < g.
+struct DotIdx {
+    int off[128];
+    short nb[128], q[128];
+};
+
+// out[d] = the sum of dot d's g partials, in a fixed order (one workgroup
+// per dot)
+__global__ void __launch_bounds__(256) k_multidot_final(const double* __restrict__ part, int g, DotIdx idx,
+                                                        double* __restrict__ out) {
+    __shared__ double lds[16];
+    const int d = blockIdx.x;
+    const double* p = part + idx.off[d] + idx.q[d];
+    const int stride = idx.nb[d];
+    double v[1] = {0.0};
+    for (int b = threadIdx.x; b < g; b += blockDim.x) v[0] += p[(long)b * stride];
+    block_sum<1>(v, lds);
+    if (threadIdx.x == 0) out[d] = v[0];
 }
 
 // The direction and its image without an operator application: with
