@@ -625,7 +625,9 @@ struct LearnedNet {
     // flexible CG: stored directions and their operator images
     int fcg_m = 0;
     std::vector<double*> fP, fQ;
-    double* mpart = nullptr;  // multi-dot partials (grid x dots), then the sums
+    double* mpart = nullptr;  // multi-dot partials (grid x dots), then three blocks of sums
+    FcgDev* fcg = nullptr;       // device state of the flexible CG
+    FcgDev* fcg_host = nullptr;  // two pinned copies of it
 };
 
 namespace {
@@ -638,6 +640,7 @@ long lvl_px(LearnedNet* N, int lv) { return (long)N->H[lv] * N->Wd[lv]; }
 void net_free(LearnedNet* N) {
     if (!N) return;
     for (void* p : N->allocs) hipFree(p);
+    if (N->fcg_host) hipHostFree(N->fcg_host);
     delete N;
 }
 
@@ -725,7 +728,9 @@ int net_build(pf_ctx* ctx, uint64_t seed) {
     N->wpart_n = ((p0 + ppb - 1) / ppb) * maxw;
     PF_TRY(net_alloc(ctx, N, &N->wpart, N->wpart_n));
     const long nv = ctx->nv;
-    PF_TRY(net_alloc(ctx, N, &N->mpart, (long)kMpartGrid * kMpartDots + kMpartDots));
+    PF_TRY(net_alloc(ctx, N, &N->mpart, (long)kMpartGrid * kMpartDots + 3 * kMpartDots));
+    PF_TRY(net_alloc(ctx, N, &N->fcg, 1));
+    PF_HIP(hipHostMalloc((void**)&N->fcg_host, 2 * sizeof(FcgDev)));
     double** sc[] = {&N->w, &N->Kw, &N->gd, &N->rr, &N->ee, &N->dd};
     for (double** q : sc) PF_TRY(net_alloc(ctx, N, q, nv));
     return sync(ctx);
@@ -1270,11 +1275,13 @@ int optimal_pair(pf_ctx* ctx, const double* r, double* a, double* c);
 // and N->gd K t on return.
 // The pieces of M(r) (stream-ordered, no host synchronisation): N->w = y,
 // N->dd = t, N->gd = K t, N->Kw = K y.
-int net_parts(pf_ctx* ctx, const double* rho, const double* r, double rn) {
+// rn: |r| on the host, or (rn_dev non-null) on the device.
+int net_parts(pf_ctx* ctx, const double* rho, const double* r, double rn, const double* rn_dev = nullptr) {
     LearnedNet* N = ctx->net;
     const long nv = ctx->nv;
-    const double rs = rn > 0 ? rn / std::sqrt((double)std::max(1, ctx->N)) : 1.0;
-    k_net_input<<<gsize(ctx->nn), kBlock, 0, ctx->stream>>>(ctx->nx, ctx->ny, rho, r, rs, N->x0);
+    const double sqrtN = std::sqrt((double)std::max(1, ctx->N));
+    const double rs = rn > 0 ? rn / sqrtN : 1.0;
+    k_net_input<<<gsize(ctx->nn), kBlock, 0, ctx->stream>>>(ctx->nx, ctx->ny, rho, r, rs, rn_dev, sqrtN, N->x0);
     PF_TRY(net_forward(ctx));
     k_net_y_nodal<<<gsize(ctx->nn), kBlock, 0, ctx->stream>>>(ctx->nn, N->y, ctx->d_fix, N->w);
     if (ctx->mg)
@@ -1284,19 +1291,6 @@ int net_parts(pf_ctx* ctx, const double* rho, const double* r, double rn) {
     PF_TRY(apply_nodal(ctx, false, N->dd, N->gd, true));                     // K t
     PF_TRY(apply_nodal(ctx, false, N->w, N->Kw, true));                      // K y
     return check_launch(ctx);
-}
-
-// (a, c) minimising ||a t + c y - e||_K from the five sums t.Kt, t.Ky, t.r,
-// y.Ky, y.r; the best-scaled t alone when the 2x2 Gram matrix is singular.
-void pair_from_sums(double tkt, double tky, double tr, double yky, double yr, double* a, double* c) {
-    const double det = tkt * yky - tky * tky;
-    if (std::isfinite(det) && det > 1e-12 * tkt * yky && yky > 0.0) {
-        *a = (yky * tr - tky * yr) / det;
-        *c = (tkt * yr - tky * tr) / det;
-    } else {
-        *a = tkt > 0.0 ? tr / tkt : 1.0;
-        *c = 0.0;
-    }
 }
 
 int net_apply(pf_ctx* ctx, const double* rho, const double* r, double* d, double* rs_out, double rnorm = -1.0) {
@@ -1320,26 +1314,20 @@ double dot_nodal(pf_ctx* ctx, const double* a, const double* b, int* rc) {
     return reduce_host(ctx, ctx->red.partials, (int)g, 1.0, 0.0, rc);
 }
 
-// (a, c) minimising ||a t + c y - e||_K with t = N->dd, y = N->w, K t = N->gd,
-// K y = N->Kw and K e = r; falls back to the best-scaled t alone when the
-// 2x2 Gram matrix is singular.
 // Several sets of dot products (a_k . b_kj) in one batch: the partial-sum
-// launches (at most 32 vectors each), one launch that reduces every dot's
-// partials on the device in a fixed order, one copy of the sums and one host
-// synchronisation.
+// launches (at most 32 vectors each) and one launch that reduces every dot's
+// partials on the device in a fixed order; multidot_batch then copies the
+// sums to the host (one synchronisation).
 struct DotJob {
     const double* a;
     std::vector<const double*> b;
     std::vector<double> out;
 };
-int multidot_batch(pf_ctx* ctx, std::vector<DotJob>& jobs) {
+// The launches of a batch, with the sums left in `sums` on the device, in
+// job order (nd of them).
+int multidot_enqueue(pf_ctx* ctx, std::vector<DotJob>& jobs, double* sums, int* nd_out) {
     LearnedNet* N = ctx->net;
     const unsigned g = std::min<unsigned>(gsize(ctx->nv / 2), (unsigned)kMpartGrid);
-    struct Chunk {
-        int job, first, nb;
-        long off;
-    };
-    std::vector<Chunk> ch;
     DotIdx idx{};
     long off = 0;
     int nd = 0;
@@ -1349,37 +1337,42 @@ int multidot_batch(pf_ctx* ctx, std::vector<DotJob>& jobs) {
         for (int f = 0; f < nbj; f += 32) {
             const int nb = std::min(32, nbj - f);
             PF_CHECK(nd + nb <= kMpartDots, PF_ERR_ARG, "too many dot products in one batch");
-            ch.push_back({j, f, nb, off});
-            for (int q = 0; q < nb; q++, nd++) {
+            PtrPack P{};
+            for (int q = 0; q < nb; q++) {
+                P.p[q] = jobs[j].b[f + q];
                 idx.off[nd] = (int)off;
                 idx.nb[nd] = (short)nb;
                 idx.q[nd] = (short)q;
+                nd++;
             }
+            double* part = N->mpart + off;
+            const double* a = jobs[j].a;
+            if (nb <= 4)
+                k_multidot<4><<<g, kBlock, 0, ctx->stream>>>(ctx->nv, nb, a, P, part);
+            else if (nb <= 8)
+                k_multidot<8><<<g, kBlock, 0, ctx->stream>>>(ctx->nv, nb, a, P, part);
+            else
+                k_multidot<32><<<g, kBlock, 0, ctx->stream>>>(ctx->nv, nb, a, P, part);
             off += (long)g * nb;
         }
     }
-    if (nd == 0) return PF_OK;
-    for (const Chunk& c : ch) {
-        PtrPack P{};
-        for (int q = 0; q < c.nb; q++) P.p[q] = jobs[c.job].b[c.first + q];
-        const double* a = jobs[c.job].a;
-        double* part = N->mpart + c.off;
-        if (c.nb <= 4)
-            k_multidot<4><<<g, kBlock, 0, ctx->stream>>>(ctx->nv, c.nb, a, P, part);
-        else if (c.nb <= 8)
-            k_multidot<8><<<g, kBlock, 0, ctx->stream>>>(ctx->nv, c.nb, a, P, part);
-        else
-            k_multidot<32><<<g, kBlock, 0, ctx->stream>>>(ctx->nv, c.nb, a, P, part);
-    }
+    if (nd > 0) k_multidot_final<<<nd, kBlock, 0, ctx->stream>>>(N->mpart, (int)g, idx, sums);
+    *nd_out = nd;
+    return check_launch(ctx);
+}
+
+int multidot_batch(pf_ctx* ctx, std::vector<DotJob>& jobs) {
+    LearnedNet* N = ctx->net;
     double* sums = N->mpart + (long)kMpartGrid * kMpartDots;
-    k_multidot_final<<<nd, kBlock, 0, ctx->stream>>>(N->mpart, (int)g, idx, sums);
-    PF_TRY(check_launch(ctx));
+    int nd;
+    PF_TRY(multidot_enqueue(ctx, jobs, sums, &nd));
+    if (nd == 0) return PF_OK;
     std::vector<double> h(nd);
     PF_HIP(hipMemcpyAsync(h.data(), sums, sizeof(double) * nd, hipMemcpyDeviceToHost, ctx->stream));
     PF_TRY(sync(ctx));
     int d = 0;
-    for (const Chunk& c : ch)
-        for (int q = 0; q < c.nb; q++) jobs[c.job].out[c.first + q] = h[d++];
+    for (DotJob& jb : jobs)
+        for (double& o : jb.out) o = h[d++];
     return PF_OK;
 }
 
@@ -1393,6 +1386,9 @@ int multidot(pf_ctx* ctx, const double* a, const double* const* b, int nb, doubl
     return PF_OK;
 }
 
+// (a, c) minimising ||a t + c y - e||_K with t = N->dd, y = N->w, K t = N->gd,
+// K y = N->Kw and K e = r; falls back to the best-scaled t alone when the
+// 2x2 Gram matrix is singular.
 int optimal_pair(pf_ctx* ctx, const double* r, double* a, double* c) {
     LearnedNet* N = ctx->net;
     double d1[3], d2[2];
@@ -1476,7 +1472,8 @@ int net_train_step(pf_ctx* ctx, int s, float lr, double* loss) {
     const double rn = norm_nodal(ctx, N->rr, &rc);
     PF_TRY(rc);
     const double rs = rn > 0 ? rn / std::sqrt((double)std::max(1, ctx->N)) : 1.0;
-    k_net_input<<<gsize(ctx->nn), kBlock, 0, ctx->stream>>>(ctx->nx, ctx->ny, ctx->d_rho, N->rr, rs, N->x0);
+    k_net_input<<<gsize(ctx->nn), kBlock, 0, ctx->stream>>>(ctx->nx, ctx->ny, ctx->d_rho, N->rr, rs, nullptr, 1.0,
+                                                            N->x0);
     PF_TRY(net_forward(ctx));
     k_net_y_nodal<<<gsize(ctx->nn), kBlock, 0, ctx->stream>>>(ctx->nn, N->y, ctx->d_fix, N->w);   // y
     PF_TRY(apply_nodal(ctx, false, N->w, N->Kw, true));                                           // K y
@@ -1570,7 +1567,6 @@ int pcg_learned(pf_ctx* ctx, const double* b, double* x, double eps, int itrmax,
         }
         N->fcg_m = m;
     }
-    std::vector<double> pq(m, 0.0);
     int rc;
     PF_TRY(diag_nodal(ctx, false));
     if (ctx->mg) PF_TRY(mg_build(ctx));
@@ -1579,96 +1575,117 @@ int pcg_learned(pf_ctx* ctx, const double* b, double* x, double eps, int itrmax,
     double *r = ctx->d_r, *z = ctx->d_z;
     PF_HIP(hipMemsetAsync(x, 0, sizeof(double) * nv, ctx->stream));
     PF_HIP(hipMemcpyAsync(r, b, sizeof(double) * nv, hipMemcpyDeviceToDevice, ctx->stream));
-    double rn = bn, best = bn;
-    int since_best = 0, k = 0, status = 2, fallbacks = 0;
+    FcgDev* S = N->fcg;
+    FcgDev* H = N->fcg_host;
+    k_fcg_init<<<1, 1, 0, ctx->stream>>>(S, bn, eps);
+    double* sums1 = N->mpart + (long)kMpartGrid * kMpartDots + kMpartDots;
+    double* sums2 = sums1 + kMpartDots;
+    const unsigned gs = gsize(nv);
+    PF_CHECK(gs <= (unsigned)kMaxPartials, PF_ERR_ARG, "grid too large for the reduction scratch");
     const int stall = 2000;
-    bool fallback = false;
-    for (; k < itrmax; k++) {
+    hipEvent_t ev[2];
+    PF_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+    PF_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+    // iteration k, stream-ordered: t, y and their images; one batch for the
+    // pair (a, c) and z.q_j = a t.q_j + c y.q_j (z = a t + c y is never
+    // formed); the direction; p.q and p.r; the step with |r|; the decisions;
+    // a copy of the state into pinned slot k mod 2
+    auto enqueue = [&](int k) -> int {
         const int nst = std::min(k, m);
         const int slot = k % m;
         double* p = N->fP[slot];
         double* q = N->fQ[slot];
-        CoefPack cf{};
         PtrPack Pp{}, Qp{};
-        std::vector<const double*> qs(nst);
+        std::vector<DotJob> jobs(2);
+        jobs[0].a = N->dd;
+        jobs[0].b = {N->gd, N->Kw, r};
+        jobs[1].a = N->w;
+        jobs[1].b = {N->Kw, r};
         for (int j = 0; j < nst; j++) {
             const int sj = (k - 1 - j + m) % m;
-            qs[j] = N->fQ[sj];
             Pp.p[j] = N->fP[sj];
             Qp.p[j] = N->fQ[sj];
+            jobs[0].b.push_back(N->fQ[sj]);
+            jobs[1].b.push_back(N->fQ[sj]);
         }
-        {
-            // t, y and their images, then one batch for the pair (a, c) and for
-            // z.q_j = a t.q_j + c y.q_j (z = a t + c y is never formed)
-            PF_TRY(net_parts(ctx, ctx->d_rho, r, rn));
-            std::vector<DotJob> jobs(2);
-            jobs[0].a = N->dd;
-            jobs[0].b = {N->gd, N->Kw, r};
-            jobs[1].a = N->w;
-            jobs[1].b = {N->Kw, r};
-            for (const double* qj : qs) {
-                jobs[0].b.push_back(qj);
-                jobs[1].b.push_back(qj);
-            }
-            PF_TRY(multidot_batch(ctx, jobs));
-            const std::vector<double>& T = jobs[0].out;
-            const std::vector<double>& Y = jobs[1].out;
-            double a, c;
-            pair_from_sums(T[0], T[1], T[2], Y[0], Y[1], &a, &c);
-            for (int j = 0; j < nst; j++) cf.c[j] = (a * T[3 + j] + c * Y[2 + j]) / pq[(k - 1 - j + m) % m];
-            // p = z - sum_j c_j p_j and q = K p = K z - sum_j c_j q_j
-            k_fcg_direction2<<<gsize(nv), kBlock, 0, ctx->stream>>>(nv, nst, a, c, N->dd, N->w, N->gd, N->Kw, Pp,
-                                                                    Qp, cf, p, q);
-        }
-        // p.q, p.r and, for the new residual's norm, q.q, q.r
-        std::vector<DotJob> step(2);
+        PF_TRY(net_parts(ctx, ctx->d_rho, r, 0.0, &S->rn));
+        int nd;
+        PF_TRY(multidot_enqueue(ctx, jobs, sums1, &nd));
+        k_fcg_coef<<<1, 64, 0, ctx->stream>>>(S, sums1, k, m, nst);
+        k_fcg_direction<<<gsize(nv), kBlock, 0, ctx->stream>>>(nv, nst, S, N->dd, N->w, N->gd, N->Kw, Pp, Qp, p, q);
+        std::vector<DotJob> step(1);
         step[0].a = p;
         step[0].b = {q, r};
-        step[1].a = q;
-        step[1].b = {q, r};
-        PF_TRY(multidot_batch(ctx, step));
-        const double pqv = step[0].out[0], pr = step[0].out[1], qq = step[1].out[0], qr = step[1].out[1];
-        if (!(pqv > 0.0) || !std::isfinite(pr)) {  // breakdown: M(r) is not a descent direction
+        PF_TRY(multidot_enqueue(ctx, step, sums2, &nd));
+        k_fcg_step<<<gs, kBlock, 0, ctx->stream>>>(nv, S, sums2, p, q, x, r, ctx->red.partials);
+        k_fcg_check<<<1, kBlock, 0, ctx->stream>>>(S, sums2, ctx->red.partials, (int)gs, slot, stall);
+        PF_HIP(hipMemcpyAsync(&H[k & 1], S, sizeof(FcgDev), hipMemcpyDeviceToHost, ctx->stream));
+        PF_HIP(hipEventRecord(ev[k & 1], ctx->stream));
+        return check_launch(ctx);
+    };
+    // iteration k is enqueued while k - 1 runs; the host then reads k - 1's
+    // state (once done is set, the FCG launches of a later iteration change
+    // nothing)
+    int k = 0, kq = 0, status = 2, fallbacks = 0;
+    double rn = bn;
+    bool fallback = false;
+    std::vector<int> pend;
+    int erc = PF_OK;
+    for (;;) {
+        if (kq < itrmax) {
+            erc = enqueue(kq);
+            if (erc != PF_OK) break;
+            pend.push_back(kq++);
+        }
+        if (pend.empty()) break;  // itrmax reached
+        if (pend.size() < 2 && kq < itrmax) continue;
+        const int kw = pend.front();
+        pend.erase(pend.begin());
+        if (hipEventSynchronize(ev[kw & 1]) != hipSuccess) {
+            erc = PF_ERR_HIP;
+            break;
+        }
+        const FcgDev st_k = H[kw & 1];
+        k = st_k.k;
+        rn = st_k.rn;
+        if (!st_k.done) continue;
+        if (st_k.flag == 2) {
             fallback = true;
             break;
         }
-        pq[slot] = pqv;
-        const double alpha = pr / pqv;
-        k_step2<<<gsize(nv), kBlock, 0, ctx->stream>>>(nv, alpha, p, q, x, r);
-        // |r - alpha q|^2 = |r|^2 - 2 alpha q.r + alpha^2 q.q; computed directly
-        // when that cancels (a step that removes nearly all of r)
-        const double rr_new = rn * rn - 2.0 * alpha * qr + alpha * alpha * qq;
-        if (rr_new > 1e-4 * rn * rn) {
-            rn = std::sqrt(rr_new);
-        } else {
-            rn = norm_nodal(ctx, r, &rc);
-            PF_TRY(rc);
-        }
-        if (!std::isfinite(rn)) {
-            fallback = true;
-            break;
-        }
-        if (rn <= eps * bn) {
-            // confirm on the true residual
-            double tn;
-            PF_TRY(true_residual(ctx, b, x, ctx->d_t0, &tn));
-            if (tn <= eps * bn) {
-                status = 1;
-                rn = tn;
-                k++;
-                break;
-            }
-            PF_HIP(hipMemcpyAsync(r, ctx->d_t0, sizeof(double) * nv, hipMemcpyDeviceToDevice, ctx->stream));
+        // confirm on the true residual
+        PF_HIP(hipStreamSynchronize(ctx->stream));
+        double tn;
+        PF_TRY(true_residual(ctx, b, x, ctx->d_t0, &tn));
+        if (tn <= eps * bn) {
+            status = 1;
             rn = tn;
+            break;
         }
-        if (rn < 0.5 * best) {
-            best = rn;
-            since_best = 0;
-        } else if (++since_best >= stall || rn > 10.0 * best) {
+        PF_HIP(hipMemcpyAsync(r, ctx->d_t0, sizeof(double) * nv, hipMemcpyDeviceToDevice, ctx->stream));
+        double best = st_k.best;
+        int since = st_k.since_best;
+        bool fb = false;
+        if (tn < 0.5 * best) {
+            best = tn;
+            since = 0;
+        } else if (++since >= stall || tn > 10.0 * best) {
+            fb = true;
+        }
+        k_fcg_resume<<<1, 1, 0, ctx->stream>>>(S, tn, best, since, fb ? 1 : 0);
+        rn = tn;
+        if (fb) {
+            k -= 1;
             fallback = true;
             break;
         }
+        pend.clear();
+        kq = k;
     }
+    hipStreamSynchronize(ctx->stream);
+    hipEventDestroy(ev[0]);
+    hipEventDestroy(ev[1]);
+    PF_TRY(erc);
     if (fallback) {
         // Jacobi-PCG on the correction K dx = b - K x from the current iterate
         // (from x = 0 when the iterate is not finite)

@@ -333,8 +333,11 @@ __global__ void k_copy_bf16(long n, const bf16* __restrict__ a, bf16* __restrict
 // ------------------------------------------------------- network input ----
 // x0[p] = (2 rho_node - 1, r_x / s, r_y / s, 0 ...), 16 channels, where
 // rho_node is the mean density of the node's elements and s the RMS of r.
+// rs scales the residual channels; with rn_dev the scale is *rn_dev / sqrtN
+// instead (1 when that is 0), read on the device.
 __global__ void k_net_input(int nx, int ny, const double* __restrict__ rho, const double* __restrict__ r, double rs,
-                            bf16* __restrict__ x0) {
+                            const double* __restrict__ rn_dev, double sqrtN, bf16* __restrict__ x0) {
+    if (rn_dev) rs = *rn_dev > 0 ? *rn_dev / sqrtN : 1.0;
     const long nn = (long)(nx + 1) * (ny + 1);
     for (long n = blockIdx.x * (long)blockDim.x + threadIdx.x; n < nn; n += (long)gridDim.x * blockDim.x) {
         int i = (int)(n / (ny + 1)), j = (int)(n - (long)i * (ny + 1));
@@ -417,33 +420,161 @@ __global__ void __launch_bounds__(256) k_multidot_final(const double* __restrict
     if (threadIdx.x == 0) out[d] = v[0];
 }
 
+// ------------------------------------------- device-resident flexible CG ---
+// The learned flexible CG keeps its scalars on the device, so the host
+// enqueues iteration k while iteration k-1 still runs and reads a copy of
+// this state one iteration behind.  flag: 0 running; 1 the recursive
+// residual fell below eps |b| (the host confirms on the true residual);
+// 2 fall back to Jacobi-PCG (breakdown, a non-finite residual, divergence
+// or stall).  k counts the iterations completed, as the host loop counted
+// them; once done is set every later FCG launch leaves x, r and the stored
+// directions alone.
+struct FcgDev {
+    double rn, bn, best, eps, a, c;
+    double pq[32], cf[32];
+    int k, done, flag, since_best;
+};
+
+__global__ void k_fcg_init(FcgDev* __restrict__ S, double bn, double eps) {
+    S->rn = bn;
+    S->bn = bn;
+    S->best = bn;
+    S->eps = eps;
+    S->a = 1.0;
+    S->c = 0.0;
+    S->k = 0;
+    S->done = 0;
+    S->flag = 0;
+    S->since_best = 0;
+}
+
+// after a recursive residual below eps |b| that the true residual tn did not
+// confirm: r has been replaced by the true residual; continue from it
+__global__ void k_fcg_resume(FcgDev* __restrict__ S, double tn, double best, int since_best, int fallback) {
+    S->rn = tn;
+    S->best = best;
+    S->since_best = since_best;
+    S->flag = fallback ? 2 : 0;
+    S->done = fallback ? 1 : 0;
+    if (fallback) S->k -= 1;  // that iteration is not counted, as in a stall exit
+}
+
+// (a, c) minimising ||a t + c y - e||_K from the five sums t.Kt, t.Ky, t.r,
+// y.Ky, y.r; the best-scaled t alone when the 2x2 Gram matrix is singular.
+__host__ __device__ inline void pair_from_sums(double tkt, double tky, double tr, double yky, double yr, double* a,
+                                               double* c) {
+    const double det = tkt * yky - tky * tky;
+    if (__builtin_isfinite(det) && det > 1e-12 * tkt * yky && yky > 0.0) {
+        *a = (yky * tr - tky * yr) / det;
+        *c = (tkt * yr - tky * tr) / det;
+    } else {
+        *a = tkt > 0.0 ? tr / tkt : 1.0;
+        *c = 0.0;
+    }
+}
+
+// Iteration k's pair (a, c) and orthogonalisation weights
+// c_j = z.q_j / p_j.q_j (z = a t + c y) from the first batch of sums:
+// T = (t.Kt, t.Ky, t.r, t.q_1..t.q_nst), then Y = (y.Ky, y.r, y.q_1..y.q_nst);
+// q_j is the direction stored j + 1 iterations back (slot (k-1-j) mod m).
+__global__ void k_fcg_coef(FcgDev* __restrict__ S, const double* __restrict__ sums, int k, int m, int nst) {
+    if (S->done || threadIdx.x != 0) return;
+    const double* T = sums;
+    const double* Y = sums + 3 + nst;
+    double a, c;
+    pair_from_sums(T[0], T[1], T[2], Y[0], Y[1], &a, &c);
+    S->a = a;
+    S->c = c;
+    for (int j = 0; j < nst; j++) S->cf[j] = (a * T[3 + j] + c * Y[2 + j]) / S->pq[(k - 1 - j + m) % m];
+}
+
 // The direction and its image without an operator application: with
 // z = a t + c y and K z = a Kt + c Ky,
 //   p = z - sum_j c_j P_j,  q = K p = K z - sum_j c_j Q_j.
-__global__ void __launch_bounds__(256) k_fcg_direction2(long n, int m, double a, double c, const double* __restrict__ t,
-                                                        const double* __restrict__ y, const double* __restrict__ Kt,
-                                                        const double* __restrict__ Ky, PtrPack P, PtrPack Q,
-                                                        CoefPack cf, double* __restrict__ p, double* __restrict__ q) {
+__global__ void __launch_bounds__(256) k_fcg_direction(long n, int nst, const FcgDev* __restrict__ S,
+                                                       const double* __restrict__ t, const double* __restrict__ y,
+                                                       const double* __restrict__ Kt, const double* __restrict__ Ky,
+                                                       PtrPack P, PtrPack Q, double* __restrict__ p,
+                                                       double* __restrict__ q) {
+    if (S->done) return;
+    const double a = S->a, c = S->c;
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
         double v = a * t[i] + c * y[i];
         double w = a * Kt[i] + c * Ky[i];
-        for (int j = 0; j < m; j++) {
-            v -= cf.c[j] * P.p[j][i];
-            w -= cf.c[j] * Q.p[j][i];
+        for (int j = 0; j < nst; j++) {
+            v -= S->cf[j] * P.p[j][i];
+            w -= S->cf[j] * Q.p[j][i];
         }
         p[i] = v;
         q[i] = w;
     }
 }
 
-// x += alpha p, r -= alpha q
-__global__ void __launch_bounds__(256) k_step2(long n, double alpha, const double* __restrict__ p,
-                                               const double* __restrict__ q, double* __restrict__ x,
-                                               double* __restrict__ r) {
+// x += alpha p, r -= alpha q with alpha = p.r / p.q (sums = (p.q, p.r)), and
+// per-workgroup partial sums of the new r.r; nothing on a breakdown (p.q <= 0
+// or a non-finite p.r), which k_fcg_check reports
+__global__ void __launch_bounds__(256) k_fcg_step(long n, const FcgDev* __restrict__ S,
+                                                  const double* __restrict__ sums, const double* __restrict__ p,
+                                                  const double* __restrict__ q, double* __restrict__ x,
+                                                  double* __restrict__ r, double* __restrict__ part) {
+    __shared__ double lds[16];
+    if (S->done) return;
+    const double pqv = sums[0], pr = sums[1];
+    if (!(pqv > 0.0) || !isfinite(pr)) return;
+    const double alpha = pr / pqv;
+    double acc[1] = {0.0};
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
         x[i] += alpha * p[i];
-        r[i] -= alpha * q[i];
+        const double ri = r[i] - alpha * q[i];
+        r[i] = ri;
+        acc[0] += ri * ri;
     }
+    block_sum<1>(acc, lds);
+    if (threadIdx.x == 0) part[blockIdx.x] = acc[0];
+}
+
+// |r| from k_fcg_step's partials (fixed order) and the decisions the host
+// loop made: breakdown, non-finite, converged (to be confirmed), stall or
+// divergence (no halving of the best |r| in `stall` iterations, or 10x above it)
+__global__ void __launch_bounds__(256) k_fcg_check(FcgDev* __restrict__ S, const double* __restrict__ sums,
+                                                   const double* __restrict__ part, int nb, int slot, int stall) {
+    __shared__ double lds[16];
+    if (S->done) return;
+    const double pqv = sums[0], pr = sums[1];
+    const bool breakdown = !(pqv > 0.0) || !isfinite(pr);
+    double v[1] = {0.0};
+    if (!breakdown)
+        for (int b = threadIdx.x; b < nb; b += blockDim.x) v[0] += part[b];
+    block_sum<1>(v, lds);
+    if (threadIdx.x != 0) return;
+    if (breakdown) {
+        S->flag = 2;
+        S->done = 1;
+        return;
+    }
+    S->pq[slot] = pqv;
+    const double rn = sqrt(v[0]);
+    if (!isfinite(rn)) {
+        S->flag = 2;
+        S->done = 1;
+        return;
+    }
+    S->rn = rn;
+    if (rn <= S->eps * S->bn) {
+        S->k += 1;
+        S->flag = 1;
+        S->done = 1;
+        return;
+    }
+    if (rn < 0.5 * S->best) {
+        S->best = rn;
+        S->since_best = 0;
+    } else if (++S->since_best >= stall || rn > 10.0 * S->best) {
+        S->flag = 2;
+        S->done = 1;
+        return;
+    }
+    S->k += 1;
 }
 
 // y channels 0, 1 -> nodal FP64 vector (0 on fixed DOFs)
