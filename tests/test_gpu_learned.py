@@ -21,8 +21,11 @@ def relerr(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
 
 
-def test_network_forward_backward_against_cpu(pa):
-    nx, ny = 32, 16
+@pytest.mark.parametrize("nx,ny", [(32, 16), (150, 130)])
+def test_network_forward_backward_against_cpu(pa, nx, ny):
+    """33 x 17 nodes fit one 4 x 64-pixel convolution tile per few rows;
+    151 x 131 spans three tile columns (the last partial) and a partial last
+    tile row, so the tiles' halo exchange is covered too."""
     ctx = pa.cantilever2d(nx, ny)
     ctx.precond_init(seed=5)
     rng = np.random.default_rng(9)
@@ -136,3 +139,21 @@ def test_fallback_when_network_is_useless(pa):
     assert st["status"] == 1 and st["true_residual"] <= 1.0000001e-8
     xj, _ = ctx.pcg(b, op=pa.PF_OP_EBE64, eps=1e-12)
     assert relerr(x, xj) < 1e-5
+
+
+def test_learned_itrmax_exit(pa):
+    """The device-resident flexible CG stops after itrmax iterations with
+    status 2 (not converged) and the last iterate, whose residual it reports."""
+    nx, ny = 96, 48
+    ctx = pa.cantilever2d(nx, ny)
+    ctx.precond_init(seed=3)
+    rng = np.random.default_rng(6)
+    ctx.precond_load((rng.standard_normal(cn.num_params()) * 0.05).astype(np.float32))
+    ctx.set_density(rng.uniform(0.2, 1.0, nx * ny))
+    b = ctx.load()
+    x, st = ctx.pcg(b, op=pa.PF_OP_LEARNED, eps=1e-8, itrmax=5)
+    assert st["status"] == 2 and st["iterations"] == 5 and st["fallbacks"] == 0
+    assert np.all(np.isfinite(x)) and 0.0 < st["rel_residual"] < 1.0
+    # the carried residual is the true one up to rounding
+    assert abs(st["rel_residual"] - st["true_residual"]) <= 1e-6 * st["true_residual"]
+    ctx.close()
