@@ -270,6 +270,20 @@ struct SnapHook {
     std::vector<int> iters;
 };
 
+// Two events (no timing), destroyed on every return path.
+struct EventPair {
+    hipEvent_t e[2] = {nullptr, nullptr};
+    hipError_t create() {
+        hipError_t r = hipEventCreateWithFlags(&e[0], hipEventDisableTiming);
+        return r != hipSuccess ? r : hipEventCreateWithFlags(&e[1], hipEventDisableTiming);
+    }
+    hipEvent_t& operator[](int i) { return e[i]; }
+    ~EventPair() {
+        for (hipEvent_t x : e)
+            if (x) hipEventDestroy(x);
+    }
+};
+
 // Workgroups of k_cg1_q4_tile: full tiles, then edge tiles (launched first).
 Cg1Tiles cg1_tiles(const pf_ctx* ctx, unsigned* grid, int ti = kCg1TI, int tj = kCg1TJ) {
     Cg1Tiles G;
@@ -316,9 +330,8 @@ int pcg_device_cg1(pf_ctx* ctx, const T* b, T* x, const T* D, T* r, T* z, T* p0,
     const int chunk = 32;
     const int mmax = itrmax + 1;  // launch m finishes iterate m; iterate itrmax is the last
     int m = 0;
-    hipEvent_t ev[2];
-    PF_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
-    PF_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+    EventPair ev;
+    PF_HIP(ev.create());
     volatile int* hflag = (volatile int*)ctx->h_red;
     hflag[0] = hflag[1] = 0;
     int slot = 0, pending = 0;
@@ -348,8 +361,6 @@ int pcg_device_cg1(pf_ctx* ctx, const T* b, T* x, const T* D, T* r, T* z, T* p0,
         slot ^= 1;
     }
     PF_HIP(hipStreamSynchronize(ctx->stream));
-    hipEventDestroy(ev[0]);
-    hipEventDestroy(ev[1]);
     k_cg1_final<<<1, kBlock, 0, ctx->stream>>>(m, part[(m - 1) & 1], (int)g1, S, eps);
     PF_TRY(check_launch(ctx));
     PF_HIP(hipMemcpyAsync(ctx->h_pcg, S, sizeof(PcgDev), hipMemcpyDeviceToHost, ctx->stream));
@@ -383,9 +394,8 @@ int pcg_device(pf_ctx* ctx, bool csr, const T* b, T* x, const T* D, T* r, T* z, 
     const int chunk = 32;
     int k = 0;
     // one chunk is kept in flight while the previous chunk's flag is read
-    hipEvent_t ev[2];
-    PF_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
-    PF_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+    EventPair ev;
+    PF_HIP(ev.create());
     volatile int* hflag = (volatile int*)ctx->h_red;  // two ints: done flags of chunks in flight
     hflag[0] = hflag[1] = 0;
     int slot = 0, pending = 0;
@@ -438,8 +448,6 @@ int pcg_device(pf_ctx* ctx, bool csr, const T* b, T* x, const T* D, T* r, T* z, 
     }
     // drain
     PF_HIP(hipStreamSynchronize(ctx->stream));
-    hipEventDestroy(ev[0]);
-    hipEventDestroy(ev[1]);
     k_pcg_final<<<1, kBlock, 0, ctx->stream>>>(k, partB, (int)g2, S, eps);
     PF_TRY(check_launch(ctx));
     PF_HIP(hipMemcpyAsync(ctx->h_pcg, S, sizeof(PcgDev), hipMemcpyDeviceToHost, ctx->stream));
@@ -599,6 +607,8 @@ struct MgState {
     double* W = nullptr;    // inversion scratch
     std::vector<void*> owned;
 };
+// pixels per workgroup of a weight-gradient launch
+constexpr int kWgradPixels = 4096;
 // dot products one batched reduction holds (multidot_batch), and its grid
 constexpr int kMpartDots = 128;
 constexpr int kMpartGrid = 1024;
@@ -724,8 +734,7 @@ int net_build(pf_ctx* ctx, uint64_t seed) {
         {&N->gx1, p1 * 16}, {&N->ge0, p0 * 16}, {&N->ga0, p0 * 16}};
     for (auto& a : acts) PF_TRY(net_alloc(ctx, N, a.p, a.n));
     PF_TRY(net_alloc(ctx, N, &N->y, p0 * 16));
-    const int ppb = 4096;
-    N->wpart_n = ((p0 + ppb - 1) / ppb) * maxw;
+    N->wpart_n = ((p0 + kWgradPixels - 1) / kWgradPixels) * maxw;
     PF_TRY(net_alloc(ctx, N, &N->wpart, N->wpart_n));
     const long nv = ctx->nv;
     PF_TRY(net_alloc(ctx, N, &N->mpart, (long)kMpartGrid * kMpartDots + 3 * kMpartDots));
@@ -747,9 +756,13 @@ void conv_fwd_t(pf_ctx* ctx, int H, int W, const bf16* X, const bf16* Wt, bf16* 
     k_conv_fwd<CI, CO><<<grid, 256, 0, ctx->stream>>>(H, W, X, Wt, Y, Yf, relu);
 }
 
+// One workgroup per kWgradPixels pixels.  The trained network depends on the
+// gradients' rounding, and so does how well it preconditions (at 96x48 a
+// different split of the pixels, with the same loss to 1%, took flexible CG
+// from 405 to 1058 iterations), so the split is kept fixed.
 template <int CI, int CO>
 void conv_wgrad_t(pf_ctx* ctx, LearnedNet* N, int H, int W, const bf16* X, const bf16* dZ, float* dW) {
-    const int ppb = 4096;
+    const int ppb = kWgradPixels;
     const long npx = (long)H * W;
     const int nb = (int)((npx + ppb - 1) / ppb);
     k_conv_wgrad<CI, CO><<<nb, 256, 0, ctx->stream>>>(H, W, X, dZ, N->wpart, ppb);
@@ -1164,9 +1177,8 @@ int pcg_mg_device(pf_ctx* ctx, const double* b, double* x, double eps, int itrma
     PF_TRY(check_launch(ctx));
     const int chunk = 8;
     int k = 0;
-    hipEvent_t ev[2];
-    PF_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
-    PF_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+    EventPair ev;
+    PF_HIP(ev.create());
     volatile int* hflag = (volatile int*)ctx->h_red;
     hflag[0] = hflag[1] = 0;
     int slot = 0, pending = 0;
@@ -1207,8 +1219,6 @@ int pcg_mg_device(pf_ctx* ctx, const double* b, double* x, double eps, int itrma
     }
     M->done = nullptr;
     hipStreamSynchronize(ctx->stream);
-    hipEventDestroy(ev[0]);
-    hipEventDestroy(ev[1]);
     PF_TRY(rc);
     k_pcg_final<<<1, kBlock, 0, ctx->stream>>>(k, partB, (int)gz, S, eps);
     PF_TRY(check_launch(ctx));
@@ -1583,9 +1593,8 @@ int pcg_learned(pf_ctx* ctx, const double* b, double* x, double eps, int itrmax,
     const unsigned gs = gsize(nv);
     PF_CHECK(gs <= (unsigned)kMaxPartials, PF_ERR_ARG, "grid too large for the reduction scratch");
     const int stall = 2000;
-    hipEvent_t ev[2];
-    PF_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
-    PF_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+    EventPair ev;
+    PF_HIP(ev.create());
     // iteration k, stream-ordered: t, y and their images; one batch for the
     // pair (a, c) and z.q_j = a t.q_j + c y.q_j (z = a t + c y is never
     // formed); the direction; p.q and p.r; the step with |r|; the decisions;
@@ -1683,8 +1692,6 @@ int pcg_learned(pf_ctx* ctx, const double* b, double* x, double eps, int itrmax,
         kq = k;
     }
     hipStreamSynchronize(ctx->stream);
-    hipEventDestroy(ev[0]);
-    hipEventDestroy(ev[1]);
     PF_TRY(erc);
     if (fallback) {
         // Jacobi-PCG on the correction K dx = b - K x from the current iterate

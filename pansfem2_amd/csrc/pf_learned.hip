@@ -182,43 +182,75 @@ __global__ void __launch_bounds__(256) k_conv_fwd(int H, int W, const bf16* __re
 // -------------------------------------------------------- weight gradient --
 // dW[tap][co][ci] = sum_p dZ[p][co] X[p + off(tap)][ci]: per workgroup a
 // range of `ppb` pixels, FP32 partials per workgroup (reduced afterwards in
-// a fixed order).
+// a fixed order).  Wave w owns taps w, w + 4 and w + 8 (< 9) with all their
+// output and input channel blocks: per 16-pixel chunk a lane loads its four
+// pixels' dZ once and X once per tap and input block, and every (tap, output
+// block, input block) accumulator sums its MFMAs over the chunks in pixel
+// order.
 template <int CIN, int COUT>
 __global__ void __launch_bounds__(256) k_conv_wgrad(int H, int W, const bf16* __restrict__ X,
                                                     const bf16* __restrict__ dZ, float* __restrict__ part, int ppb) {
-    const int wave = threadIdx.x >> 6, l = threadIdx.x & 63, g = l >> 4;
+    const int wave = threadIdx.x >> 6, l = threadIdx.x & 63, g = l >> 4, c16 = l & 15;
     const long npx = (long)H * W;
     const long p_begin = (long)blockIdx.x * ppb;
     const long p_end = min(npx, p_begin + ppb);
-    constexpr int CT = COUT / 16, KT = CIN / 16, ITEMS = 9 * CT * KT;
+    constexpr int CT = COUT / 16, KT = CIN / 16, NT = 3;  // taps per wave (at most)
+    const int ntap = wave == 0 ? 3 : 2;
     float* out = part + (long)blockIdx.x * 9 * COUT * CIN;
-    for (int item = wave; item < ITEMS; item += 4) {
-        const int tap = item / (CT * KT), rem = item - tap * CT * KT, ct = rem / KT, kt = rem - ct * KT;
-        const int di = tap / 3 - 1, dj = tap % 3 - 1;
-        const int co = ct * 16 + (l & 15), ci = kt * 16 + (l & 15);
-        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-        for (long p0 = p_begin; p0 < p_end; p0 += 16) {
-            bf16x4 a, b;
+    f32x4 acc[NT][CT][KT];
 #pragma unroll
-            for (int t = 0; t < 4; t++) {
-                const long p = p0 + 4 * g + t;
-                short av = 0, bv = 0;
-                if (p < p_end) {
-                    av = (short)dZ[p * COUT + co];
-                    const int pi = (int)(p / W), pj = (int)(p - (long)pi * W);
-                    const int ii = pi + di, jj = pj + dj;
-                    if (ii >= 0 && ii < H && jj >= 0 && jj < W) bv = (short)X[((long)ii * W + jj) * CIN + ci];
-                }
-                a[t] = av;
-                b[t] = bv;
-            }
-            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
-        }
+    for (int h = 0; h < NT; h++)
+#pragma unroll
+        for (int ct = 0; ct < CT; ct++)
+#pragma unroll
+            for (int kt = 0; kt < KT; kt++) acc[h][ct][kt] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (long p0 = p_begin; p0 < p_end; p0 += 16) {
+        int pi[4], pj[4];
+        bool ok[4];
+        bf16x4 a[CT];
 #pragma unroll
         for (int t = 0; t < 4; t++) {
-            const int cor = ct * 16 + 4 * g + t;
-            out[((long)tap * COUT + cor) * CIN + ci] = acc[t];
+            const long p = p0 + 4 * g + t;
+            ok[t] = p < p_end;
+            pi[t] = ok[t] ? (int)(p / W) : 0;
+            pj[t] = ok[t] ? (int)(p - (long)pi[t] * W) : 0;
+#pragma unroll
+            for (int ct = 0; ct < CT; ct++) a[ct][t] = ok[t] ? (short)dZ[p * COUT + ct * 16 + c16] : (short)0;
         }
+#pragma unroll
+        for (int h = 0; h < NT; h++) {
+            if (h >= ntap) break;
+            const int tap = wave + 4 * h;
+            const int di = tap / 3 - 1, dj = tap % 3 - 1;
+#pragma unroll
+            for (int kt = 0; kt < KT; kt++) {
+                bf16x4 b;
+#pragma unroll
+                for (int t = 0; t < 4; t++) {
+                    const int ii = pi[t] + di, jj = pj[t] + dj;
+                    b[t] = (ok[t] && ii >= 0 && ii < H && jj >= 0 && jj < W)
+                               ? (short)X[((long)ii * W + jj) * CIN + kt * 16 + c16]
+                               : (short)0;
+                }
+#pragma unroll
+                for (int ct = 0; ct < CT; ct++)
+                    acc[h][ct][kt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[ct], b, acc[h][ct][kt], 0, 0, 0);
+            }
+        }
+    }
+#pragma unroll
+    for (int h = 0; h < NT; h++) {
+        if (h >= ntap) break;
+        const int tap = wave + 4 * h;
+#pragma unroll
+        for (int ct = 0; ct < CT; ct++)
+#pragma unroll
+            for (int kt = 0; kt < KT; kt++)
+#pragma unroll
+                for (int t = 0; t < 4; t++) {
+                    const int cor = ct * 16 + 4 * g + t;
+                    out[((long)tap * COUT + cor) * CIN + kt * 16 + c16] = acc[h][ct][kt][t];
+                }
     }
 }
 
