@@ -130,7 +130,9 @@ def learned_vs_jacobi(pa, nx, ny, train_iters=30, collect_every=20, collect_max=
     pairs, as a correction over the multigrid V-cycle (FP32); then three
     solvers take the system of a later OC iteration to a 1e-8 FP64 true
     relative residual: plain FP64 Jacobi-PCG, PCG with the V-cycle alone
-    (PF_OP_MG), and flexible CG with the network over the V-cycle."""
+    (PF_OP_MG), and flexible CG with the network over the V-cycle.  Then the
+    run's OC iterations 35-100 are repeated from the same state with each of
+    the three solvers (to 1e-8) and timed whole."""
     ctx = pa.cantilever2d(nx, ny)
     ctx.set_pcg_form(pa.PF_PCG_SINGLE_REDUCTION)  # the Jacobi leg at its fastest
     ctx.precond_init(seed=11)
@@ -148,6 +150,7 @@ def learned_vs_jacobi(pa, nx, ny, train_iters=30, collect_every=20, collect_max=
     loss0, loss1 = ctx.precond_train(steps, lr=lr, seed=17)
     t_train = time.perf_counter() - t0
     rr = ctx.simp_run(1, pa.simp_default_params(), s0=s)
+    s34 = rr["s"]
     ctx.set_density(rr["rho"])
     b = ctx.load()
     out = {"design": f"OC iteration {train_iters + 4} of the {nx}x{ny} loop", "pairs": pairs,
@@ -167,6 +170,19 @@ def learned_vs_jacobi(pa, nx, ny, train_iters=30, collect_every=20, collect_max=
     out["network_gain_iterations"] = out["multigrid_pcg"]["iterations"] / out["learned_fcg"]["iterations"]
     out["compliance_rel_diff"] = abs(comp["learned_fcg"] - comp["jacobi_fp64"]) / abs(comp["jacobi_fp64"])
     out["compliance_rel_diff_multigrid"] = abs(comp["multigrid_pcg"] - comp["jacobi_fp64"]) / abs(comp["jacobi_fp64"])
+    # the rest of the run, OC iterations 35-100, from the same state with each
+    # solver to 1e-8: whole OC iterations timed (solve, filter, OC)
+    loop = {}
+    for label, kw in (("jacobi_fp64", {}), ("multigrid_pcg", {"solver": pa.PF_OP_MG}),
+                      ("learned_fcg", {"learned": 1})):
+        h = ctx.simp_run(66, pa.simp_default_params(cg_eps=1e-8, **kw), s0=s34)
+        loop[label] = {"seconds": float(np.sum(h["seconds"])), "cg_iterations": int(np.sum(h["cg_iters"])),
+                       "compliance_last": float(h["f"][-1])}
+    for label in ("multigrid_pcg", "learned_fcg"):
+        loop[label]["speedup_vs_jacobi"] = loop["jacobi_fp64"]["seconds"] / loop[label]["seconds"]
+        loop[label]["compliance_rel_diff"] = (abs(loop[label]["compliance_last"] - loop["jacobi_fp64"]["compliance_last"])
+                                              / loop["jacobi_fp64"]["compliance_last"])
+    out["oc_iterations_35_100"] = loop
     ctx.close()
     return out
 
