@@ -14,7 +14,8 @@ single-reduction form at this size (its recomputing kernel) against
 ScalingCG's order.
 
 128 x 64 x 64 hex8 (1.6M DOFs): the matrix-free operator against the
-assembled one, its symmetry, and the FP64 solve's true residual.
+assembled one, its symmetry, the FP64 solve's true residual and the
+mixed-precision solve against it.
 """
 import numpy as np
 import pytest
@@ -140,6 +141,12 @@ def test_c5_hex8_128x64x64_operator_and_solve(pa):
     assert st["status"] == 1 and st["rel_residual"] < 1e-8 and st["true_residual"] < 1e-7
     # downward line load: the tip moves down, compliance positive
     assert u @ b > 0
+    # FP32 matrix-free inner PCG inside FP64 refinement, to a 1e-8 true residual
+    u32, s32 = ctx.pcg(b, op=pa.PF_OP_EBE32, eps=1e-8)
+    print(f"  mixed {s32['iterations']} it {s32['outer']} outer {s32['seconds']:.2f} s, "
+          f"|u32-u64|/|u64| {rel(u32, u):.2e}", flush=True)
+    assert s32["status"] == 1 and s32["true_residual"] <= 1.0000001e-8
+    assert rel(u32, u) < 1e-5
     ctx.close()
 
 

@@ -1,8 +1,8 @@
 """GPU parity of the hex8 (3D solid) path: the device element matrix
 against the reference's sample fixture and the oracle, the matrix-free and
 assembled operators, the solve of the reference's own hex8 sample
-(sample/solid), the 3D filter, and a short 3D SIMP loop against the
-oracle's loop."""
+(sample/solid), the mixed-precision solve, the 3D filter, and a short 3D
+SIMP loop against the oracle's loop."""
 import math
 
 import numpy as np
@@ -94,6 +94,26 @@ def test_hex8_csr_bit_exact_and_operators(pa):
     assert np.array_equal(ctx.apply_K(x, pa.PF_OP_CSR64), y_ref)
     assert rel(ctx.apply_K(x, pa.PF_OP_EBE64), y_ref) < 1e-13
     assert rel(ctx.apply_K(x, pa.PF_OP_EBE32), y_ref) < 1e-5
+
+
+def test_hex8_mixed_precision_solve(pa):
+    """FP32 matrix-free hex8 inner PCG inside FP64 iterative refinement: the
+    solve ends at a 1e-8 FP64 true residual and agrees with the FP64
+    Jacobi-PCG solution to what the residuals allow."""
+    nx, ny, nz = 32, 16, 16
+    ctx = pa.cantilever3d(nx, ny, nz)
+    rng = np.random.default_rng(31)
+    ctx.set_density(rng.uniform(0.2, 1.0, nx * ny * nz))
+    b = ctx.load()
+    x64, s64 = ctx.pcg(b, op=pa.PF_OP_EBE64, eps=1e-10)
+    x32, s32 = ctx.pcg(b, op=pa.PF_OP_EBE32, eps=1e-8)
+    print(f"hex8 mixed: {s32['iterations']} inner iterations, {s32['outer']} outer, "
+          f"{s32['fallbacks']} fallbacks; fp64 {s64['iterations']}", flush=True)
+    assert s64["status"] == 1 and s32["status"] == 1
+    assert s32["true_residual"] <= 1.0000001e-8
+    assert rel(x32, x64) < 1e-5
+    assert abs(x32 @ b - x64 @ b) / abs(x64 @ b) < 1e-8
+    ctx.close()
 
 
 def test_hex8_filter_bit_exact(pa):
