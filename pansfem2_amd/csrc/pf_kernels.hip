@@ -599,11 +599,16 @@ __global__ void __launch_bounds__(256) k_cg1_q4_tile(int nx, int ny, Cg1Tiles G,
             ev[u] = (h < es && ei >= 0 && ei < nx && ej >= 0 && ej < ny) ? E[(long)ny * ei + ej] : (T)0;
         }
     };
+    // own nodes in the operator phase: thread t takes column t mod tj of rows
+    // 2 (t / tj) and 2 (t / tj) + 1, so the two nodes share six of their nine
+    // LDS neighbours and two of their four element moduli
+    static_assert(OWN == 2 && TI * TJ == 512 && 256 % TJ == 0, "two vertically adjacent own nodes per thread");
+    const int oc = threadIdx.x % tj, orow = 2 * (threadIdx.x / tj);
     auto load_px = [&](Vec2<T>(&pv)[OWN], Vec2<T>(&xv)[OWN]) {
 #pragma unroll
         for (int o = 0; o < OWN; o++) {
-            const int q = threadIdx.x + o * 256;
-            const int r = q / tj, c = q - r * tj;
+            const int r = orow + o, c = oc;
+            const int q = r * tj + c;
             const int i = i0 + r, j = j0 + c;
             pv[o] = xv[o] = Vec2<T>{0, 0};
             if (prev && has && q < own && i <= nx && j <= ny) {
@@ -667,39 +672,50 @@ __global__ void __launch_bounds__(256) k_cg1_q4_tile(int nx, int ny, Cg1Tiles G,
         Vec2<T> pv[OWN], xv[OWN];
         load_px(pv, xv);
         __syncthreads();
+        // LDS rows orow .. orow + 3 and columns oc .. oc + 2 (halo coordinates),
+        // element moduli rows orow .. orow + 2 and columns oc .. oc + 1
+        Vec2<T> v[4][3];
+        T em[3][2];
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+#pragma unroll
+            for (int b = 0; b < 3; b++) v[a][b] = su[(orow + a) * ss + oc + b];
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+#pragma unroll
+            for (int b = 0; b < 2; b++) em[a][b] = se[(orow + a) * (tj + 1) + oc + b];
 #pragma unroll
         for (int o = 0; o < OWN; o++) {
-            const int q = threadIdx.x + o * 256;
-            const int r = q / tj, c = q - r * tj;
+            const int r = orow + o, c = oc;
+            const int q = r * tj + c;
             const int i = i0 + r, j = j0 + c;
-            if (q >= own || i > nx || j > ny) continue;
-            const Vec2<T>* s0 = su + r * ss + c;  // LDS rows r, r+1, r+2 from column c
+            if (i > nx || j > ny) continue;
             T ax = 0, ay = 0, ex, ey;
             if (i >= 1 && j >= 1) {  // element (i-1, j-1): this node is TR
-                elem_rows_q4(K0, 2, s0[0], s0[ss], s0[ss + 1], s0[1], ex, ey);
-                T Ee = se[r * (tj + 1) + c];
+                elem_rows_q4(K0, 2, v[o][0], v[o + 1][0], v[o + 1][1], v[o][1], ex, ey);
+                const T Ee = em[o][0];
                 ax += Ee * ex; ay += Ee * ey;
             }
             if (i >= 1 && j < ny) {  // element (i-1, j): BR
-                elem_rows_q4(K0, 1, s0[1], s0[ss + 1], s0[ss + 2], s0[2], ex, ey);
-                T Ee = se[r * (tj + 1) + c + 1];
+                elem_rows_q4(K0, 1, v[o][1], v[o + 1][1], v[o + 1][2], v[o][2], ex, ey);
+                const T Ee = em[o][1];
                 ax += Ee * ex; ay += Ee * ey;
             }
             if (i < nx && j >= 1) {  // element (i, j-1): TL
-                elem_rows_q4(K0, 3, s0[ss], s0[2 * ss], s0[2 * ss + 1], s0[ss + 1], ex, ey);
-                T Ee = se[(r + 1) * (tj + 1) + c];
+                elem_rows_q4(K0, 3, v[o + 1][0], v[o + 2][0], v[o + 2][1], v[o + 1][1], ex, ey);
+                const T Ee = em[o + 1][0];
                 ax += Ee * ex; ay += Ee * ey;
             }
             if (i < nx && j < ny) {  // element (i, j): BL
-                elem_rows_q4(K0, 0, s0[ss + 1], s0[2 * ss + 1], s0[2 * ss + 2], s0[ss + 2], ex, ey);
-                T Ee = se[(r + 1) * (tj + 1) + c + 1];
+                elem_rows_q4(K0, 0, v[o + 1][1], v[o + 2][1], v[o + 2][2], v[o + 1][2], ex, ey);
+                const T Ee = em[o + 1][1];
                 ax += Ee * ex; ay += Ee * ey;
             }
             const long n = (long)(ny + 1) * i + j;
             const uint8_t f = fix[n];
             if (f & 1) ax = 0;
             if (f & 2) ay = 0;
-            const Vec2<T> un = s0[ss + 1];
+            const Vec2<T> un = v[o + 1][1];
             store2(V.wout, n, Vec2<T>{ax, ay});
             acc[1] += (double)un.x * (double)ax;
             acc[1] += (double)un.y * (double)ay;
