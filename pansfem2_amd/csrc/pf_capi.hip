@@ -574,9 +574,10 @@ int pcg_ir(pf_ctx* ctx, const double* b, double* x, double eps, int itrmax, doub
 
 struct NetLayer {
     int cin = 0, cout = 0, level = 0, relu = 1;
+    int ntap = 9;  // 3x3 (quad4) or 3x3x3 (hex8)
     float *W = nullptr, *m = nullptr, *v = nullptr, *g = nullptr;
     bf16 *Wb = nullptr, *Wtb = nullptr;
-    long size() const { return 9L * cin * cout; }
+    long size() const { return (long)ntap * cin * cout; }
 };
 
 // Levels of the V-cycle (k_mg_* kernels): level 0 is the grid (E, K0, the
@@ -614,7 +615,8 @@ constexpr int kMpartDots = 128;
 constexpr int kMpartGrid = 1024;
 
 struct LearnedNet {
-    int H[4] = {0}, Wd[4] = {0};
+    int dim = 2;                            // 2: quad4 nodes (H x Wd), 3: hex8 nodes (H x Wd x Dz)
+    int H[4] = {0}, Wd[4] = {0}, Dz[4] = {1, 1, 1, 1};
     std::vector<NetLayer> L;  // c0a c0b c1a c1b c2a c2b c3a c3b d2 d1 d0 out
     // activations (level, channels) and their gradients
     bf16 *x0, *a0, *e0, *x1, *a1, *e1, *x2, *a2, *e2, *x3, *a3, *b3, *u2, *d2, *u1, *d1, *u0, *d0;
@@ -645,7 +647,7 @@ namespace {
 const int kNetLayers = 12;
 int moduli_from_rho(pf_ctx* ctx);
 
-long lvl_px(LearnedNet* N, int lv) { return (long)N->H[lv] * N->Wd[lv]; }
+long lvl_px(LearnedNet* N, int lv) { return (long)N->H[lv] * N->Wd[lv] * N->Dz[lv]; }
 
 void net_free(LearnedNet* N) {
     if (!N) return;
@@ -677,22 +679,25 @@ double urand(uint64_t& s) { return (double)(splitmix(s) >> 11) * (1.0 / 90071992
 
 int net_refresh_bf16(pf_ctx* ctx, LearnedNet* N) {
     for (NetLayer& l : N->L) {
-        k_weights_bf16<<<nblocks(l.size()), kBlock, 0, ctx->stream>>>(l.cout, l.cin, l.W, l.Wb, l.Wtb);
+        k_weights_bf16<<<nblocks(l.size()), kBlock, 0, ctx->stream>>>(l.ntap, l.cout, l.cin, l.W, l.Wb, l.Wtb);
     }
     return check_launch(ctx);
 }
 
 int net_build(pf_ctx* ctx, uint64_t seed) {
-    PF_CHECK(is2d(ctx), PF_ERR_UNSUPPORTED, "learned preconditioner: quad4 grids only");
     net_free(ctx->net);
     LearnedNet* N = new LearnedNet();
     ctx->net = N;
+    N->dim = is2d(ctx) ? 2 : 3;
     N->H[0] = ctx->nx + 1;
     N->Wd[0] = ctx->ny + 1;
+    N->Dz[0] = N->dim == 3 ? ctx->nz + 1 : 1;
     for (int lv = 1; lv < 4; lv++) {
         N->H[lv] = (N->H[lv - 1] + 1) / 2;
         N->Wd[lv] = (N->Wd[lv - 1] + 1) / 2;
+        N->Dz[lv] = N->dim == 3 ? (N->Dz[lv - 1] + 1) / 2 : 1;
     }
+    const int ntap = N->dim == 3 ? 27 : 9;
     const int spec[kNetLayers][4] = {  // cin, cout, level, relu
         {16, 16, 0, 1}, {16, 16, 0, 1}, {16, 32, 1, 1}, {32, 32, 1, 1}, {32, 32, 2, 1}, {32, 32, 2, 1},
         {32, 64, 3, 1}, {64, 32, 3, 1}, {32, 32, 2, 1}, {32, 16, 1, 1}, {16, 16, 0, 1}, {16, 16, 0, 0}};
@@ -704,6 +709,7 @@ int net_build(pf_ctx* ctx, uint64_t seed) {
         l.cout = spec[i][1];
         l.level = spec[i][2];
         l.relu = spec[i][3];
+        l.ntap = ntap;
         long n = l.size();
         maxw = std::max(maxw, n);
         PF_TRY(net_alloc(ctx, N, &l.W, n));
@@ -715,7 +721,7 @@ int net_build(pf_ctx* ctx, uint64_t seed) {
         // He-uniform initialisation; the output layer starts near zero so that
         // the preconditioner starts at Jacobi
         std::vector<float> w(n);
-        double a = std::sqrt(6.0 / (9.0 * l.cin)) * (i == kNetLayers - 1 ? 0.01 : 1.0);
+        double a = std::sqrt(6.0 / ((double)ntap * l.cin)) * (i == kNetLayers - 1 ? 0.01 : 1.0);
         for (long q = 0; q < n; q++) w[q] = (float)((2.0 * urand(rs) - 1.0) * a);
         PF_HIP(hipMemcpyAsync(l.W, w.data(), sizeof(float) * n, hipMemcpyHostToDevice, ctx->stream));
         PF_TRY(sync(ctx));
@@ -792,20 +798,69 @@ int conv_wgrad(pf_ctx* ctx, int cin, int cout, int H, int W, const bf16* X, cons
     return PF_ERR_UNSUPPORTED;
 }
 
+// hex8 networks: 3x3x3 convolutions on an H x W x Dp node grid
+template <int CI, int CO>
+void conv3_fwd_t(pf_ctx* ctx, int H, int W, int Dp, const bf16* X, const bf16* Wt, bf16* Y, float* Yf, int relu) {
+    dim3 grid((unsigned)(H * W), (Dp + 63) / 64);
+    k_conv3_fwd<CI, CO><<<grid, 256, 0, ctx->stream>>>(H, W, Dp, X, Wt, Y, Yf, relu);
+}
+
+template <int CI, int CO>
+void conv3_wgrad_t(pf_ctx* ctx, LearnedNet* N, int H, int W, int Dp, const bf16* X, const bf16* dZ, float* dW) {
+    const int ppb = kWgradPixels;
+    const long npx = (long)H * W * Dp;
+    const int nb = (int)((npx + ppb - 1) / ppb);
+    k_conv3_wgrad<CI, CO><<<nb, 256, 0, ctx->stream>>>(H, W, Dp, X, dZ, N->wpart, ppb);
+    const int n = 27 * CI * CO;
+    k_wgrad_reduce<<<nblocks(n), kBlock, 0, ctx->stream>>>(N->wpart, nb, n, dW);
+}
+
+int conv3_fwd(pf_ctx* ctx, int cin, int cout, int H, int W, int Dp, const bf16* X, const bf16* Wt, bf16* Y,
+              float* Yf, int relu) {
+#define PF_CONV3_CASE(a, b) \
+    if (cin == a && cout == b) { conv3_fwd_t<a, b>(ctx, H, W, Dp, X, Wt, Y, Yf, relu); return check_launch(ctx); }
+    PF_CONV3_CASE(16, 16) PF_CONV3_CASE(16, 32) PF_CONV3_CASE(32, 16) PF_CONV3_CASE(32, 32) PF_CONV3_CASE(32, 64)
+    PF_CONV3_CASE(64, 32)
+#undef PF_CONV3_CASE
+    ctx->err = "conv3: unsupported channel counts";
+    return PF_ERR_UNSUPPORTED;
+}
+
+int conv3_wgrad(pf_ctx* ctx, int cin, int cout, int H, int W, int Dp, const bf16* X, const bf16* dZ, float* dW) {
+    LearnedNet* N = ctx->net;
+#define PF_WG3_CASE(a, b) \
+    if (cin == a && cout == b) { conv3_wgrad_t<a, b>(ctx, N, H, W, Dp, X, dZ, dW); return check_launch(ctx); }
+    PF_WG3_CASE(16, 16) PF_WG3_CASE(16, 32) PF_WG3_CASE(32, 16) PF_WG3_CASE(32, 32) PF_WG3_CASE(32, 64)
+    PF_WG3_CASE(64, 32)
+#undef PF_WG3_CASE
+    ctx->err = "wgrad3: unsupported channel counts";
+    return PF_ERR_UNSUPPORTED;
+}
+
 int net_forward(pf_ctx* ctx) {
     LearnedNet* N = ctx->net;
     auto& L = N->L;
+    const bool d3 = N->dim == 3;
     auto lay = [&](int li, const bf16* X, bf16* Y, float* Yf) {
         NetLayer& l = L[li];
+        if (d3)
+            return conv3_fwd(ctx, l.cin, l.cout, N->H[l.level], N->Wd[l.level], N->Dz[l.level], X, l.Wb, Y, Yf,
+                             l.relu);
         return conv_fwd(ctx, l.cin, l.cout, N->H[l.level], N->Wd[l.level], X, l.Wb, Y, Yf, l.relu);
     };
     auto pool = [&](int lv, int C, const bf16* in, bf16* out) {
         long n = lvl_px(N, lv + 1) * C / 8;  // 8 channels per thread
-        k_pool2<<<gsize(n), kBlock, 0, ctx->stream>>>(N->H[lv], N->Wd[lv], C, in, out);
+        if (d3)
+            k_pool3<<<gsize(n), kBlock, 0, ctx->stream>>>(N->H[lv], N->Wd[lv], N->Dz[lv], C, in, out);
+        else
+            k_pool2<<<gsize(n), kBlock, 0, ctx->stream>>>(N->H[lv], N->Wd[lv], C, in, out);
     };
     auto upadd = [&](int lv, int C, const bf16* coarse, const bf16* skip, bf16* out) {
         long n = lvl_px(N, lv) * C / 8;
-        k_up2_add<<<gsize(n), kBlock, 0, ctx->stream>>>(N->H[lv], N->Wd[lv], C, coarse, skip, out);
+        if (d3)
+            k_up3_add<<<gsize(n), kBlock, 0, ctx->stream>>>(N->H[lv], N->Wd[lv], N->Dz[lv], C, coarse, skip, out);
+        else
+            k_up2_add<<<gsize(n), kBlock, 0, ctx->stream>>>(N->H[lv], N->Wd[lv], C, coarse, skip, out);
     };
     PF_TRY(lay(0, N->x0, N->a0, nullptr));
     PF_TRY(lay(1, N->a0, N->e0, nullptr));
@@ -834,9 +889,15 @@ int net_backward(pf_ctx* ctx) {
     auto& L = N->L;
     // layer li: output gradient gz (already masked by ReLU), input X -> dW, and
     // the input gradient gx (if non-null)
+    const bool d3 = N->dim == 3;
     auto back = [&](int li, const bf16* X, bf16* gz, bf16* gx) -> int {
         NetLayer& l = L[li];
-        const int H = N->H[l.level], W = N->Wd[l.level];
+        const int H = N->H[l.level], W = N->Wd[l.level], Dp = N->Dz[l.level];
+        if (d3) {
+            PF_TRY(conv3_wgrad(ctx, l.cin, l.cout, H, W, Dp, X, gz, l.g));
+            if (gx) PF_TRY(conv3_fwd(ctx, l.cout, l.cin, H, W, Dp, gz, l.Wtb, gx, nullptr, 0));
+            return PF_OK;
+        }
         PF_TRY(conv_wgrad(ctx, l.cin, l.cout, H, W, X, gz, l.g));
         if (gx) PF_TRY(conv_fwd(ctx, l.cout, l.cin, H, W, gz, l.Wtb, gx, nullptr, 0));
         return PF_OK;
@@ -847,11 +908,17 @@ int net_backward(pf_ctx* ctx) {
     };
     auto up_bwd = [&](int lv, int C, const bf16* gfine, bf16* gcoarse) {
         long n = lvl_px(N, lv + 1) * C;
-        k_up2_bwd<<<gsize(n), kBlock, 0, ctx->stream>>>(N->H[lv], N->Wd[lv], C, gfine, gcoarse);
+        if (d3)
+            k_up3_bwd<<<gsize(n), kBlock, 0, ctx->stream>>>(N->H[lv], N->Wd[lv], N->Dz[lv], C, gfine, gcoarse);
+        else
+            k_up2_bwd<<<gsize(n), kBlock, 0, ctx->stream>>>(N->H[lv], N->Wd[lv], C, gfine, gcoarse);
     };
     auto pool_bwd_add = [&](int lv, int C, const bf16* gcoarse, bf16* gfine) {
         long n = lvl_px(N, lv) * C;
-        k_pool2_bwd_add<<<gsize(n), kBlock, 0, ctx->stream>>>(N->H[lv], N->Wd[lv], C, gcoarse, gfine);
+        if (d3)
+            k_pool3_bwd_add<<<gsize(n), kBlock, 0, ctx->stream>>>(N->H[lv], N->Wd[lv], N->Dz[lv], C, gcoarse, gfine);
+        else
+            k_pool2_bwd_add<<<gsize(n), kBlock, 0, ctx->stream>>>(N->H[lv], N->Wd[lv], C, gcoarse, gfine);
     };
     auto copy = [&](long n, const bf16* a, bf16* b) { k_copy_bf16<<<gsize(n), kBlock, 0, ctx->stream>>>(n, a, b); };
     const long p0 = lvl_px(N, 0), p1 = lvl_px(N, 1), p2 = lvl_px(N, 2);
@@ -1285,15 +1352,25 @@ int optimal_pair(pf_ctx* ctx, const double* r, double* a, double* c);
 // and N->gd K t on return.
 // The pieces of M(r) (stream-ordered, no host synchronisation): N->w = y,
 // N->dd = t, N->gd = K t, N->Kw = K y.
+// the network's input channels for residual r (scale rs, or *rn_dev / sqrtN)
+void net_input(pf_ctx* ctx, const double* rho, const double* r, double rs, const double* rn_dev, double sqrtN) {
+    LearnedNet* N = ctx->net;
+    if (N->dim == 3)
+        k_net_input3<<<gsize(ctx->nn), kBlock, 0, ctx->stream>>>(ctx->nx, ctx->ny, ctx->nz, rho, r, rs, rn_dev, sqrtN,
+                                                                 N->x0);
+    else
+        k_net_input<<<gsize(ctx->nn), kBlock, 0, ctx->stream>>>(ctx->nx, ctx->ny, rho, r, rs, rn_dev, sqrtN, N->x0);
+}
+
 // rn: |r| on the host, or (rn_dev non-null) on the device.
 int net_parts(pf_ctx* ctx, const double* rho, const double* r, double rn, const double* rn_dev = nullptr) {
     LearnedNet* N = ctx->net;
     const long nv = ctx->nv;
     const double sqrtN = std::sqrt((double)std::max(1, ctx->N));
     const double rs = rn > 0 ? rn / sqrtN : 1.0;
-    k_net_input<<<gsize(ctx->nn), kBlock, 0, ctx->stream>>>(ctx->nx, ctx->ny, rho, r, rs, rn_dev, sqrtN, N->x0);
+    net_input(ctx, rho, r, rs, rn_dev, sqrtN);
     PF_TRY(net_forward(ctx));
-    k_net_y_nodal<<<gsize(ctx->nn), kBlock, 0, ctx->stream>>>(ctx->nn, N->y, ctx->d_fix, N->w);
+    k_net_y_nodal<<<gsize(ctx->nn), kBlock, 0, ctx->stream>>>(ctx->nn, ctx->ndpn, N->y, ctx->d_fix, N->w);
     if (ctx->mg)
         PF_TRY(mg_apply(ctx, r, N->dd));  // t = V(r)
     else
@@ -1482,10 +1559,9 @@ int net_train_step(pf_ctx* ctx, int s, float lr, double* loss) {
     const double rn = norm_nodal(ctx, N->rr, &rc);
     PF_TRY(rc);
     const double rs = rn > 0 ? rn / std::sqrt((double)std::max(1, ctx->N)) : 1.0;
-    k_net_input<<<gsize(ctx->nn), kBlock, 0, ctx->stream>>>(ctx->nx, ctx->ny, ctx->d_rho, N->rr, rs, nullptr, 1.0,
-                                                            N->x0);
+    net_input(ctx, ctx->d_rho, N->rr, rs, nullptr, 1.0);
     PF_TRY(net_forward(ctx));
-    k_net_y_nodal<<<gsize(ctx->nn), kBlock, 0, ctx->stream>>>(ctx->nn, N->y, ctx->d_fix, N->w);   // y
+    k_net_y_nodal<<<gsize(ctx->nn), kBlock, 0, ctx->stream>>>(ctx->nn, ctx->ndpn, N->y, ctx->d_fix, N->w);   // y
     PF_TRY(apply_nodal(ctx, false, N->w, N->Kw, true));                                           // K y
     if (ctx->mg) {
         PF_TRY(mg_build(ctx));                                                                     // this pair's moduli
@@ -1510,7 +1586,7 @@ int net_train_step(pf_ctx* ctx, int s, float lr, double* loss) {
     *loss = dot_nodal(ctx, ctx->d_t0, ctx->d_t1, &rc) / er;
     PF_TRY(rc);
     k_scale_d<<<gsize(nv), kBlock, 0, ctx->stream>>>(nv, 2.0 * c / er, ctx->d_t1);
-    k_net_dy_from_nodal<<<gsize(ctx->nn), kBlock, 0, ctx->stream>>>(ctx->nn, ctx->d_t1, ctx->d_fix, N->gy);
+    k_net_dy_from_nodal<<<gsize(ctx->nn), kBlock, 0, ctx->stream>>>(ctx->nn, ctx->ndpn, ctx->d_t1, ctx->d_fix, N->gy);
     PF_TRY(net_backward(ctx));
     N->adam_t++;
     const float b1 = 0.9f, b2 = 0.999f;
@@ -1527,11 +1603,15 @@ int net_collect_random(pf_ctx* ctx, int count, uint64_t seed, int* total) {
     LearnedNet* N = ctx->net;
     const long nv = ctx->nv, ne = ctx->ne;
     int levels = 1;
-    while ((1 << levels) < std::max(ctx->nx, ctx->ny) / 2) levels++;
+    while ((1 << levels) < std::max(std::max(ctx->nx, ctx->ny), N->dim == 3 ? ctx->nz : 0) / 2) levels++;
     for (int q = 0; q < count && N->nsamp < N->cap; q++) {
         const int slot = N->nsamp++;
-        k_random_field<<<gsize(ctx->nn), kBlock, 0, ctx->stream>>>(ctx->nx, ctx->ny, seed + 7919ull * q, levels,
-                                                                  ctx->d_fix, ctx->d_t0);
+        if (N->dim == 3)
+            k_random_field3<<<gsize(ctx->nn), kBlock, 0, ctx->stream>>>(ctx->nx, ctx->ny, ctx->nz, seed + 7919ull * q,
+                                                                       levels, ctx->d_fix, ctx->d_t0);
+        else
+            k_random_field<<<gsize(ctx->nn), kBlock, 0, ctx->stream>>>(ctx->nx, ctx->ny, seed + 7919ull * q, levels,
+                                                                      ctx->d_fix, ctx->d_t0);
         PF_TRY(apply_nodal(ctx, false, ctx->d_t0, ctx->d_t1, true));
         k_pair_store<<<gsize(nv), kBlock, 0, ctx->stream>>>(nv, ctx->d_t1, ctx->d_t0, N->ds_r + (size_t)slot * nv,
                                                            N->ds_e + (size_t)slot * nv);

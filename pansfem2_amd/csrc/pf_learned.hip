@@ -391,6 +391,228 @@ __global__ void k_net_input(int nx, int ny, const double* __restrict__ rho, cons
     }
 }
 
+// ------------------------------------------------- hex8 (3D) network ----
+// The same network on the H x W x Dp node grid of a hex8 mesh (node
+// (i, j, k) at ((i W + j) Dp + k), channels last): 3x3x3 convolutions with
+// tap = 9 (di + 1) + 3 (dj + 1) + (dk + 1), 2x2x2 pooling and upsampling.
+
+// One workgroup = one (i, j) line, 4 wavefronts x 16 pixels along k; inputs
+// and weights read from global memory per tap.
+template <int CIN, int COUT>
+__global__ void __launch_bounds__(256) k_conv3_fwd(int H, int W, int Dp, const bf16* __restrict__ X,
+                                                   const bf16* __restrict__ Wt, bf16* __restrict__ Y,
+                                                   float* __restrict__ Yf, int relu) {
+    const int i = blockIdx.x / W, j = blockIdx.x - i * W;
+    const int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const int k0 = blockIdx.y * 64 + wave * 16;
+    if (k0 >= Dp) return;
+    const int pk = k0 + (l & 15);
+    const int g = l >> 4;
+    f32x4 acc[COUT / 16];
+#pragma unroll
+    for (int c = 0; c < COUT / 16; c++) acc[c] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const bf16x4 zero = {0, 0, 0, 0};
+    for (int tap = 0; tap < 27; tap++) {
+        const int ii = i + tap / 9 - 1, jj = j + (tap / 3) % 3 - 1, kk = pk + tap % 3 - 1;
+        const bool in = ii >= 0 && ii < H && jj >= 0 && jj < W && kk >= 0 && kk < Dp;
+        const bf16* xp = X + (((long)ii * W + jj) * Dp + kk) * CIN;
+#pragma unroll
+        for (int kc = 0; kc < CIN / 16; kc++) {
+            const int ci = kc * 16 + 4 * g;
+            const bf16x4 b = in ? ld4(xp + ci) : zero;
+#pragma unroll
+            for (int c = 0; c < COUT / 16; c++) {
+                const int co = c * 16 + (l & 15);
+                const bf16x4 a = ld4(Wt + ((long)(tap * COUT + co) * CIN + ci));
+                acc[c] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[c], 0, 0, 0);
+            }
+        }
+    }
+    if (pk >= Dp) return;
+    const long o = (((long)i * W + j) * Dp + pk) * COUT;
+#pragma unroll
+    for (int c = 0; c < COUT / 16; c++) {
+        float v[4];
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            v[t] = acc[c][t];
+            if (relu && v[t] < 0.f) v[t] = 0.f;
+        }
+        const int co = c * 16 + 4 * g;
+        if (Yf) {
+#pragma unroll
+            for (int t = 0; t < 4; t++) Yf[o + co + t] = v[t];
+        } else {
+            bf16x4 pk4 = {(short)f2bf(v[0]), (short)f2bf(v[1]), (short)f2bf(v[2]), (short)f2bf(v[3])};
+            *reinterpret_cast<bf16x4*>(Y + o + co) = pk4;
+        }
+    }
+}
+
+// dW[tap][co][ci] = sum_p dZ[p][co] X[p + off(tap)][ci] over `ppb` pixels per
+// workgroup (FP32 partials per workgroup); each wave takes every 4th
+// (tap, output block, input block) item over all of the workgroup's pixels.
+template <int CIN, int COUT>
+__global__ void __launch_bounds__(256) k_conv3_wgrad(int H, int W, int Dp, const bf16* __restrict__ X,
+                                                     const bf16* __restrict__ dZ, float* __restrict__ part, int ppb) {
+    const int wave = threadIdx.x >> 6, l = threadIdx.x & 63, g = l >> 4;
+    const long plane = (long)W * Dp, npx = (long)H * plane;
+    const long p_begin = (long)blockIdx.x * ppb;
+    const long p_end = min(npx, p_begin + ppb);
+    constexpr int CT = COUT / 16, KT = CIN / 16, ITEMS = 27 * CT * KT;
+    float* out = part + (long)blockIdx.x * 27 * COUT * CIN;
+    for (int item = wave; item < ITEMS; item += 4) {
+        const int tap = item / (CT * KT), rem = item - tap * CT * KT, ct = rem / KT, kt = rem - ct * KT;
+        const int di = tap / 9 - 1, dj = (tap / 3) % 3 - 1, dk = tap % 3 - 1;
+        const int co = ct * 16 + (l & 15), ci = kt * 16 + (l & 15);
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+        for (long p0 = p_begin; p0 < p_end; p0 += 16) {
+            bf16x4 a, b;
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                const long p = p0 + 4 * g + t;
+                short av = 0, bv = 0;
+                if (p < p_end) {
+                    av = (short)dZ[p * COUT + co];
+                    const int pi = (int)(p / plane);
+                    const long rem2 = p - (long)pi * plane;
+                    const int pj = (int)(rem2 / Dp), pk = (int)(rem2 - (long)pj * Dp);
+                    const int ii = pi + di, jj = pj + dj, kk = pk + dk;
+                    if (ii >= 0 && ii < H && jj >= 0 && jj < W && kk >= 0 && kk < Dp)
+                        bv = (short)X[(((long)ii * W + jj) * Dp + kk) * CIN + ci];
+                }
+                a[t] = av;
+                b[t] = bv;
+            }
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+        }
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            const int cor = ct * 16 + 4 * g + t;
+            out[((long)tap * COUT + cor) * CIN + ci] = acc[t];
+        }
+    }
+}
+
+// 2x2x2 average pooling with ceil (partial blocks divide by their own
+// count), eight channels per thread
+__global__ void k_pool3(int H, int W, int Dp, int C, const bf16* __restrict__ in, bf16* __restrict__ out) {
+    const int Hc = (H + 1) / 2, Wc = (W + 1) / 2, Dc = (Dp + 1) / 2, C8 = C / 8;
+    const long n8 = (long)Hc * Wc * Dc * C8;
+    for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n8; q += (long)gridDim.x * blockDim.x) {
+        const int c8 = (int)(q % C8);
+        const long pc = q / C8;
+        const int I = (int)(pc / ((long)Wc * Dc));
+        const long rem = pc - (long)I * Wc * Dc;
+        const int J = (int)(rem / Dc), K = (int)(rem - (long)J * Dc);
+        float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        int cnt = 0;
+        for (int a = 0; a < 2; a++)
+            for (int b = 0; b < 2; b++)
+                for (int c = 0; c < 2; c++) {
+                    const int i = 2 * I + a, j = 2 * J + b, k = 2 * K + c;
+                    if (i < H && j < W && k < Dp) {
+                        float f[8];
+                        unpack8(*reinterpret_cast<const uint4*>(in + (((long)i * W + j) * Dp + k) * C + 8 * c8), f);
+#pragma unroll
+                        for (int t = 0; t < 8; t++) s[t] += f[t];
+                        cnt++;
+                    }
+                }
+#pragma unroll
+        for (int t = 0; t < 8; t++) s[t] = s[t] / (float)cnt;
+        *reinterpret_cast<uint4*>(out + 8 * q) = pack8(s);
+    }
+}
+
+// d_fine += d_coarse / count (backward of k_pool3, accumulated)
+__global__ void k_pool3_bwd_add(int H, int W, int Dp, int C, const bf16* __restrict__ dout, bf16* __restrict__ din) {
+    const int Wc = (W + 1) / 2, Dc = (Dp + 1) / 2;
+    const long plane = (long)W * Dp, n = (long)H * plane * C;
+    for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
+        const int c = (int)(q % C);
+        const long p = q / C;
+        const int i = (int)(p / plane);
+        const long rem = p - (long)i * plane;
+        const int j = (int)(rem / Dp), k = (int)(rem - (long)j * Dp);
+        const int I = i / 2, J = j / 2, K = k / 2;
+        const int cnt = ((2 * I + 1 < H) ? 2 : 1) * ((2 * J + 1 < W) ? 2 : 1) * ((2 * K + 1 < Dp) ? 2 : 1);
+        din[q] = f2bf(bf2f(din[q]) + bf2f(dout[(((long)I * Wc + J) * Dc + K) * C + c]) / (float)cnt);
+    }
+}
+
+// out (fine) = nearest-upsampled coarse + skip, eight channels per thread
+__global__ void k_up3_add(int H, int W, int Dp, int C, const bf16* __restrict__ coarse, const bf16* __restrict__ skip,
+                          bf16* __restrict__ out) {
+    const int Wc = (W + 1) / 2, Dc = (Dp + 1) / 2, C8 = C / 8;
+    const long plane = (long)W * Dp, n8 = (long)H * plane * C8;
+    for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n8; q += (long)gridDim.x * blockDim.x) {
+        const int c8 = (int)(q % C8);
+        const long p = q / C8;
+        const int i = (int)(p / plane);
+        const long rem = p - (long)i * plane;
+        const int j = (int)(rem / Dp), k = (int)(rem - (long)j * Dp);
+        float fc[8], fs[8];
+        unpack8(*reinterpret_cast<const uint4*>(coarse + (((long)(i / 2) * Wc + j / 2) * Dc + k / 2) * C + 8 * c8), fc);
+        unpack8(*reinterpret_cast<const uint4*>(skip + 8 * q), fs);
+#pragma unroll
+        for (int t = 0; t < 8; t++) fc[t] = fc[t] + fs[t];
+        *reinterpret_cast<uint4*>(out + 8 * q) = pack8(fc);
+    }
+}
+
+// d_coarse = sum of the fine gradient over each 2x2x2 block
+__global__ void k_up3_bwd(int H, int W, int Dp, int C, const bf16* __restrict__ dfine, bf16* __restrict__ dcoarse) {
+    const int Hc = (H + 1) / 2, Wc = (W + 1) / 2, Dc = (Dp + 1) / 2;
+    const long n = (long)Hc * Wc * Dc * C;
+    for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
+        const int c = (int)(q % C);
+        const long pc = q / C;
+        const int I = (int)(pc / ((long)Wc * Dc));
+        const long rem = pc - (long)I * Wc * Dc;
+        const int J = (int)(rem / Dc), K = (int)(rem - (long)J * Dc);
+        float s = 0.f;
+        for (int a = 0; a < 2; a++)
+            for (int b = 0; b < 2; b++)
+                for (int cc = 0; cc < 2; cc++) {
+                    const int i = 2 * I + a, j = 2 * J + b, k = 2 * K + cc;
+                    if (i < H && j < W && k < Dp) s += bf2f(dfine[(((long)i * W + j) * Dp + k) * C + c]);
+                }
+        dcoarse[q] = f2bf(s);
+    }
+}
+
+// (2 rho_node - 1, r_x / s, r_y / s, r_z / s) on the hex8 nodes, rho_node the
+// mean over the node's (up to eight) elements; element (ei, ej, ek) at
+// (ny nz) ei + nz ej + ek
+__global__ void k_net_input3(int nx, int ny, int nz, const double* __restrict__ rho, const double* __restrict__ r,
+                             double rs, const double* __restrict__ rn_dev, double sqrtN, bf16* __restrict__ x0) {
+    if (rn_dev) rs = *rn_dev > 0 ? *rn_dev / sqrtN : 1.0;
+    const long plane = (long)(ny + 1) * (nz + 1), nn = (long)(nx + 1) * plane;
+    for (long n = blockIdx.x * (long)blockDim.x + threadIdx.x; n < nn; n += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(n / plane);
+        const long rem = n - (long)i * plane;
+        const int j = (int)(rem / (nz + 1)), k = (int)(rem - (long)j * (nz + 1));
+        double s = 0.0;
+        int cnt = 0;
+        for (int a = -1; a <= 0; a++)
+            for (int b = -1; b <= 0; b++)
+                for (int c = -1; c <= 0; c++) {
+                    const int ei = i + a, ej = j + b, ek = k + c;
+                    if (ei >= 0 && ei < nx && ej >= 0 && ej < ny && ek >= 0 && ek < nz) {
+                        s += rho[((long)ny * nz) * ei + (long)nz * ej + ek];
+                        cnt++;
+                    }
+                }
+        float f[16] = {0};
+        f[0] = (float)(2.0 * (s / cnt) - 1.0);
+        f[1] = (float)(r[3 * n] / rs);
+        f[2] = (float)(r[3 * n + 1] / rs);
+        f[3] = (float)(r[3 * n + 2] / rs);
+        for (int c = 0; c < 16; c++) x0[n * 16 + c] = f2bf(f[c]);
+    }
+}
+
 // Up to 32 dot products against one vector in a single pass:
 // part[block * nb + j] = sum_i a_i * b_j[i], fixed-order block sums.
 struct PtrPack {
@@ -609,22 +831,22 @@ __global__ void __launch_bounds__(256) k_fcg_check(FcgDev* __restrict__ S, const
     S->k += 1;
 }
 
-// y channels 0, 1 -> nodal FP64 vector (0 on fixed DOFs)
-__global__ void k_net_y_nodal(long nn, const float* __restrict__ y, const uint8_t* __restrict__ fix,
+// y channels 0 .. nd-1 -> nodal FP64 vector of nd DOFs per node (0 on fixed DOFs)
+__global__ void k_net_y_nodal(long nn, int nd, const float* __restrict__ y, const uint8_t* __restrict__ fix,
                               double* __restrict__ v) {
     for (long n = blockIdx.x * (long)blockDim.x + threadIdx.x; n < nn; n += (long)gridDim.x * blockDim.x) {
         uint8_t f = fix[n];
-        for (int c = 0; c < 2; c++) v[2 * n + c] = (f & (1u << c)) ? 0.0 : (double)y[n * 16 + c];
+        for (int c = 0; c < nd; c++) v[nd * n + c] = (f & (1u << c)) ? 0.0 : (double)y[n * 16 + c];
     }
 }
 
-// nodal FP64 gradient -> 16-channel BF16 output gradient (channels 0, 1)
-__global__ void k_net_dy_from_nodal(long nn, const double* __restrict__ g, const uint8_t* __restrict__ fix,
+// nodal FP64 gradient -> 16-channel BF16 output gradient (channels 0 .. nd-1)
+__global__ void k_net_dy_from_nodal(long nn, int nd, const double* __restrict__ g, const uint8_t* __restrict__ fix,
                                     bf16* __restrict__ dy) {
     for (long n = blockIdx.x * (long)blockDim.x + threadIdx.x; n < nn; n += (long)gridDim.x * blockDim.x) {
         uint8_t f = fix[n];
         float v[16] = {0};
-        for (int c = 0; c < 2; c++) v[c] = (f & (1u << c)) ? 0.f : (float)g[2 * n + c];
+        for (int c = 0; c < nd; c++) v[c] = (f & (1u << c)) ? 0.f : (float)g[nd * n + c];
         for (int c = 0; c < 16; c++) dy[n * 16 + c] = f2bf(v[c]);
     }
 }
@@ -658,6 +880,37 @@ __global__ void k_random_field(int nx, int ny, uint64_t seed, int levels, const 
                 acc += (double)sp * ((1 - fi) * ((1 - fj) * a + fj * cc) + fi * ((1 - fj) * b + fj * d));
             }
             v[2 * n + c] = (f & (1u << c)) ? 0.0 : acc;
+        }
+    }
+}
+
+// the random field of k_random_field on hex8 nodes: trilinear lattice noise
+__global__ void k_random_field3(int nx, int ny, int nz, uint64_t seed, int levels, const uint8_t* __restrict__ fix,
+                                double* __restrict__ v) {
+    const long plane = (long)(ny + 1) * (nz + 1), nn = (long)(nx + 1) * plane;
+    for (long n = blockIdx.x * (long)blockDim.x + threadIdx.x; n < nn; n += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(n / plane);
+        const long rem = n - (long)i * plane;
+        const int j = (int)(rem / (nz + 1)), k = (int)(rem - (long)j * (nz + 1));
+        const uint8_t f = fix[n];
+        for (int c = 0; c < 3; c++) {
+            double acc = 0.0;
+            for (int l = 0; l < levels; l++) {
+                const int sp = 1 << l;
+                const int I = i / sp, J = j / sp, K = k / sp;
+                const double fi = (double)(i - I * sp) / sp, fj = (double)(j - J * sp) / sp,
+                             fk = (double)(k - K * sp) / sp;
+                double val = 0.0;
+                for (int a = 0; a < 2; a++)
+                    for (int b = 0; b < 2; b++)
+                        for (int cc = 0; cc < 2; cc++) {
+                            const double w = (a ? fi : 1 - fi) * (b ? fj : 1 - fj) * (cc ? fk : 1 - fk);
+                            // the lattice index packs (J, K) into the second coordinate
+                            val += w * lattice_noise(seed, l, I + a, (J + b) * 65536 + (K + cc), c);
+                        }
+                acc += (double)sp * val;
+            }
+            v[3 * n + c] = (f & (1u << c)) ? 0.0 : acc;
         }
     }
 }
@@ -702,15 +955,16 @@ __global__ void k_adam(int n, float* __restrict__ w, float* __restrict__ m, floa
     }
 }
 
-// Wb[tap][co][ci] = bf16(W); Wtb[tap'][ci][co] = bf16(W[8 - tap'][co][ci])
-__global__ void k_weights_bf16(int cout, int cin, const float* __restrict__ W, bf16* __restrict__ Wb,
+// Wb[tap][co][ci] = bf16(W); Wtb[tap'][ci][co] = bf16(W[ntap - 1 - tap'][co][ci])
+// (ntap = 9 or 27: reversing the tap index negates every offset)
+__global__ void k_weights_bf16(int ntap, int cout, int cin, const float* __restrict__ W, bf16* __restrict__ Wb,
                                bf16* __restrict__ Wtb) {
-    const int n = 9 * cout * cin;
+    const int n = ntap * cout * cin;
     for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
         const int tap = q / (cout * cin), rem = q - tap * cout * cin, co = rem / cin, ci = rem - co * cin;
         bf16 h = f2bf(W[q]);
         Wb[q] = h;
-        Wtb[((long)(8 - tap) * cin + ci) * cout + co] = h;
+        Wtb[((long)(ntap - 1 - tap) * cin + ci) * cout + co] = h;
     }
 }
 
