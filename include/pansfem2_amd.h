@@ -316,15 +316,16 @@ int pf_mg_level(pf_ctx* ctx, int level, int* levels, int* nx, int* ny, int* nz, 
 int pf_profile_kernels(pf_ctx* ctx, int op, int reps, double* ms);
 
 /* ---- learned preconditioner -------------------------------------------- */
-/* 4-level 3x3 convolutional network on the nodal grid of a quad4 problem
- * (BF16 weights and activations, FP32 accumulation on MFMA), input
- * (2 rho_node - 1, r/s) with s the RMS of r, output a field y.  Inside
- * flexible CG (PF_OP_LEARNED, Notay's FCG(32)) the preconditioner is
- * M(r) = a (r ./ D) + c y, D the Jacobi diagonal, with (a, c) minimising the
- * energy norm of the error over span{r ./ D, y}; Jacobi-PCG takes over from
- * the current iterate on breakdown, divergence or stagnation.  No
- * counterpart in the reference.  pf_precond_init builds the network for the
- * current grid with seeded weights. */
+/* 4-level convolutional network on the nodal grid, 3x3 on a quad4 problem
+ * and 3x3x3 on a hex8 one (BF16 weights and activations, FP32 accumulation
+ * on MFMA), input (2 rho_node - 1, r/s) with s the RMS of r, output a field
+ * y.  Inside flexible CG (PF_OP_LEARNED, Notay's FCG(m)) the preconditioner
+ * is M(r) = a t + c y, t the Jacobi step r ./ D or, after pf_mg_setup, the
+ * multigrid V-cycle, with (a, c) minimising the energy norm of the error
+ * over span{t, y}; Jacobi-PCG takes over from the current iterate on
+ * breakdown, divergence or stagnation.  No counterpart in the reference.
+ * pf_precond_init builds the network for the current grid with seeded
+ * weights. */
 int pf_precond_init(pf_ctx* ctx, int channels, uint64_t seed);
 /* Number of network parameters (FP32 master copy). */
 int pf_precond_num_params(pf_ctx* ctx, int64_t* count);
@@ -344,7 +345,8 @@ int pf_precond_train(pf_ctx* ctx, int steps, double lr, uint64_t seed, double* l
 /* d = M(r) for a free-DOF residual with the current moduli. */
 int pf_precond_apply(pf_ctx* ctx, const double* r_free, double* d_free);
 /* Raw network pass on an input tensor x0 (H x W x 16, H = nx+1, W = ny+1,
- * FP32 values rounded to BF16) -> y (H x W x 16, FP32), and the parameter
+ * or H x W x Dp x 16 with Dp = nz+1 on hex8; FP32 values rounded to BF16)
+ * -> y (same shape, FP32), and the parameter
  * gradient of sum(y * dy) (grad in pf_precond_save order).  For checks
  * against a CPU restatement of the network. */
 int pf_precond_forward_raw(pf_ctx* ctx, const float* x0, float* y);
