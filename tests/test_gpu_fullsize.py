@@ -16,7 +16,8 @@ ScalingCG's order.
 128 x 64 x 64 hex8 (1.6M DOFs): the matrix-free operator against the
 assembled one, its symmetry, the FP64 solve's true residual and the
 mixed-precision solve against it; the learned (3x3x3) preconditioner over
-the hex8 V-cycle; 60 OC iterations with learned and with multigrid solves.
+the hex8 V-cycle; 60 OC iterations with learned, multigrid and
+mixed-precision solves.
 """
 import numpy as np
 import pytest
@@ -226,4 +227,30 @@ def test_c5_hex8_simp_loop_60_iterations_multigrid(pa):
     assert abs(np.mean(r["rho"]) - 0.3) < 2e-3
     assert np.min(r["rho"]) >= 0.0 and np.max(r["rho"]) <= 1.0
     assert np.max(r["cg_iters"]) < 100000
+    ctx.close()
+
+
+def test_c5_hex8_simp_loop_60_iterations_mixed_precision(pa):
+    """The hex8 workload's 60 OC iterations with every solve in mixed
+    precision (FP32 matrix-free inner PCG inside FP64 refinement, to the
+    loop's 1e-10 FP64 true residual): the same size-independent properties
+    as the multigrid loop, and the same final compliance to 1e-6."""
+    nx, ny, nz = 128, 64, 64
+    print("c5 hex8 SIMP loop, 60 OC iterations, mixed-precision solves", flush=True)
+    ctx = pa.cantilever3d(nx, ny, nz)
+    import time
+    t0 = time.time()
+    r = ctx.simp_run(60, pa.simp_default_params(vf=0.3, mixed=1), s0=0.3)
+    dt = time.time() - t0
+    f = np.asarray(r["f"])
+    print(f"  {dt:.1f} s, compliance {f[0]:.4g} -> {f[-1]:.6g}, inner CG iterations per solve "
+          f"{np.mean(r['cg_iters']):.0f} (max {np.max(r['cg_iters'])}), volume {np.mean(r['rho']):.4f}", flush=True)
+    assert len(f) == 60 and np.all(np.isfinite(f))
+    assert f[-1] < 0.5 * f[0]
+    assert abs(np.mean(r["rho"]) - 0.3) < 2e-3
+    assert np.min(r["rho"]) >= 0.0 and np.max(r["rho"]) <= 1.0
+    # the multigrid loop ends at 1.786 (printed by its test); both solve to 1e-10
+    ctx.mg_setup(1)
+    rm = ctx.simp_run(60, pa.simp_default_params(vf=0.3, solver=pa.PF_OP_MG), s0=0.3)
+    assert abs(f[-1] - rm["f"][-1]) / rm["f"][-1] < 1e-6
     ctx.close()
