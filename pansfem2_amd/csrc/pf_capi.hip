@@ -71,6 +71,40 @@ K1Shape k1_shape(int nx, int ny) {
     return sh;
 }
 
+// Hex8 K1 with the fused p update (k_pcg_k1_h8_tile): kH8Slab-node slabs of a
+// node plane, chunks of PF_H8_PLANES planes per workgroup (doubled until the
+// grid fits the reduction scratch).  per == 0: the plane is too long for the
+// kernel's registers (nz > 211), and the three-launch form runs instead.
+#ifndef PF_H8_PLANES
+#define PF_H8_PLANES 4
+#endif
+struct H8Shape {
+    int nslab, planes, nwork, per;
+    unsigned grid;
+    size_t lds;
+};
+H8Shape h8_shape(const pf_ctx* ctx, size_t tsize) {
+    H8Shape sh{};
+    const long pl = (long)(ctx->ny + 1) * (ctx->nz + 1);
+    const int nv = 3 * (kH8Slab + 2 * (ctx->nz + 2));
+    sh.per = nv <= 5 * 256 ? 5 : (nv <= 8 * 256 ? 8 : 0);
+    sh.nslab = (int)((pl + kH8Slab - 1) / kH8Slab);
+    sh.planes = PF_H8_PLANES;
+    for (;;) {
+        const long nwork = (long)sh.nslab * ((ctx->nx + 1 + sh.planes - 1) / sh.planes);
+        const long grid = 8 * ((nwork + 7) / 8);
+        if (grid <= kMaxPartials || sh.planes > ctx->nx) {
+            sh.nwork = (int)nwork;
+            sh.grid = (unsigned)grid;
+            break;
+        }
+        sh.planes *= 2;
+    }
+    if (sh.grid > (unsigned)kMaxPartials || !ctx->d_K0g) sh.per = 0;
+    sh.lds = (size_t)3 * nv * tsize;
+    return sh;
+}
+
 // workgroups of the streaming PCG vector kernel (few, so that the operator
 // kernel's redundant read of its partials stays small)
 constexpr int k2_cap() { return 512; }
@@ -117,6 +151,17 @@ int check_launch(pf_ctx* ctx) {
 bool is2d(pf_ctx* ctx) { return ctx->dim == 2; }
 void release_net(pf_ctx* ctx);
 G3 g3(pf_ctx* ctx) { return G3{ctx->nx, ctx->ny, ctx->nz}; }
+
+template <class T>
+void launch_h8_k1_tile(pf_ctx* ctx, const H8Shape& sh, int k, const T* pold, const T* z, T* pnew, T* q, const T* Ev,
+                       const T* Kg, const double* partB, int nbB, double* partA, PcgDev* S, double eps) {
+    if (sh.per == 5)
+        k_pcg_k1_h8_tile<T, 5><<<sh.grid, kBlock, sh.lds, ctx->stream>>>(
+            g3(ctx), sh.nslab, sh.planes, sh.nwork, k, pold, z, pnew, q, Ev, Kg, ctx->d_fix, partB, nbB, partA, S, eps);
+    else
+        k_pcg_k1_h8_tile<T, 8><<<sh.grid, kBlock, sh.lds, ctx->stream>>>(
+            g3(ctx), sh.nslab, sh.planes, sh.nwork, k, pold, z, pnew, q, Ev, Kg, ctx->d_fix, partB, nbB, partA, S, eps);
+}
 
 void mg_free(pf_ctx* ctx);
 void free_grid(pf_ctx* ctx) {
@@ -386,7 +431,8 @@ int pcg_device(pf_ctx* ctx, bool csr, const T* b, T* x, const T* D, T* r, T* z, 
     double* partB = ctx->red.partials + 2 * kMaxPartials;
     const bool h8 = !csr && !is2d(ctx);
     const K1Shape sh = k1_shape(ctx->nx, ctx->ny);
-    const unsigned g1 = csr ? gsize(ctx->N) : (h8 ? gsize(ctx->nn) : sh.grid);
+    const H8Shape s8 = h8 ? h8_shape(ctx, sizeof(T)) : H8Shape{};
+    const unsigned g1 = csr ? gsize(ctx->N) : (h8 ? (s8.per ? s8.grid : gsize(ctx->nn)) : sh.grid);
     const unsigned g2 = std::min<unsigned>(gsize(n), (unsigned)k2_cap());
     PF_CHECK(g1 <= (unsigned)kMaxPartials, PF_ERR_ARG, "grid too large for the reduction scratch");
     k_pcg_init<T><<<g2, kBlock, 0, ctx->stream>>>(n, b, D, x, r, z, p0, partB, S);
@@ -405,9 +451,15 @@ int pcg_device(pf_ctx* ctx, bool csr, const T* b, T* x, const T* D, T* r, T* z, 
             T* pold = (k & 1) ? p1 : p0;
             T* pnew = (k & 1) ? p0 : p1;
             if (h8) {
-                // three launches: p = beta p + z in place (head + convergence), q = K p, update
                 const T* Ev = std::is_same<T, float>::value ? (const T*)ctx->d_Ef : (const T*)ctx->d_E;
                 const T* Kv = std::is_same<T, float>::value ? (const T*)ctx->d_K0f : (const T*)ctx->d_K0;
+                if (s8.per) {
+                    const T* Kg = std::is_same<T, float>::value ? (const T*)ctx->d_K0gf : (const T*)ctx->d_K0g;
+                    launch_h8_k1_tile<T>(ctx, s8, k, pold, z, pnew, q, Ev, Kg, partB, (int)g2, partA, S, eps);
+                    k_pcg_k2<T><<<g2, kBlock, 0, ctx->stream>>>(n, k, x, r, z, pnew, q, D, partA, (int)g1, partB, S);
+                    continue;
+                }
+                // three launches: p = beta p + z in place (head + convergence), q = K p, update
                 k_pcg_k3<T><<<g2, kBlock, 0, ctx->stream>>>(n, k, p0, z, partB, (int)g2, S, eps);
                 k_pcg_k1_h8<T><<<g1, kBlock, 0, ctx->stream>>>(g3(ctx), p0, q, Ev, Kv, ctx->d_fix, partA, S);
                 k_pcg_k2<T><<<g2, kBlock, 0, ctx->stream>>>(n, k, x, r, z, p0, q, D, partA, (int)g1, partB, S);
@@ -1213,7 +1265,8 @@ int pcg_mg_device(pf_ctx* ctx, const double* b, double* x, double eps, int itrma
     double* partB = ctx->red.partials + 2 * kMaxPartials;
     const bool h8 = !is2d(ctx);
     const K1Shape sh = h8 ? K1Shape{0, 0, 0u} : k1_shape(ctx->nx, ctx->ny);
-    const unsigned g1 = h8 ? gsize(ctx->nn) : sh.grid;
+    const H8Shape s8 = h8 ? h8_shape(ctx, sizeof(double)) : H8Shape{};
+    const unsigned g1 = h8 ? (s8.per ? s8.grid : gsize(ctx->nn)) : sh.grid;
     const unsigned g2 = std::min<unsigned>(gsize(n), (unsigned)k2_cap());
     PF_CHECK(g1 <= (unsigned)kMaxPartials, PF_ERR_ARG, "grid too large for the reduction scratch");
     double *r = ctx->d_r, *z = ctx->d_z, *p0 = ctx->d_p0, *p1 = ctx->d_p1, *q = ctx->d_q;
@@ -1256,7 +1309,10 @@ int pcg_mg_device(pf_ctx* ctx, const double* b, double* x, double eps, int itrma
         for (; k < kend; k++) {
             double* pold = (k & 1) ? p1 : p0;
             double* pnew = (k & 1) ? p0 : p1;
-            if (h8) {
+            if (h8 && s8.per) {
+                launch_h8_k1_tile<double>(ctx, s8, k, pold, z, pnew, q, ctx->d_E, ctx->d_K0g, partB, (int)gz, partA, S,
+                                          eps);
+            } else if (h8) {
                 // p = beta p + z in place (head and convergence test), q = K p
                 k_pcg_k3<double><<<g2, kBlock, 0, ctx->stream>>>(n, k, p0, z, partB, (int)g2, S, eps);
                 k_pcg_k1_h8<double><<<g1, kBlock, 0, ctx->stream>>>(g3(ctx), p0, q, ctx->d_E, ctx->d_K0, ctx->d_fix,
@@ -2235,6 +2291,8 @@ int pf_destroy(pf_ctx* ctx) {
     ctx->mma = nullptr;
     dfree(ctx->d_K0);
     dfree(ctx->d_K0f);
+    dfree(ctx->d_K0g);
+    dfree(ctx->d_K0gf);
     dfree(ctx->red.partials);
     dfree(ctx->red.counter);
     dfree(ctx->red.out);
@@ -2351,6 +2409,16 @@ int pf_set_element_matrix(pf_ctx* ctx, const double* K0, int nd) {
     PF_TRY(dalloc(ctx, &ctx->d_K0f, nd * nd));
     PF_HIP(hipMemcpyAsync(ctx->d_K0, K0, sizeof(double) * nd * nd, hipMemcpyHostToDevice, ctx->stream));
     PF_HIP(hipMemcpyAsync(ctx->d_K0f, k32.data(), sizeof(float) * nd * nd, hipMemcpyHostToDevice, ctx->stream));
+    if (nd == 24) {
+        std::vector<double> g64(nd * nd);
+        std::vector<float> g32(nd * nd);
+        h8_pack_k0(K0, g64.data());
+        h8_pack_k0(K0, g32.data());
+        PF_TRY(dalloc(ctx, &ctx->d_K0g, nd * nd));
+        PF_TRY(dalloc(ctx, &ctx->d_K0gf, nd * nd));
+        PF_HIP(hipMemcpyAsync(ctx->d_K0g, g64.data(), sizeof(double) * nd * nd, hipMemcpyHostToDevice, ctx->stream));
+        PF_HIP(hipMemcpyAsync(ctx->d_K0gf, g32.data(), sizeof(float) * nd * nd, hipMemcpyHostToDevice, ctx->stream));
+    }
     return sync(ctx);
 }
 

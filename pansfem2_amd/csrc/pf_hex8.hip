@@ -194,6 +194,251 @@ __global__ void __launch_bounds__(256) k_pcg_k1_h8(G3 g, const T* __restrict__ p
     store_partials<1>(acc, lds, partA);
 }
 
+// Hex8 K1 with K3's p update fused into it (one launch in place of k_pcg_k3 +
+// k_pcg_k1_h8): beta and the convergence test from K2's partials, p = beta
+// p_old + z, q = K p, the partial p.q.  A workgroup owns kH8Slab consecutive
+// nodes of a node plane (flattened index f = (nz+1) j + k within plane i, one
+// node per thread) for a chunk of planes i0 <= i < i1, and marches along i
+// with a ring of three node planes of p in LDS.  Each LDS plane spans
+// f in [s - H, s + kH8Slab + H), H = nz + 2, which holds every 3x3x3
+// neighbour of the slab's nodes, and is read from global memory as one
+// contiguous range of 3 (kH8Slab + 2H) values; the loads of plane i+2 are in
+// flight while plane i is computed.  p_old and p_new are distinct buffers
+// (other workgroups read p_old on their halos).  Moduli are read per node
+// from global memory (each is used by the 8 nodes of its element); K0g is
+// the element matrix packed by h8_pack_k0.
+constexpr int kH8Slab = 256;
+
+// The 64 (neighbour, element) pairs of a hex8 node in the order in which
+// k_pcg_k1_h8_tile consumes the element matrix: neighbour (a, b, c) = node
+// (i+a-1, j+b-1, k+c-1) in row-major order, then the node's elements
+// (ea, eb, ec) that hold it, in which it is local node ln = (a-ea, b-eb, c-ec)
+// and the node itself local node me = (1-ea, 1-eb, 1-ec).  neighbour(a, b, c)
+// opens each neighbour, group() each run of pairs with the same ea, and
+// pair(ea, eb, ec, me, ln, pos) visits a pair whose 3x3 block K0[3 me + r][3 ln + c]
+// sits at pos + 3 r + c of the packed matrix (h8_pack_k0).
+template <class N, class G, class F>
+__host__ __device__ __forceinline__ void h8_pairs(N&& neighbour, G&& group, F&& pair) {
+    int pos = 0;
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                neighbour(a, b, c);
+#pragma unroll
+                for (int ea = (a == 2 ? 1 : 0); ea <= (a == 0 ? 0 : 1); ea++) {
+                    group();
+#pragma unroll
+                    for (int eb = (b == 2 ? 1 : 0); eb <= (b == 0 ? 0 : 1); eb++)
+#pragma unroll
+                        for (int ec = (c == 2 ? 1 : 0); ec <= (c == 0 ? 0 : 1); ec++) {
+                            const int me = 4 * (1 - ec) + ((1 - eb) == 0 ? (1 - ea) : 2 + ea);
+                            const int ln = 4 * (c - ec) + ((b - eb) == 0 ? (a - ea) : 3 - (a - ea));
+                            pair(ea, eb, ec, me, ln, pos);
+                            pos += 9;
+                        }
+                }
+            }
+}
+
+// The 576 entries of a 24x24 hex8 element matrix in h8_pairs order.
+template <class T>
+inline void h8_pack_k0(const double* K0, T* out) {
+    h8_pairs([](int, int, int) {}, [] {},
+             [&](int, int, int, int me, int ln, int pos) {
+                 for (int r = 0; r < 3; r++)
+                     for (int c = 0; c < 3; c++) out[pos + 3 * r + c] = (T)K0[(3 * me + r) * 24 + 3 * ln + c];
+             });
+}
+
+// y = the node's three rows of K p (sums over its elements of E_e K0 p_e),
+// with K0 packed by h8_pack_k0.
+template <class T, class KP>
+__device__ inline void h8_node_rows(const T* __restrict__ P0, const T* __restrict__ P1, const T* __restrict__ P2, int L,
+                                    int l, int nz1, const T (&Ee)[2][2][2], KP K0, T (&y)[3]) {
+#pragma clang fp contract(fast)  // not a check-path kernel: FMA allowed
+    T acc[2][2][2][3];
+#pragma unroll
+    for (int ea = 0; ea < 2; ea++)
+#pragma unroll
+        for (int eb = 0; eb < 2; eb++)
+#pragma unroll
+            for (int ec = 0; ec < 2; ec++)
+#pragma unroll
+                for (int r = 0; r < 3; r++) acc[ea][eb][ec][r] = 0;
+    // The packed entries are wave-uniform scalar loads, contiguous within a
+    // group; all 576 would not fit the scalar registers.  Each group takes the
+    // pointer through an empty volatile asm, and each pair's three sums pass
+    // through another: volatile asm statements keep their order, so a group's
+    // loads follow the previous group's FMAs, and its FMAs follow its loads.
+    T u0 = 0, u1 = 0, u2 = 0;
+    KP Kg = K0;
+    h8_pairs(
+        [&](int a, int b, int c) {
+            const T* Pa = a == 0 ? P0 : (a == 1 ? P1 : P2);
+            const int idx = l + (b - 1) * nz1 + (c - 1);
+            u0 = Pa[idx];
+            u1 = Pa[L + idx];
+            u2 = Pa[2 * L + idx];
+        },
+        [&] {
+            Kg = K0;
+            asm volatile("" : "+s"(Kg));
+        },
+        [&](int ea, int eb, int ec, int, int, int pos) {
+            const auto k = Kg + pos;
+#pragma unroll
+            for (int r = 0; r < 3; r++) {
+                T v = acc[ea][eb][ec][r];
+                v = v + k[3 * r] * u0;
+                v = v + k[3 * r + 1] * u1;
+                v = v + k[3 * r + 2] * u2;
+                acc[ea][eb][ec][r] = v;
+            }
+            asm volatile("" ::"v"(acc[ea][eb][ec][0]), "v"(acc[ea][eb][ec][1]), "v"(acc[ea][eb][ec][2]));
+        });
+    y[0] = y[1] = y[2] = 0;
+#pragma unroll
+    for (int ea = 0; ea < 2; ea++)
+#pragma unroll
+        for (int eb = 0; eb < 2; eb++)
+#pragma unroll
+            for (int ec = 0; ec < 2; ec++)
+#pragma unroll
+                for (int r = 0; r < 3; r++) y[r] += Ee[ea][eb][ec] * acc[ea][eb][ec][r];
+}
+
+// PER = values of one LDS plane per thread: 3 (kH8Slab + 2 (nz + 2)) <= 256 PER.
+template <class T, int PER>
+__global__ void __launch_bounds__(256) k_pcg_k1_h8_tile(G3 g, int nslab, int planes, int nwork, int k,
+                                                        const T* __restrict__ pold, const T* __restrict__ z,
+                                                        T* __restrict__ pnew, T* __restrict__ q,
+                                                        const T* __restrict__ E, const T* __restrict__ K0g,
+                                                        const uint8_t* __restrict__ fix,
+                                                        const double* __restrict__ partB, int nbB,
+                                                        double* __restrict__ partA, PcgDev* __restrict__ S,
+                                                        double eps) {
+#pragma clang fp contract(fast)  // not a check-path kernel: FMA allowed
+    extern __shared__ double h8_smem[];
+    __shared__ double lds[32];
+    T* sp = reinterpret_cast<T*>(h8_smem);  // [3 planes][3 components][L]
+    const int nz1 = g.nz + 1, H = g.nz + 2, L = kH8Slab + 2 * H, NV = 3 * L;
+    const long PL = (long)(g.ny + 1) * nz1;
+    // work items of one XCD (workgroup b runs on XCD b mod 8) are contiguous,
+    // so that neighbouring chunks share their boundary planes in that XCD's L2
+    const int per = gridDim.x >> 3;
+    const int w = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    const bool has = w < nwork;
+    const int chunk = has ? w / nslab : 0, slab = has ? w - chunk * nslab : 0;
+    const int i0 = chunk * planes, i1 = has ? min(g.nx + 1, i0 + planes) : i0;
+    const long s = (long)slab * kH8Slab;
+    auto load_plane = [&](int ip, T (&pv)[PER], T (&zv)[PER]) {
+        const long base = 3 * (PL * ip + s - H);
+        const bool inplane = has && ip >= 0 && ip <= g.nx;
+#pragma unroll
+        for (int u = 0; u < PER; u++) {
+            const int h = threadIdx.x + 256 * u;
+            const long fl = s - H + h / 3;
+            pv[u] = 0;
+            zv[u] = 0;
+            if (inplane && h < NV && fl >= 0 && fl < PL) {
+                pv[u] = pold[base + h];
+                zv[u] = z[base + h];
+            }
+        }
+    };
+    const long f = s + threadIdx.x;
+    const bool own = has && f < PL;
+    const int j = own ? (int)(f / nz1) : 0, kk = own ? (int)(f - (long)j * nz1) : 0;
+    const int l = threadIdx.x + H;
+    // the moduli and fixity bits of plane i are loaded one plane ahead: vector
+    // loads complete in order, so loading them after plane i+2's prefetch
+    // would make the compute of plane i wait for that prefetch as well
+    T Ecur[2][2][2], Enext[2][2][2];
+    uint8_t fcur = 0, fnext = 0;
+    auto load_e = [&](int i, T (&Ev)[2][2][2], uint8_t& fx) {
+#pragma unroll
+        for (int ea = 0; ea < 2; ea++)
+#pragma unroll
+            for (int eb = 0; eb < 2; eb++)
+#pragma unroll
+                for (int ec = 0; ec < 2; ec++) {
+                    const int ei = i - 1 + ea, ej = j - 1 + eb, ek = kk - 1 + ec;
+                    const bool ok = own && ei >= 0 && ei < g.nx && ej >= 0 && ej < g.ny && ek >= 0 && ek < g.nz;
+                    Ev[ea][eb][ec] = ok ? E[g.elem(ei, ej, ek)] : (T)0;
+                }
+        fx = own ? fix[PL * i + f] : (uint8_t)0;
+    };
+    // the three planes around the first one, its moduli, the previous
+    // launch's partials and the done flag are all issued before the first wait
+    T pa[PER], za[PER], pb[PER], zb[PER], po[PER], zz[PER];
+    load_plane(i0 - 1, pa, za);
+    load_plane(i0, pb, zb);
+    load_plane(i0 + 1, po, zz);
+    load_e(i0, Ecur, fcur);
+    PartRegs<2, 8> pr;
+    load_partials(partB, nbB, pr);
+    if (S->done) return;
+    double beta;
+    bool stop;
+    {
+        double v[2];
+        sum_partials(pr, partB, nbB, lds, v);
+        pcg_head_sums(k, v, S, eps, beta, stop);
+    }
+    if (stop) return;
+    const T tb = (T)beta;
+    auto store_plane = [&](int ip, const T (&pv)[PER], const T (&zv)[PER]) {
+        T* dst = sp + ((ip + 3) % 3) * NV;
+#pragma unroll
+        for (int u = 0; u < PER; u++) {
+            const int h = threadIdx.x + 256 * u;
+            if (h < NV) {
+                const int nd = h / 3, c = h - 3 * nd;
+                dst[c * L + nd] = tb * pv[u] + zv[u];  // xeaxpy: p = beta p + z (CG.h:42-49)
+            }
+        }
+    };
+    store_plane(i0 - 1, pa, za);
+    store_plane(i0, pb, zb);
+    double acc[1] = {0.0};
+    for (int i = i0; i < i1; i++) {
+        __syncthreads();  // plane i-1's compute (which read the slot of plane i+1) is done
+        store_plane(i + 1, po, zz);
+        __syncthreads();
+        if (i + 2 <= i1) load_plane(i + 2, po, zz);  // in flight during the compute
+        if (i + 1 < i1) load_e(i + 1, Enext, fnext);
+        if (own) {
+            const T* P0 = sp + ((i + 2) % 3) * NV;
+            const T* P1 = sp + (i % 3) * NV;
+            const T* P2 = sp + ((i + 1) % 3) * NV;
+            // the packed element matrix is never written: the constant address
+            // space keeps its uniform loads scalar
+            T y[3];
+            h8_node_rows(P0, P1, P2, L, l, nz1, Ecur, (const __attribute__((address_space(4))) T*)K0g, y);
+            const long n = PL * i + f;
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                const T o = (fcur & (1u << d)) ? (T)0 : y[d];
+                const T pv = P1[d * L + l];
+                pnew[3 * n + d] = pv;
+                q[3 * n + d] = o;
+                acc[0] += (double)pv * (double)o;
+            }
+        }
+#pragma unroll
+        for (int ea = 0; ea < 2; ea++)
+#pragma unroll
+            for (int eb = 0; eb < 2; eb++)
+#pragma unroll
+                for (int ec = 0; ec < 2; ec++) Ecur[ea][eb][ec] = Enext[ea][eb][ec];
+        fcur = fnext;
+    }
+    store_partials<1>(acc, lds, partA);
+}
+
 // ---------------------------------------------------- filter / compliance --
 __global__ void k_centroids_h8(G3 g, double lx, double ly, double lz, double* __restrict__ cg) {
     const long ne = g.nelem();

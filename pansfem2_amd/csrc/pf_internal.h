@@ -63,6 +63,8 @@ struct pf_ctx {
     std::vector<double> h_K0;
     double* d_K0 = nullptr;
     float* d_K0f = nullptr;
+    double* d_K0g = nullptr;  // hex8: K0 packed in the operator kernel's order (h8_pack_k0)
+    float* d_K0gf = nullptr;
 
     // material / design
     double E1 = 1.0, E0 = 1e-9, p = 3.0, nu = 0.3, t = 1.0;

@@ -96,6 +96,32 @@ def test_hex8_csr_bit_exact_and_operators(pa):
     assert rel(ctx.apply_K(x, pa.PF_OP_EBE32), y_ref) < 1e-5
 
 
+@pytest.mark.parametrize("nx,ny,nz", [(9, 5, 3), (13, 20, 20), (6, 3, 90), (4, 2, 230)])
+def test_hex8_pcg_plane_kernel_shapes(pa, nx, ny, nz):
+    """The matrix-free hex8 PCG (one operator launch per iteration that marches
+    over chunks of node planes, 256-node slabs of a plane per workgroup) against
+    the PCG over the assembled CSR matrix: partial plane chunks (9x5x3),
+    several slabs per plane (13x20x20), long planes that take the kernel's
+    wider register set (nz = 90), and planes too long for it (nz = 230, the
+    three-launch form).  FP64 to 1e-10, and the FP32 inner solve of the mixed
+    solver to a 1e-8 true residual."""
+    print(f"hex8 pcg {nx}x{ny}x{nz}", flush=True)
+    ctx = pa.cantilever3d(nx, ny, nz)
+    rng = np.random.default_rng(nx * 10000 + ny * 100 + nz)
+    ctx.set_density(rng.uniform(1e-2, 1.0, nx * ny * nz))
+    ctx.assemble_csr()
+    b = ctx.load()
+    xc, sc = ctx.pcg(b, op=pa.PF_OP_CSR64, eps=1e-10)
+    xe, se = ctx.pcg(b, op=pa.PF_OP_EBE64, eps=1e-10)
+    assert sc["status"] == 1 and se["status"] == 1
+    assert rel(xe, xc) < 1e-8, rel(xe, xc)
+    assert abs(se["iterations"] - sc["iterations"]) <= max(2, sc["iterations"] // 100)
+    x32, s32 = ctx.pcg(b, op=pa.PF_OP_EBE32, eps=1e-8)
+    assert s32["status"] == 1 and s32["true_residual"] <= 1.0000001e-8
+    assert rel(x32, xc) < 1e-6
+    ctx.close()
+
+
 def test_hex8_mixed_precision_solve(pa):
     """FP32 matrix-free hex8 inner PCG inside FP64 iterative refinement: the
     solve ends at a 1e-8 FP64 true residual and agrees with the FP64
