@@ -428,13 +428,15 @@ __global__ void __launch_bounds__(256) k_pcg_k1_h8_tile(G3 g, int nslab, int pla
                 acc[0] += (double)pv * (double)o;
             }
         }
+        if (i + 1 < i1) {
 #pragma unroll
-        for (int ea = 0; ea < 2; ea++)
+            for (int ea = 0; ea < 2; ea++)
 #pragma unroll
-            for (int eb = 0; eb < 2; eb++)
+                for (int eb = 0; eb < 2; eb++)
 #pragma unroll
-                for (int ec = 0; ec < 2; ec++) Ecur[ea][eb][ec] = Enext[ea][eb][ec];
-        fcur = fnext;
+                    for (int ec = 0; ec < 2; ec++) Ecur[ea][eb][ec] = Enext[ea][eb][ec];
+            fcur = fnext;
+        }
     }
     store_partials<1>(acc, lds, partA);
 }
