@@ -254,8 +254,9 @@ inline void h8_pack_k0(const double* K0, T* out) {
 }
 
 // y = the node's three rows of K p (sums over its elements of E_e K0 p_e),
-// with K0 packed by h8_pack_k0.
-template <class T, class KP>
+// with K0 packed by h8_pack_k0.  SCALAR: K0 is read with wave-uniform scalar
+// loads (grouped as described below); else from LDS.
+template <bool SCALAR, class T, class KP>
 __device__ inline void h8_node_rows(const T* __restrict__ P0, const T* __restrict__ P1, const T* __restrict__ P2, int L,
                                     int l, int nz1, const T (&Ee)[2][2][2], KP K0, T (&y)[3]) {
 #pragma clang fp contract(fast)  // not a check-path kernel: FMA allowed
@@ -285,7 +286,7 @@ __device__ inline void h8_node_rows(const T* __restrict__ P0, const T* __restric
         },
         [&] {
             Kg = K0;
-            asm volatile("" : "+s"(Kg));
+            if constexpr (SCALAR) asm volatile("" : "+s"(Kg));
         },
         [&](int ea, int eb, int ec, int, int, int pos) {
             const auto k = Kg + pos;
@@ -297,7 +298,8 @@ __device__ inline void h8_node_rows(const T* __restrict__ P0, const T* __restric
                 v = v + k[3 * r + 2] * u2;
                 acc[ea][eb][ec][r] = v;
             }
-            asm volatile("" ::"v"(acc[ea][eb][ec][0]), "v"(acc[ea][eb][ec][1]), "v"(acc[ea][eb][ec][2]));
+            if constexpr (SCALAR)
+                asm volatile("" ::"v"(acc[ea][eb][ec][0]), "v"(acc[ea][eb][ec][1]), "v"(acc[ea][eb][ec][2]));
         });
     y[0] = y[1] = y[2] = 0;
 #pragma unroll
@@ -324,6 +326,12 @@ __global__ void __launch_bounds__(256) k_pcg_k1_h8_tile(G3 g, int nslab, int pla
     extern __shared__ double h8_smem[];
     __shared__ double lds[32];
     T* sp = reinterpret_cast<T*>(h8_smem);  // [3 planes][3 components][L]
+    // FP32: the packed element matrix is staged in LDS (measured faster than
+    // scalar loads there; in FP64 the scalar loads are faster)
+    constexpr bool kScalarK0 = sizeof(T) == 8;
+    __shared__ T sK[kScalarK0 ? 1 : 576];
+    if constexpr (!kScalarK0)
+        for (int t = threadIdx.x; t < 576; t += 256) sK[t] = K0g[t];  // read after the loop's first barrier
     const int nz1 = g.nz + 1, H = g.nz + 2, L = kH8Slab + 2 * H, NV = 3 * L;
     const long PL = (long)(g.ny + 1) * nz1;
     // work items of one XCD (workgroup b runs on XCD b mod 8) are contiguous,
@@ -417,7 +425,10 @@ __global__ void __launch_bounds__(256) k_pcg_k1_h8_tile(G3 g, int nslab, int pla
             // the packed element matrix is never written: the constant address
             // space keeps its uniform loads scalar
             T y[3];
-            h8_node_rows(P0, P1, P2, L, l, nz1, Ecur, (const __attribute__((address_space(4))) T*)K0g, y);
+            if constexpr (kScalarK0)
+                h8_node_rows<true>(P0, P1, P2, L, l, nz1, Ecur, (const __attribute__((address_space(4))) T*)K0g, y);
+            else
+                h8_node_rows<false>(P0, P1, P2, L, l, nz1, Ecur, (const T*)sK, y);
             const long n = PL * i + f;
 #pragma unroll
             for (int d = 0; d < 3; d++) {
