@@ -204,9 +204,10 @@ __global__ void __launch_bounds__(256) k_pcg_k1_h8(G3 g, const T* __restrict__ p
 // neighbour of the slab's nodes, and is read from global memory as one
 // contiguous range of 3 (kH8Slab + 2H) values; the loads of plane i+2 are in
 // flight while plane i is computed.  p_old and p_new are distinct buffers
-// (other workgroups read p_old on their halos).  Moduli are read per node
-// from global memory (each is used by the 8 nodes of its element); K0g is
-// the element matrix packed by h8_pack_k0.
+// (other workgroups read p_old on their halos).  Moduli and fixity bits are
+// read per node from global memory (a modulus is used by the 8 nodes of its
+// element), one plane ahead.  K0g is the element matrix packed by
+// h8_pack_k0, read by scalar loads in FP64 and staged in LDS in FP32.
 constexpr int kH8Slab = 256;
 
 // The 64 (neighbour, element) pairs of a hex8 node in the order in which
@@ -269,8 +270,8 @@ __device__ inline void h8_node_rows(const T* __restrict__ P0, const T* __restric
             for (int ec = 0; ec < 2; ec++)
 #pragma unroll
                 for (int r = 0; r < 3; r++) acc[ea][eb][ec][r] = 0;
-    // The packed entries are wave-uniform scalar loads, contiguous within a
-    // group; all 576 would not fit the scalar registers.  Each group takes the
+    // SCALAR: the packed entries are wave-uniform scalar loads, contiguous
+    // within a group; all 576 would not fit the scalar registers.  Each group takes the
     // pointer through an empty volatile asm, and each pair's three sums pass
     // through another: volatile asm statements keep their order, so a group's
     // loads follow the previous group's FMAs, and its FMAs follow its loads.
