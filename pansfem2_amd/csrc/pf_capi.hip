@@ -152,6 +152,11 @@ bool is2d(pf_ctx* ctx) { return ctx->dim == 2; }
 void release_net(pf_ctx* ctx);
 G3 g3(pf_ctx* ctx) { return G3{ctx->nx, ctx->ny, ctx->nz}; }
 
+// the hex8 multigrid's finest level runs k_mg3_residual0_tile on this shape
+struct Mg3KeTile {
+    const H8Shape* sh;
+};
+
 template <class T>
 void launch_h8_k1_tile(pf_ctx* ctx, const H8Shape& sh, int k, const T* pold, const T* z, T* pnew, T* q, const T* Ev,
                        const T* Kg, const double* partB, int nbB, double* partA, PcgDev* S, double eps) {
@@ -1184,14 +1189,29 @@ float* mg_cycle(pf_ctx* ctx, int l) {
         float *x = a.x, *res = a.res, *xo = a.x2;
         auto run3 = [&](auto ke) {
             const unsigned gv = gsize(a.nv), gn = gsize(a.nn);
+            // res = r - A xin, or with xout, xout = xin + omega D^-1 (r - A xin)
+            auto residual = [&](const float* xin, float* rout, float* xout) {
+                if constexpr (std::is_same<decltype(ke), Mg3KeTile>::value) {
+                    const H8Shape& t = *ke.sh;
+                    if (t.per == 5)
+                        k_mg3_residual0_tile<5><<<t.grid, kBlock, t.lds, ctx->stream>>>(
+                            g3(ctx), t.nslab, t.planes, t.nwork, ctx->d_Ef, ctx->d_K0gf, a.D, fx3, S->omega, a.r, xin,
+                            rout, xout, S->done);
+                    else
+                        k_mg3_residual0_tile<8><<<t.grid, kBlock, t.lds, ctx->stream>>>(
+                            g3(ctx), t.nslab, t.planes, t.nwork, ctx->d_Ef, ctx->d_K0gf, a.D, fx3, S->omega, a.r, xin,
+                            rout, xout, S->done);
+                } else {
+                    k_mg3_residual<<<gn, kBlock, 0, ctx->stream>>>(a.nx, a.ny, a.nz, ke, a.D, fx3, S->omega, a.r, xin,
+                                                                   rout, xout, S->done);
+                }
+            };
             k_mg3_update<<<gv, kBlock, 0, ctx->stream>>>(a.nv, S->omega, nullptr, nullptr, a.r, a.D, fx3, x, S->done);
-            k_mg3_residual<<<gn, kBlock, 0, ctx->stream>>>(a.nx, a.ny, a.nz, ke, a.D, fx3, S->omega, a.r, x, res,
-                                                           nullptr, S->done);
+            residual(x, res, nullptr);
             for (int k = 1; k < S->nu; k++) {
                 k_mg3_update<<<gv, kBlock, 0, ctx->stream>>>(a.nv, S->omega, x, res, a.r, a.D, fx3, xo, S->done);
                 std::swap(x, xo);
-                k_mg3_residual<<<gn, kBlock, 0, ctx->stream>>>(a.nx, a.ny, a.nz, ke, a.D, fx3, S->omega, a.r, x, res,
-                                                               nullptr, S->done);
+                residual(x, res, nullptr);
             }
             MgLevel& b = S->L[l + 1];
             k_mg3_restrict<<<gsize(b.nn), kBlock, 0, ctx->stream>>>(b.nx, b.ny, b.nz, res, b.fix, b.r, S->done);
@@ -1199,15 +1219,19 @@ float* mg_cycle(pf_ctx* ctx, int l) {
             k_mg3_prolong<<<gn, kBlock, 0, ctx->stream>>>(a.nx, a.ny, a.nz, x, xc, fx3, xo, S->done);
             std::swap(x, xo);
             for (int k = 0; k < S->nu; k++) {
-                k_mg3_residual<<<gn, kBlock, 0, ctx->stream>>>(a.nx, a.ny, a.nz, ke, a.D, fx3, S->omega, a.r, x,
-                                                               nullptr, xo, S->done);
+                residual(x, nullptr, xo);
                 std::swap(x, xo);
             }
         };
-        if (l == 0)
-            run3(Mg3KeShared{ctx->d_Ef, ctx->d_K0f});
-        else
+        if (l == 0) {
+            const H8Shape s8 = h8_shape(ctx, sizeof(float));
+            if (s8.per)
+                run3(Mg3KeTile{&s8});
+            else
+                run3(Mg3KeShared{ctx->d_Ef, ctx->d_K0f});
+        } else {
             run3(Mg3KeFull{a.Ke32});
+        }
         return x;
     }
     const uint8_t* fx = l == 0 ? ctx->d_fix : a.fix;

@@ -255,9 +255,11 @@ inline void h8_pack_k0(const double* K0, T* out) {
 }
 
 // y = the node's three rows of K p (sums over its elements of E_e K0 p_e),
-// with K0 packed by h8_pack_k0.  SCALAR: K0 is read with wave-uniform scalar
-// loads (grouped as described below); else from LDS.
-template <bool SCALAR, class T, class KP>
+// with K0 packed by h8_pack_k0.  kK0Scalar: K0 is read with wave-uniform
+// scalar loads, in groups as described below; kK0LdsGrouped: from LDS, in the
+// same groups; kK0Lds: from LDS, scheduled freely.
+enum { kK0Scalar, kK0LdsGrouped, kK0Lds };
+template <int MODE, class T, class KP>
 __device__ inline void h8_node_rows(const T* __restrict__ P0, const T* __restrict__ P1, const T* __restrict__ P2, int L,
                                     int l, int nz1, const T (&Ee)[2][2][2], KP K0, T (&y)[3]) {
 #pragma clang fp contract(fast)  // not a check-path kernel: FMA allowed
@@ -270,8 +272,8 @@ __device__ inline void h8_node_rows(const T* __restrict__ P0, const T* __restric
             for (int ec = 0; ec < 2; ec++)
 #pragma unroll
                 for (int r = 0; r < 3; r++) acc[ea][eb][ec][r] = 0;
-    // SCALAR: the packed entries are wave-uniform scalar loads, contiguous
-    // within a group; all 576 would not fit the scalar registers.  Each group takes the
+    // Grouped: the packed entries are wave-uniform loads, contiguous within a
+    // group; all 576 would not fit the registers.  Each group takes the
     // pointer through an empty volatile asm, and each pair's three sums pass
     // through another: volatile asm statements keep their order, so a group's
     // loads follow the previous group's FMAs, and its FMAs follow its loads.
@@ -287,7 +289,8 @@ __device__ inline void h8_node_rows(const T* __restrict__ P0, const T* __restric
         },
         [&] {
             Kg = K0;
-            if constexpr (SCALAR) asm volatile("" : "+s"(Kg));
+            if constexpr (MODE == kK0Scalar) asm volatile("" : "+s"(Kg));
+            if constexpr (MODE == kK0LdsGrouped) asm volatile("" : "+v"(Kg));
         },
         [&](int ea, int eb, int ec, int, int, int pos) {
             const auto k = Kg + pos;
@@ -299,7 +302,7 @@ __device__ inline void h8_node_rows(const T* __restrict__ P0, const T* __restric
                 v = v + k[3 * r + 2] * u2;
                 acc[ea][eb][ec][r] = v;
             }
-            if constexpr (SCALAR)
+            if constexpr (MODE != kK0Lds)
                 asm volatile("" ::"v"(acc[ea][eb][ec][0]), "v"(acc[ea][eb][ec][1]), "v"(acc[ea][eb][ec][2]));
         });
     y[0] = y[1] = y[2] = 0;
@@ -427,9 +430,10 @@ __global__ void __launch_bounds__(256) k_pcg_k1_h8_tile(G3 g, int nslab, int pla
             // space keeps its uniform loads scalar
             T y[3];
             if constexpr (kScalarK0)
-                h8_node_rows<true>(P0, P1, P2, L, l, nz1, Ecur, (const __attribute__((address_space(4))) T*)K0g, y);
+                h8_node_rows<kK0Scalar>(P0, P1, P2, L, l, nz1, Ecur, (const __attribute__((address_space(4))) T*)K0g,
+                                        y);
             else
-                h8_node_rows<false>(P0, P1, P2, L, l, nz1, Ecur, (const T*)sK, y);
+                h8_node_rows<kK0Lds>(P0, P1, P2, L, l, nz1, Ecur, (const T*)sK, y);
             const long n = PL * i + f;
 #pragma unroll
             for (int d = 0; d < 3; d++) {
@@ -451,6 +455,118 @@ __global__ void __launch_bounds__(256) k_pcg_k1_h8_tile(G3 g, int nslab, int pla
         }
     }
     store_partials<1>(acc, lds, partA);
+}
+
+// The finest level's smoothing step of the hex8 multigrid V-cycle on the
+// plane ring of k_pcg_k1_h8_tile (same slabs, chunks and grid): v = r - A x
+// at each node, A the fine operator (FP32 moduli times the packed element
+// matrix, staged in LDS), masked at fixed DOFs; res = v, or with x_out,
+// x_out = x + omega v / D (the damped Jacobi sweep).
+template <int PER>
+__global__ void __launch_bounds__(256) k_mg3_residual0_tile(G3 g, int nslab, int planes, int nwork,
+                                                            const float* __restrict__ E,
+                                                            const float* __restrict__ K0g,
+                                                            const float* __restrict__ D,
+                                                            const uint8_t* __restrict__ fix, float omega,
+                                                            const float* __restrict__ r, const float* __restrict__ x,
+                                                            float* __restrict__ res, float* __restrict__ x_out,
+                                                            const int* __restrict__ done) {
+#pragma clang fp contract(fast)  // preconditioner arithmetic: FMA allowed
+    if (done && *done) return;
+    extern __shared__ double h8_smem[];
+    float* sp = reinterpret_cast<float*>(h8_smem);  // [3 planes][3 components][L]
+    __shared__ __align__(16) float sK[576];
+    for (int t = threadIdx.x; t < 576; t += 256) sK[t] = K0g[t];  // read after the loop's first barrier
+    const int nz1 = g.nz + 1, H = g.nz + 2, L = kH8Slab + 2 * H, NV = 3 * L;
+    const long PL = (long)(g.ny + 1) * nz1;
+    const int per = gridDim.x >> 3;
+    const int w = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    const bool has = w < nwork;
+    const int chunk = has ? w / nslab : 0, slab = has ? w - chunk * nslab : 0;
+    const int i0 = chunk * planes, i1 = has ? min(g.nx + 1, i0 + planes) : i0;
+    const long s = (long)slab * kH8Slab;
+    auto load_plane = [&](int ip, float (&xv)[PER]) {
+        const long base = 3 * (PL * ip + s - H);
+        const bool inplane = has && ip >= 0 && ip <= g.nx;
+#pragma unroll
+        for (int u = 0; u < PER; u++) {
+            const int h = threadIdx.x + 256 * u;
+            const long fl = s - H + h / 3;
+            xv[u] = (inplane && h < NV && fl >= 0 && fl < PL) ? x[base + h] : 0.f;
+        }
+    };
+    auto store_plane = [&](int ip, const float (&xv)[PER]) {
+        float* dst = sp + ((ip + 3) % 3) * NV;
+#pragma unroll
+        for (int u = 0; u < PER; u++) {
+            const int h = threadIdx.x + 256 * u;
+            if (h < NV) {
+                const int nd = h / 3, c = h - 3 * nd;
+                dst[c * L + nd] = xv[u];
+            }
+        }
+    };
+    const long f = s + threadIdx.x;
+    const bool own = has && f < PL;
+    const int j = own ? (int)(f / nz1) : 0, kk = own ? (int)(f - (long)j * nz1) : 0;
+    const int l = threadIdx.x + H;
+    float Ecur[2][2][2], Enext[2][2][2];
+    uint8_t fcur = 0, fnext = 0;
+    auto load_e = [&](int i, float (&Ev)[2][2][2], uint8_t& fx) {
+#pragma unroll
+        for (int ea = 0; ea < 2; ea++)
+#pragma unroll
+            for (int eb = 0; eb < 2; eb++)
+#pragma unroll
+                for (int ec = 0; ec < 2; ec++) {
+                    const int ei = i - 1 + ea, ej = j - 1 + eb, ek = kk - 1 + ec;
+                    const bool ok = own && ei >= 0 && ei < g.nx && ej >= 0 && ej < g.ny && ek >= 0 && ek < g.nz;
+                    Ev[ea][eb][ec] = ok ? E[g.elem(ei, ej, ek)] : 0.f;
+                }
+        fx = own ? fix[PL * i + f] : (uint8_t)0;
+    };
+    float xa[PER], xb[PER], xc[PER];
+    load_plane(i0 - 1, xa);
+    load_plane(i0, xb);
+    load_plane(i0 + 1, xc);
+    load_e(i0, Ecur, fcur);
+    store_plane(i0 - 1, xa);
+    store_plane(i0, xb);
+    for (int i = i0; i < i1; i++) {
+        __syncthreads();  // plane i-1's compute (which read the slot of plane i+1) is done
+        store_plane(i + 1, xc);
+        __syncthreads();
+        if (i + 2 <= i1) load_plane(i + 2, xc);  // in flight during the compute
+        if (i + 1 < i1) load_e(i + 1, Enext, fnext);
+        if (own) {
+            const float* P0 = sp + ((i + 2) % 3) * NV;
+            const float* P1 = sp + (i % 3) * NV;
+            const float* P2 = sp + ((i + 1) % 3) * NV;
+            float y[3];
+            h8_node_rows<kK0LdsGrouped>(P0, P1, P2, L, l, nz1, Ecur,
+                                        (const __attribute__((address_space(3))) float*)sK, y);
+            const long n = PL * i + f;
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                float v = r[3 * n + d] - y[d];
+                if (x_out) v = P1[d * L + l] + omega * v / D[3 * n + d];
+                if ((fcur >> d) & 1) v = 0.f;
+                if (x_out)
+                    x_out[3 * n + d] = v;
+                else
+                    res[3 * n + d] = v;
+            }
+        }
+        if (i + 1 < i1) {
+#pragma unroll
+            for (int ea = 0; ea < 2; ea++)
+#pragma unroll
+                for (int eb = 0; eb < 2; eb++)
+#pragma unroll
+                    for (int ec = 0; ec < 2; ec++) Ecur[ea][eb][ec] = Enext[ea][eb][ec];
+            fcur = fnext;
+        }
+    }
 }
 
 // ---------------------------------------------------- filter / compliance --
